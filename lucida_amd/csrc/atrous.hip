@@ -1,0 +1,178 @@
+// Edge-avoiding a-trous denoiser for gfx950.
+//
+// Reference: shaders/denoise/atrous.wgsl:73-198 with host driver
+// crates/gpu/src/denoise.rs:301-408; CPU twin crates/core/src/denoise.rs:134.
+// Same algorithm: demodulate by albedo (floored at 1e-3), a 3x3 luminance
+// deviation measured once, `iterations` passes of the 5x5 B-spline kernel at
+// stride 2^level weighted by normal, depth, luminance and albedo stops, then
+// remodulate. Changes for the GPU: the guides are packed once into two
+// 16-byte planes (normal+depth, albedo+luminance) so each tap reads 48 bytes
+// instead of a 64-byte Accum plus the colour, the colour plane carries its
+// luminance in .w, the deviation is fused into the demodulate pass's
+// successor, and the remodulation is fused into the last filter pass.
+#include <hip/hip_runtime.h>
+
+#include "context.hpp"
+#include "shading.hpp"
+
+namespace lucida {
+namespace {
+
+using namespace dev;
+
+constexpr int TX = 16, TY = 8;
+
+__device__ inline float lum3(float x, float y, float z) { return x * 0.2126f + y * 0.7152f + z * 0.0722f; }
+
+// Means of the accumulated channels, and the demodulated colour.
+__global__ __launch_bounds__(TX* TY) void k_dn_prepare(const GpuAccum* __restrict__ in, uint32_t w, uint32_t h,
+                                                       float4* __restrict__ color, float4* __restrict__ gnd,
+                                                       float4* __restrict__ galb) {
+    const uint32_t x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const uint32_t i = y * w + x;
+    const float4* a = reinterpret_cast<const float4*>(in + i);
+    const float4 a0 = a[0], a1 = a[1], a2 = a[2];
+    const float inv = 1.0f / fmaxf(a0.w, 1.0f);
+    const V3 alb = vmax(V3{a1.x * inv, a1.y * inv, a1.z * inv}, v3s(1.0e-3f));
+    const V3 c = V3{a0.x * inv, a0.y * inv, a0.z * inv} / alb;
+    color[i] = make_float4(c.x, c.y, c.z, luminance(c));
+    gnd[i] = make_float4(a2.x * inv, a2.y * inv, a2.z * inv, a1.w * inv);
+    galb[i] = make_float4(alb.x, alb.y, alb.z, luminance(alb));
+}
+
+// denoise.rs:111 local_deviation: 3x3 luminance standard deviation.
+__global__ __launch_bounds__(TX* TY) void k_dn_deviation(const float4* __restrict__ color, uint32_t w, uint32_t h,
+                                                         float* __restrict__ dev) {
+    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    if (x >= (int)w || y >= (int)h) return;
+    float sum = 0.0f, sum_sq = 0.0f, n = 0.0f;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int px = x + dx, py = y + dy;
+            if (px < 0 || py < 0 || px >= (int)w || py >= (int)h) continue;
+            const float l = color[py * w + px].w;
+            sum += l;
+            sum_sq += l * l;
+            n += 1.0f;
+        }
+    const float mean = sum / n;
+    dev[y * w + x] = sqrtf(fmaxf(sum_sq / n - mean * mean, 0.0f));
+}
+
+// One a-trous pass (atrous.wgsl:129). When `out` is non-null this is the last
+// pass and it writes the remodulated Accum-shaped result instead.
+__global__ __launch_bounds__(TX* TY) void k_dn_pass(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                    const float4* __restrict__ gnd, const float4* __restrict__ galb,
+                                                    const float* __restrict__ dev, uint32_t w, uint32_t h,
+                                                    int stride, lucida_denoise_params prm, GpuAccum* __restrict__ out) {
+    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    if (x >= (int)w || y >= (int)h) return;
+    const float K[5] = {1.0f / 16.0f, 1.0f / 4.0f, 3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};
+    const uint32_t ci = y * w + x;
+    const float4 cg = gnd[ci];
+    const float4 cc = src[ci];
+    const float ca = fmaxf(galb[ci].w, 1.0e-3f);
+    const float lum_scale = prm.sigma_luminance * fmaxf(dev[ci], 1.0e-4f);
+    const float sd = prm.sigma_depth * fmaxf(fabsf(cg.w), 1.0e-3f);
+    const float sa = prm.sigma_albedo * ca;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, ws = 0.0f;
+    for (int ky = 0; ky < 5; ++ky) {
+        const int py = y + (ky - 2) * stride;
+        if (py < 0 || py >= (int)h) continue;
+        for (int kx = 0; kx < 5; ++kx) {
+            const int px = x + (kx - 2) * stride;
+            if (px < 0 || px >= (int)w) continue;
+            const uint32_t qi = py * w + px;
+            const float4 qg = gnd[qi];
+            const float4 qc = src[qi];
+            const float qa = galb[qi].w;
+            const float nd = fmaxf(cg.x * qg.x + cg.y * qg.y + cg.z * qg.z, 0.0f);
+            const float w_n = powf(nd, prm.sigma_normal);
+            const float w_d = expf(-fabsf(cg.w - qg.w) / sd);
+            const float w_l = expf(-fabsf(cc.w - qc.w) / lum_scale);
+            const float w_a = expf(-fabsf(ca - qa) / sa);
+            const float wt = K[kx] * K[ky] * w_n * w_d * w_l * w_a;
+            sx += qc.x * wt;
+            sy += qc.y * wt;
+            sz += qc.z * wt;
+            ws += wt;
+        }
+    }
+    float4 r = cc;
+    if (ws > 0.0f) {
+        r.x = sx / ws;
+        r.y = sy / ws;
+        r.z = sz / ws;
+        r.w = lum3(r.x, r.y, r.z);
+    }
+    if (!out) {
+        dst[ci] = r;
+        return;
+    }
+    const float4 al = galb[ci];
+    float4* o = reinterpret_cast<float4*>(out + ci);
+    o[0] = make_float4(r.x * al.x, r.y * al.y, r.z * al.z, 1.0f);
+    o[1] = make_float4(al.x, al.y, al.z, cg.w);
+    o[2] = make_float4(cg.x, cg.y, cg.z, 0.0f);
+    o[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Zero iterations: remodulate the demodulated colour unchanged.
+__global__ __launch_bounds__(TX* TY) void k_dn_modulate(const float4* __restrict__ src, const float4* __restrict__ gnd,
+                                                        const float4* __restrict__ galb, uint32_t w, uint32_t h,
+                                                        GpuAccum* __restrict__ out) {
+    const uint32_t x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const uint32_t ci = y * w + x;
+    const float4 r = src[ci], al = galb[ci], cg = gnd[ci];
+    float4* o = reinterpret_cast<float4*>(out + ci);
+    o[0] = make_float4(r.x * al.x, r.y * al.y, r.z * al.z, 1.0f);
+    o[1] = make_float4(al.x, al.y, al.z, cg.w);
+    o[2] = make_float4(cg.x, cg.y, cg.z, 0.0f);
+    o[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+}  // namespace
+}  // namespace lucida
+
+using namespace lucida;
+
+extern "C" {
+
+lucida_status lucida_denoise_atrous(lucida_ctx* ctx, const lucida_film* in, const lucida_denoise_params* params,
+                                    lucida_film* out) {
+    if (!ctx || !in || !params || !out) return set_error(LUCIDA_INVALID_ARG, "lucida_denoise_atrous: null argument");
+    if (in == out) return set_error(LUCIDA_INVALID_ARG, "lucida_denoise_atrous: input and output must differ");
+    if (in->width != out->width || in->height != out->height)
+        return set_error(LUCIDA_INVALID_ARG, "lucida_denoise_atrous: film sizes differ");
+    if (params->iterations > 16) return set_error(LUCIDA_INVALID_ARG, "lucida_denoise_atrous: at most 16 iterations");
+    clear_error();
+    const uint32_t w = in->width, h = in->height;
+    const size_t n = (size_t)w * h;
+    LUCIDA_HIP(ctx->dn_buf.ensure(n * (16 * 4 + 4)), LUCIDA_OOM);
+    float4* a = ctx->dn_buf.as<float4>();
+    float4* b = a + n;
+    float4* gnd = a + 2 * n;
+    float4* galb = a + 3 * n;
+    float* dev = reinterpret_cast<float*>(a + 4 * n);
+    const dim3 blk(TX, TY), grd((w + TX - 1) / TX, (h + TY - 1) / TY);
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_dn_prepare, grd, blk, 0, s, in->accum, w, h, a, gnd, galb);
+    if (params->iterations == 0) {
+        hipLaunchKernelGGL(k_dn_modulate, grd, blk, 0, s, a, gnd, galb, w, h, out->accum);
+    } else {
+        hipLaunchKernelGGL(k_dn_deviation, grd, blk, 0, s, a, w, h, dev);
+        for (uint32_t level = 0; level < params->iterations; ++level) {
+            const bool last = level + 1 == params->iterations;
+            hipLaunchKernelGGL(k_dn_pass, grd, blk, 0, s, a, b, gnd, galb, dev, w, h, 1 << level, *params,
+                               last ? out->accum : nullptr);
+            std::swap(a, b);
+        }
+    }
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
+    return LUCIDA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,311 @@
+// Context, device memory, scene residency and films behind the C ABI.
+//
+// Stand-ins for the reference's harness plumbing: Gpu::new / check_errors
+// (crates/gpu/src/lib.rs:144-215) and the per-render buffer uploads of
+// wavefront::render_native_on_inner (crates/gpu/src/wavefront.rs:215-296).
+// Unlike the reference, a scene is uploaded once and stays resident.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "context.hpp"
+
+namespace lucida {
+
+static thread_local std::string g_last_error;
+
+lucida_status set_error(lucida_status code, const char* msg) {
+    g_last_error = msg ? msg : "";
+    return code;
+}
+void clear_error() { g_last_error.clear(); }
+
+lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
+                                GpuBvhNode* nodes, size_t* num_nodes, uint32_t* prim_indices,
+                                GpuTriangle* tris_sorted, float* stage_us);
+
+template <class T>
+static lucida_status upload(lucida_ctx* ctx, T** dst, const T* src, size_t count, size_t min_count = 1) {
+    size_t bytes = (count > min_count ? count : min_count) * sizeof(T);
+    LUCIDA_HIP(hipMalloc(reinterpret_cast<void**>(dst), bytes), LUCIDA_OOM);
+    LUCIDA_HIP(hipMemsetAsync(*dst, 0, bytes, ctx->stream), LUCIDA_VALIDATION);
+    if (count && src)
+        LUCIDA_HIP(hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream), LUCIDA_VALIDATION);
+    return LUCIDA_OK;
+}
+
+static void scene_free(lucida_scene* s) {
+    void* ptrs[] = {s->materials, s->primitives, s->positions, s->vertex_attrs, s->triangles, s->bvh_nodes,
+                    s->lights,    s->env_pixels, s->env_conditional, s->env_marginal};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+}  // namespace lucida
+
+using namespace lucida;
+
+extern "C" {
+
+const char* lucida_last_error(void) { return g_last_error.c_str(); }
+
+lucida_status lucida_ctx_create(int device, lucida_ctx** out) {
+    if (!out) return set_error(LUCIDA_INVALID_ARG, "lucida_ctx_create: null out");
+    clear_error();
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return set_error(LUCIDA_NO_ADAPTER, "no HIP device is visible");
+    if (device < 0 || device >= count) return set_error(LUCIDA_NO_DEVICE, "device index out of range");
+    LUCIDA_HIP(hipSetDevice(device), LUCIDA_NO_DEVICE);
+    hipDeviceProp_t prop;
+    LUCIDA_HIP(hipGetDeviceProperties(&prop, device), LUCIDA_NO_DEVICE);
+    auto* ctx = new lucida_ctx();
+    ctx->device = device;
+    ctx->cu_count = prop.multiProcessorCount;
+    ctx->name = prop.gcnArchName;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        delete ctx;
+        return set_error(LUCIDA_NO_DEVICE, "could not create the context's stream or events");
+    }
+    *out = ctx;
+    return LUCIDA_OK;
+}
+
+void lucida_ctx_destroy(lucida_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    DevBuf* bufs[] = {&ctx->lb_bounds, &ctx->lb_partial, &ctx->lb_codes, &ctx->lb_vals,   &ctx->lb_tmpk,
+                      &ctx->lb_tmpv,   &ctx->lb_hist,    &ctx->lb_karras, &ctx->lb_parent, &ctx->lb_nodebox,
+                      &ctx->lb_subtree, &ctx->lb_flags,  &ctx->lb_misc,  &ctx->lb_rank,   &ctx->lb_nodes,
+                      &ctx->lb_tris,   &ctx->lb_pos,     &ctx->lb_prims, &ctx->ggx_table, &ctx->wf_planes,
+                      &ctx->wf_hits,   &ctx->wf_shadow,  &ctx->wf_queues, &ctx->wf_counters, &ctx->wf_uniforms,
+                      &ctx->dn_buf};
+    for (DevBuf* b : bufs) b->release();
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+lucida_status lucida_ctx_synchronize(lucida_ctx* ctx) {
+    if (!ctx) return set_error(LUCIDA_INVALID_ARG, "lucida_ctx_synchronize: null ctx");
+    LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_ctx_info(lucida_ctx* ctx, char* name, size_t name_len, int* compute_units) {
+    if (!ctx) return set_error(LUCIDA_INVALID_ARG, "lucida_ctx_info: null ctx");
+    if (name && name_len) {
+        std::strncpy(name, ctx->name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (compute_units) *compute_units = ctx->cu_count;
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_device_alloc(lucida_ctx* ctx, size_t bytes, void** dptr) {
+    if (!ctx || !dptr) return set_error(LUCIDA_INVALID_ARG, "lucida_device_alloc: null argument");
+    LUCIDA_HIP(hipMalloc(dptr, bytes ? bytes : 4), LUCIDA_OOM);
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_device_free(lucida_ctx* ctx, void* dptr) {
+    if (!ctx) return set_error(LUCIDA_INVALID_ARG, "lucida_device_free: null ctx");
+    if (dptr) {
+        LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipFree(dptr), LUCIDA_VALIDATION);
+    }
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_copy_to_device(lucida_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return set_error(LUCIDA_INVALID_ARG, "lucida_copy_to_device: null argument");
+    if (!bytes) return LUCIDA_OK;
+    LUCIDA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_copy_to_host(lucida_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return set_error(LUCIDA_INVALID_ARG, "lucida_copy_to_host: null argument");
+    if (!bytes) return LUCIDA_OK;
+    LUCIDA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), LUCIDA_READBACK);
+    LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_READBACK);
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_scene_create(lucida_ctx* ctx, const lucida_scene_desc* d, lucida_scene** out) {
+    if (!ctx || !d || !out) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_create: null argument");
+    clear_error();
+    if (d->num_materials == 0) return set_error(LUCIDA_INVALID_ARG, "scene has no materials");
+    for (size_t i = 0; i < d->num_triangles; ++i) {
+        const GpuTriangle& t = d->triangles[i];
+        if (t.i0 >= d->num_positions || t.i1 >= d->num_positions || t.i2 >= d->num_positions ||
+            t.material >= d->num_materials)
+            return set_error(LUCIDA_INVALID_ARG, "triangle index or material out of range");
+    }
+    for (size_t i = 0; i < d->num_primitives; ++i)
+        if (d->primitives[i].material >= d->num_materials)
+            return set_error(LUCIDA_INVALID_ARG, "primitive material out of range");
+    for (size_t i = 0; i < d->num_lights; ++i)
+        if (d->lights[i].material >= d->num_materials) return set_error(LUCIDA_INVALID_ARG, "light material out of range");
+    for (size_t i = 0; i < d->num_bvh_nodes; ++i) {
+        const GpuBvhNode& nd = d->bvh_nodes[i];
+        if (nd.count == 0 ? (size_t)nd.left_first + 1 >= d->num_bvh_nodes
+                          : (size_t)nd.left_first + nd.count > (d->num_instances ? d->num_primitives + d->num_instances
+                                                                                 : d->num_triangles))
+            return set_error(LUCIDA_INVALID_ARG, "BVH node references out of range");
+    }
+    if (d->num_instances) return set_error(LUCIDA_INVALID_ARG, "instanced scenes are not supported by this build yet");
+    auto* s = new lucida_scene();
+    s->ctx = ctx;
+    lucida_status st = LUCIDA_OK;
+#define UP(dst, src, cnt)                                  \
+    if (st == LUCIDA_OK) st = upload(ctx, &s->dst, src, cnt);
+    UP(materials, d->materials, d->num_materials);
+    std::vector<GpuPrimitive> prims(d->primitives, d->primitives + d->num_primitives);
+    UP(primitives, prims.data(), prims.size());
+    UP(positions, d->positions, d->num_positions * 4);
+    UP(vertex_attrs, d->vertex_attrs, d->num_positions);
+    UP(triangles, d->triangles, d->num_triangles);
+    UP(lights, d->lights, d->num_lights);
+    size_t env_px = (size_t)d->env_width * d->env_height;
+    UP(env_pixels, d->env_pixels, env_px ? env_px * 4 : 0);
+    UP(env_conditional, d->env_conditional, env_px ? (size_t)(d->env_width + 1) * d->env_height : 0);
+    UP(env_marginal, d->env_marginal, env_px ? (size_t)d->env_height + 1 : 0);
+#undef UP
+    s->num_materials = d->num_materials;
+    s->num_primitives = d->num_primitives;
+    s->num_instances = d->num_instances;
+    s->num_positions = d->num_positions;
+    s->num_triangles = d->num_triangles;
+    s->num_lights = d->num_lights;
+    s->tlas_root = d->tlas_root;
+    s->env_width = d->env_width;
+    s->env_height = d->env_height;
+    s->env_total_weight = d->env_total_weight;
+    if (st == LUCIDA_OK) {
+        if (d->num_bvh_nodes) {
+            st = upload(ctx, &s->bvh_nodes, d->bvh_nodes, d->num_bvh_nodes);
+            s->num_bvh_nodes = d->num_bvh_nodes;
+        } else if (d->build_lbvh && d->num_triangles) {
+            // LBVH on the device, then the triangles permuted into leaf order
+            // (Bvh::compact_primitives, bvh.rs:517) so leaves index them directly.
+            size_t n = d->num_triangles;
+            GpuTriangle* sorted = nullptr;
+            if (hipMalloc(&s->bvh_nodes, (2 * n - 1) * sizeof(GpuBvhNode)) != hipSuccess ||
+                hipMalloc(&sorted, n * sizeof(GpuTriangle)) != hipSuccess) {
+                st = set_error(LUCIDA_OOM, "scene BVH allocation failed");
+            } else {
+                size_t num_nodes = 0;
+                st = lbvh_build_device(ctx, s->triangles, n, s->positions, s->bvh_nodes, &num_nodes, nullptr, sorted,
+                                       nullptr);
+                if (st == LUCIDA_OK) {
+                    (void)hipStreamSynchronize(ctx->stream);
+                    (void)hipFree(s->triangles);
+                    s->triangles = sorted;
+                    sorted = nullptr;
+                    s->num_bvh_nodes = num_nodes;
+                }
+            }
+            if (sorted) (void)hipFree(sorted);
+        }
+    }
+    if (st == LUCIDA_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
+        st = set_error(LUCIDA_VALIDATION, "scene upload failed");
+    if (st != LUCIDA_OK) {
+        scene_free(s);
+        delete s;
+        return st;
+    }
+    *out = s;
+    return LUCIDA_OK;
+}
+
+void lucida_scene_destroy(lucida_scene* s) {
+    if (!s) return;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    scene_free(s);
+    delete s;
+}
+
+lucida_status lucida_scene_num_bvh_nodes(lucida_scene* s, size_t* out) {
+    if (!s || !out) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_num_bvh_nodes: null argument");
+    *out = s->num_bvh_nodes;
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_scene_download_bvh(lucida_scene* s, GpuBvhNode* nodes, GpuTriangle* tris) {
+    if (!s) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_download_bvh: null scene");
+    hipStream_t st = s->ctx->stream;
+    if (nodes && s->num_bvh_nodes)
+        LUCIDA_HIP(hipMemcpyAsync(nodes, s->bvh_nodes, s->num_bvh_nodes * sizeof(GpuBvhNode), hipMemcpyDeviceToHost, st),
+                   LUCIDA_READBACK);
+    if (tris && s->num_triangles)
+        LUCIDA_HIP(hipMemcpyAsync(tris, s->triangles, s->num_triangles * sizeof(GpuTriangle), hipMemcpyDeviceToHost, st),
+                   LUCIDA_READBACK);
+    LUCIDA_HIP(hipStreamSynchronize(st), LUCIDA_READBACK);
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_film_create(lucida_ctx* ctx, uint32_t width, uint32_t height, lucida_film** out) {
+    if (!ctx || !out || width == 0 || height == 0) return set_error(LUCIDA_INVALID_ARG, "lucida_film_create: bad argument");
+    auto* f = new lucida_film();
+    f->ctx = ctx;
+    f->width = width;
+    f->height = height;
+    size_t bytes = (size_t)width * height * sizeof(GpuAccum);
+    if (hipMalloc(&f->accum, bytes) != hipSuccess) {
+        delete f;
+        return set_error(LUCIDA_OOM, "film allocation failed");
+    }
+    if (hipMemsetAsync(f->accum, 0, bytes, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        (void)hipFree(f->accum);
+        delete f;
+        return set_error(LUCIDA_VALIDATION, "film clear failed");
+    }
+    *out = f;
+    return LUCIDA_OK;
+}
+
+void lucida_film_destroy(lucida_film* f) {
+    if (!f) return;
+    (void)hipStreamSynchronize(f->ctx->stream);
+    (void)hipFree(f->accum);
+    delete f;
+}
+
+lucida_status lucida_film_clear(lucida_film* f) {
+    if (!f) return set_error(LUCIDA_INVALID_ARG, "lucida_film_clear: null film");
+    LUCIDA_HIP(hipMemsetAsync(f->accum, 0, (size_t)f->width * f->height * sizeof(GpuAccum), f->ctx->stream),
+               LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipStreamSynchronize(f->ctx->stream), LUCIDA_VALIDATION);
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_film_download(lucida_film* f, GpuAccum* host_out) {
+    if (!f || !host_out) return set_error(LUCIDA_INVALID_ARG, "lucida_film_download: null argument");
+    LUCIDA_HIP(hipMemcpyAsync(host_out, f->accum, (size_t)f->width * f->height * sizeof(GpuAccum),
+                              hipMemcpyDeviceToHost, f->ctx->stream),
+               LUCIDA_READBACK);
+    LUCIDA_HIP(hipStreamSynchronize(f->ctx->stream), LUCIDA_READBACK);
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_film_upload(lucida_film* f, const GpuAccum* host_in) {
+    if (!f || !host_in) return set_error(LUCIDA_INVALID_ARG, "lucida_film_upload: null argument");
+    LUCIDA_HIP(hipMemcpyAsync(f->accum, host_in, (size_t)f->width * f->height * sizeof(GpuAccum),
+                              hipMemcpyHostToDevice, f->ctx->stream),
+               LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipStreamSynchronize(f->ctx->stream), LUCIDA_VALIDATION);
+    return LUCIDA_OK;
+}
+
+void* lucida_film_device_ptr(lucida_film* f) { return f ? f->accum : nullptr; }
+
+}  // extern "C"

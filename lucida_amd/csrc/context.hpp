@@ -1,0 +1,97 @@
+// Context, scene and film objects behind the C ABI, and the HIP error plumbing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/lucida.h"
+#include "status.hpp"
+
+namespace lucida {
+
+// Evaluates a HIP call; on failure records the message and returns `code`
+// from the enclosing function.
+#define LUCIDA_HIP(expr, code)                                                                 \
+    do {                                                                                       \
+        hipError_t e__ = (expr);                                                               \
+        if (e__ != hipSuccess) {                                                               \
+            char b__[512];                                                                     \
+            snprintf(b__, sizeof b__, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e__), \
+                     __FILE__, __LINE__);                                                      \
+            return ::lucida::set_error(code, b__);                                             \
+        }                                                                                      \
+    } while (0)
+
+// A device buffer owned by a context, grown on demand and reused.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= bytes) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return reinterpret_cast<T*>(ptr); }
+};
+
+}  // namespace lucida
+
+struct lucida_ctx {
+    int device = 0;
+    int cu_count = 0;
+    std::string name;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // LBVH scratch, reused between builds.
+    lucida::DevBuf lb_bounds, lb_partial, lb_codes, lb_vals, lb_tmpk, lb_tmpv, lb_hist, lb_karras, lb_parent,
+        lb_nodebox, lb_subtree, lb_flags, lb_misc, lb_rank, lb_nodes, lb_tris, lb_pos, lb_prims;
+    // GGX directional-albedo table (energy compensation), 32 x 32 floats.
+    lucida::DevBuf ggx_table;
+    bool ggx_ready = false;
+    // Wavefront path state, hits, shadow rays, queues, counters, uniforms.
+    lucida::DevBuf wf_planes, wf_hits, wf_shadow, wf_queues, wf_counters, wf_uniforms;
+    size_t wf_capacity = 0;
+    // Denoiser planes.
+    lucida::DevBuf dn_buf;
+};
+
+struct lucida_scene {
+    lucida_ctx* ctx = nullptr;
+    GpuMaterial* materials = nullptr;
+    GpuPrimitive* primitives = nullptr;  // analytic primitives, then instances
+    float* positions = nullptr;
+    GpuVertexAttr* vertex_attrs = nullptr;
+    GpuTriangle* triangles = nullptr;
+    GpuBvhNode* bvh_nodes = nullptr;
+    GpuLight* lights = nullptr;
+    float* env_pixels = nullptr;
+    float* env_conditional = nullptr;
+    float* env_marginal = nullptr;
+    size_t num_materials = 0, num_primitives = 0, num_instances = 0, num_positions = 0, num_triangles = 0,
+           num_bvh_nodes = 0, num_lights = 0;
+    uint32_t tlas_root = 0, env_width = 0, env_height = 0;
+    float env_total_weight = 0.0f;
+};
+
+struct lucida_film {
+    lucida_ctx* ctx = nullptr;
+    uint32_t width = 0, height = 0;
+    GpuAccum* accum = nullptr;
+};
+
+namespace lucida {
+// Ensures the context's GGX energy table is resident (computed on the device).
+lucida_status ensure_ggx_table(lucida_ctx* ctx);
+}  // namespace lucida

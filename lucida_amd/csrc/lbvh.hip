@@ -1,0 +1,776 @@
+// GPU linear BVH for gfx950: prepare (padded triangle bounds + centroid
+// bounds), 30-bit Morton codes, an LDS-staged LSD radix sort ranked with
+// wave64 ballots, the Karras hierarchy, an atomic bottom-up fit, and a relayout
+// into the reference's 32-byte GpuBvhNode tree, all on the device.
+//
+// Reference algorithm: crates/core/src/lbvh.rs (CPU twin) and
+// shaders/lbvh/{prepare,radix,hierarchy,fit}.wgsl; host driver
+// crates/gpu/src/lbvh.rs:369-665. Morton codes, the sorted order, the Karras
+// topology, node boxes and the relayout match the CPU twin bit for bit; this
+// file is compiled with -ffp-contract=off and correctly rounded division so
+// the pad / centroid / normalise arithmetic is the twin's.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "context.hpp"
+
+namespace lucida {
+namespace {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t MAX_LEAF = 4;  // lbvh.rs:185 LBVH_MAX_LEAF
+
+constexpr int PREP_BLOCK = 256;
+constexpr int SORT_BLOCK = 256;
+constexpr int SORT_WAVES = SORT_BLOCK / 64;
+constexpr int SORT_ITEMS = 8;
+constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
+constexpr int RADIX = 256;
+
+__device__ inline float fmin3(float a, float b, float c) { return fminf(fminf(a, b), c); }
+
+// ---------------------------------------------------------------------------
+// prepare: triangle_bounds (bvh.rs:202, prepare.wgsl:48) and the centroid
+// bounds (lbvh.rs:446-449). Each block reduces its centroids with wave
+// reductions; the last block to finish folds the per-block partials, so the
+// result needs no float atomics and is deterministic.
+// ---------------------------------------------------------------------------
+
+__device__ inline float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ __launch_bounds__(PREP_BLOCK) void k_prepare(const GpuTriangle* __restrict__ tris,
+                                                        const float4* __restrict__ pos, uint32_t n,
+                                                        LucidaAabb* __restrict__ tri_bounds,
+                                                        float* __restrict__ partial, uint32_t* __restrict__ ticket,
+                                                        LucidaAabb* __restrict__ scene_box) {
+    const uint32_t i = blockIdx.x * PREP_BLOCK + threadIdx.x;
+    float c[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    if (i < n) {
+        GpuTriangle t = tris[i];
+        float4 p0 = pos[t.i0], p1 = pos[t.i1], p2 = pos[t.i2];
+        float lo[3] = {fminf(fminf(fminf(INFINITY, p0.x), p1.x), p2.x), fminf(fminf(fminf(INFINITY, p0.y), p1.y), p2.y),
+                       fminf(fminf(fminf(INFINITY, p0.z), p1.z), p2.z)};
+        float hi[3] = {fmaxf(fmaxf(fmaxf(-INFINITY, p0.x), p1.x), p2.x),
+                       fmaxf(fmaxf(fmaxf(-INFINITY, p0.y), p1.y), p2.y),
+                       fmaxf(fmaxf(fmaxf(-INFINITY, p0.z), p1.z), p2.z)};
+        if (!(lo[0] > hi[0])) {  // Aabb::pad skips an empty box (bvh.rs:100)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float scale = 1.0f + fmaxf(fabsf(lo[a]), fabsf(hi[a]));
+                float p = 1.0e-6f * scale;
+                lo[a] = lo[a] - p;
+                hi[a] = hi[a] + p;
+            }
+        }
+        LucidaAabb b;
+        b.lo[0] = lo[0]; b.lo[1] = lo[1]; b.lo[2] = lo[2]; b._p0 = 0.0f;
+        b.hi[0] = hi[0]; b.hi[1] = hi[1]; b.hi[2] = hi[2]; b._p1 = 0.0f;
+        tri_bounds[i] = b;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float ca = (lo[a] + hi[a]) * 0.5f;
+            c[a] = fminf(INFINITY, ca);
+            c[3 + a] = fmaxf(-INFINITY, ca);
+        }
+    }
+    __shared__ float red[PREP_BLOCK / 64][6];
+    __shared__ bool last;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c[a] = wave_min(c[a]);
+        c[3 + a] = wave_max(c[3 + a]);
+    }
+    if (lane == 0)
+        for (int a = 0; a < 6; ++a) red[wave][a] = c[a];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r[6];
+        for (int a = 0; a < 6; ++a) r[a] = red[0][a];
+        for (int w = 1; w < PREP_BLOCK / 64; ++w)
+            for (int a = 0; a < 3; ++a) {
+                r[a] = fminf(r[a], red[w][a]);
+                r[3 + a] = fmaxf(r[3 + a], red[w][3 + a]);
+            }
+        for (int a = 0; a < 6; ++a) partial[blockIdx.x * 6 + a] = r[a];
+        __threadfence();
+        uint32_t done = atomicAdd(ticket, 1u);
+        last = (done == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // The last block folds every partial.
+    float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += PREP_BLOCK) {
+        for (int a = 0; a < 3; ++a) {
+            r[a] = fminf(r[a], __builtin_nontemporal_load(&partial[b * 6 + a]));
+            r[3 + a] = fmaxf(r[3 + a], __builtin_nontemporal_load(&partial[b * 6 + 3 + a]));
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        r[a] = wave_min(r[a]);
+        r[3 + a] = wave_max(r[3 + a]);
+    }
+    if (lane == 0)
+        for (int a = 0; a < 6; ++a) red[wave][a] = r[a];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < PREP_BLOCK / 64; ++w)
+            for (int a = 0; a < 3; ++a) {
+                red[0][a] = fminf(red[0][a], red[w][a]);
+                red[0][3 + a] = fmaxf(red[0][3 + a], red[w][3 + a]);
+            }
+        LucidaAabb s;
+        s.lo[0] = red[0][0]; s.lo[1] = red[0][1]; s.lo[2] = red[0][2]; s._p0 = 0.0f;
+        s.hi[0] = red[0][3]; s.hi[1] = red[0][4]; s.hi[2] = red[0][5]; s._p1 = 0.0f;
+        *scene_box = s;
+        *ticket = 0u;  // ready for the next build
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Morton codes: normalise (lbvh.rs:109) then morton3d (lbvh.rs:92).
+// ---------------------------------------------------------------------------
+
+__device__ inline uint32_t expand_bits(uint32_t v) {
+    uint32_t x = v & 0x000003FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// clamp(v * 1023, 0, 1023) then a saturating cast to u32 with NaN -> 0, as
+// Rust's `as u32` does.
+__device__ inline uint32_t quantise(float v) {
+    float s = v * 1023.0f;
+    if (s < 0.0f) s = 0.0f;
+    if (s > 1023.0f) s = 1023.0f;
+    return s >= 0.0f ? (uint32_t)s : 0u;
+}
+
+__global__ __launch_bounds__(PREP_BLOCK) void k_morton(const LucidaAabb* __restrict__ tri_bounds, uint32_t n,
+                                                       const LucidaAabb* __restrict__ scene_box,
+                                                       uint32_t* __restrict__ codes, uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * PREP_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const LucidaAabb sb = *scene_box;
+    const LucidaAabb b = tri_bounds[i];
+    uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float ca = (b.lo[a] + b.hi[a]) * 0.5f;
+        float e = sb.hi[a] - sb.lo[a];
+        float nv = e > 0.0f ? (ca - sb.lo[a]) / e : 0.5f;
+        q[a] = quantise(nv);
+    }
+    codes[i] = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+    vals[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort, 8-bit digits. Per pass: per-tile digit histograms (LDS
+// atomics), one exclusive scan over the digit-major histogram, then a stable
+// scatter whose in-tile ranks come from wave64 ballots: eight ballots give
+// every lane the set of lanes holding its digit, a popcount below the lane
+// gives its rank in the wave, and a per-digit prefix over the block's waves
+// (in LDS) orders the waves. Tiles, rounds, waves and lanes are visited in
+// input order, so equal keys keep their order.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                           uint32_t shift, uint32_t* __restrict__ hist,
+                                                           uint32_t tiles) {
+    __shared__ uint32_t h[RADIX];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * SORT_TILE;
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        uint32_t idx = base + r * SORT_BLOCK + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (RADIX - 1)], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+constexpr int SCAN_BLOCK = 1024;
+
+// Exclusive scan of `count` words in place, one block: every thread scans a
+// contiguous chunk, the chunk totals are scanned in LDS.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(uint32_t* __restrict__ data, uint32_t count) {
+    __shared__ uint32_t sums[SCAN_BLOCK];
+    const uint32_t per = (count + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    const uint32_t begin = threadIdx.x * per;
+    const uint32_t end = begin + per < count ? begin + per : count;
+    uint32_t s = 0;
+    for (uint32_t k = begin; k < end; ++k) s += data[k];
+    sums[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+        uint32_t v = threadIdx.x >= (uint32_t)off ? sums[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sums[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = sums[threadIdx.x] - s;  // exclusive prefix of this chunk
+    for (uint32_t k = begin; k < end; ++k) {
+        uint32_t v = data[k];
+        data[k] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __restrict__ keys,
+                                                              const uint32_t* __restrict__ vals, uint32_t n,
+                                                              uint32_t shift, const uint32_t* __restrict__ offsets,
+                                                              uint32_t tiles, uint32_t* __restrict__ keys_out,
+                                                              uint32_t* __restrict__ vals_out) {
+    __shared__ uint32_t wave_cnt[SORT_WAVES][RADIX];
+    __shared__ uint32_t running[RADIX];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    running[threadIdx.x] = offsets[threadIdx.x * tiles + blockIdx.x];
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+#pragma unroll
+        for (int w = 0; w < SORT_WAVES; ++w) wave_cnt[w][threadIdx.x] = 0;
+        const uint32_t idx = base + r * SORT_BLOCK + threadIdx.x;
+        const bool valid = idx < n;
+        uint32_t key = valid ? keys[idx] : 0u;
+        uint32_t val = valid ? vals[idx] : 0u;
+        uint32_t digit = (key >> shift) & (RADIX - 1);
+        uint64_t same = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (digit >> b) & 1u;
+            uint64_t m = __ballot(bit);
+            same &= bit ? m : ~m;
+        }
+        const uint32_t rank = __popcll(same & lt_mask);
+        __syncthreads();  // zeroing of wave_cnt done
+        if (valid && rank == 0) wave_cnt[wave][digit] = (uint32_t)__popcll(same);
+        __syncthreads();
+        {
+            uint32_t acc = running[threadIdx.x];
+#pragma unroll
+            for (int w = 0; w < SORT_WAVES; ++w) {
+                uint32_t t = wave_cnt[w][threadIdx.x];
+                wave_cnt[w][threadIdx.x] = acc;
+                acc += t;
+            }
+            running[threadIdx.x] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t dst = wave_cnt[wave][digit] + rank;
+            keys_out[dst] = key;
+            vals_out[dst] = val;
+        }
+        __syncthreads();  // wave_cnt is read above before the next round clears it
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Karras hierarchy (lbvh.rs:211-292, hierarchy.wgsl:46-117): one thread per
+// internal node; leaves are ids n-1 .. 2n-2.
+// ---------------------------------------------------------------------------
+
+__device__ inline int lcp(const uint32_t* __restrict__ codes, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = codes[i], b = codes[j];
+    if (a == b) {
+        uint32_t x = (uint32_t)i ^ (uint32_t)j;
+        return 32 + (x == 0 ? 32 : __clz(x));
+    }
+    return __clz(a ^ b);
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ codes, uint32_t n,
+                                                uint2* __restrict__ karras, uint2* __restrict__ ranges,
+                                                uint32_t* __restrict__ parent) {
+    const uint32_t gi = blockIdx.x * 256 + threadIdx.x;
+    if (n < 2 || gi >= n - 1) return;
+    const int N = (int)n, i = (int)gi;
+    int dl = lcp(codes, N, i, i + 1) - lcp(codes, N, i, i - 1);
+    int d = (dl > 0) - (dl < 0);
+    int delta_min = lcp(codes, N, i, i - d);
+    int l_max = 2;
+    while (lcp(codes, N, i, i + l_max * d) > delta_min) l_max *= 2;
+    int l = 0;
+    for (int t = l_max / 2; t >= 1; t /= 2)
+        if (lcp(codes, N, i, i + (l + t) * d) > delta_min) l += t;
+    int j = i + l * d;
+    int first = i < j ? i : j, last = i < j ? j : i;
+    int delta_node = lcp(codes, N, i, j);
+    int s = 0, t = l;
+    while (t > 1) {
+        t = (t + 1) / 2;
+        if (lcp(codes, N, i, i + (s + t) * d) > delta_node) s += t;
+    }
+    int split = i + s * d + (d < 0 ? d : 0);
+    const uint32_t leaf_base = n - 1;
+    uint32_t left = split == first ? leaf_base + (uint32_t)split : (uint32_t)split;
+    uint32_t right = split + 1 == last ? leaf_base + (uint32_t)(split + 1) : (uint32_t)(split + 1);
+    karras[gi] = make_uint2(left, right);
+    ranges[gi] = make_uint2((uint32_t)first, (uint32_t)last);
+    parent[left] = gi;
+    parent[right] = gi;
+}
+
+// ---------------------------------------------------------------------------
+// Atomic bottom-up fit (the CPU twin's visit counting, lbvh.rs:490-525): one
+// thread per leaf climbs; at each internal node the first arrival stops and the
+// second, which now sees both children, writes the union, the subtree size and
+// the count of internal nodes the relayout will emit under it. Agent-scope
+// acquire/release orders the child boxes across XCDs.
+// ---------------------------------------------------------------------------
+
+// Plain loads: they follow the acquiring atomic, which invalidated this CU's
+// caches at agent scope.
+__device__ inline void box_load(const LucidaAabb* p, float* lo, float* hi) {
+    float4 a = reinterpret_cast<const float4*>(p)[0];
+    float4 b = reinterpret_cast<const float4*>(p)[1];
+    lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+    hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
+}
+__device__ inline void box_store(LucidaAabb* p, const float* lo, const float* hi) {
+    reinterpret_cast<float4*>(p)[0] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    reinterpret_cast<float4*>(p)[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_fit(const LucidaAabb* __restrict__ tri_bounds,
+                                             const uint32_t* __restrict__ order, uint32_t n,
+                                             const uint2* __restrict__ karras, const uint32_t* __restrict__ parent,
+                                             LucidaAabb* node_box, uint32_t* subtree, uint32_t* inner,
+                                             uint32_t* flags) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t leaf_base = n - 1;
+    float lo[3], hi[3];
+    {
+        LucidaAabb b = tri_bounds[order[k]];
+        for (int a = 0; a < 3; ++a) { lo[a] = b.lo[a]; hi[a] = b.hi[a]; }
+    }
+    box_store(&node_box[leaf_base + k], lo, hi);
+    uint32_t node = parent[leaf_base + k];
+    while (node != NONE) {
+        __threadfence();
+        uint32_t old = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;  // first arrival: the sibling's thread finishes this node
+        const uint2 c = karras[node];
+        float llo[3], lhi[3], rlo[3], rhi[3];
+        box_load(&node_box[c.x], llo, lhi);
+        box_load(&node_box[c.y], rlo, rhi);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(llo[a], rlo[a]);
+            hi[a] = fmaxf(lhi[a], rhi[a]);
+        }
+        uint32_t sl = c.x >= leaf_base ? 1u : __hip_atomic_load(&subtree[c.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t sr = c.y >= leaf_base ? 1u : __hip_atomic_load(&subtree[c.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t il = c.x >= leaf_base ? 0u : __hip_atomic_load(&inner[c.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t ir = c.y >= leaf_base ? 0u : __hip_atomic_load(&inner[c.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t size = sl + sr;
+        box_store(&node_box[node], lo, hi);
+        __hip_atomic_store(&subtree[node], size, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&inner[node], size > MAX_LEAF ? 1u + il + ir : 0u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        node = parent[node];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Relayout (lbvh.rs:315-406) on the device. The reference walks the tree
+// depth first, left child first, collapsing subtrees of <= 4 primitives into
+// leaves and appending each emitted internal node's two children as an
+// adjacent pair. So an emitted internal node X with pre-order rank R(X) among
+// emitted internal nodes has its children at 1 + 2 R(X), and
+// R(child) = R(X) + 1 (+ the emitted internal count of the left sibling's
+// subtree, for a right child). Leaves list primitives in sorted order, so
+// prim_indices is the sorted order itself and a leaf starts at its range's
+// first sorted position.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_rank(uint32_t n, const uint2* __restrict__ karras,
+                                              const uint32_t* __restrict__ parent,
+                                              const uint32_t* __restrict__ subtree,
+                                              const uint32_t* __restrict__ inner, uint32_t* __restrict__ rank) {
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= n - 1) return;
+    if (subtree[x] <= MAX_LEAF) return;  // not an emitted internal node
+    uint32_t r = 0, cur = x;
+    while (cur != 0) {
+        uint32_t p = parent[cur];
+        uint2 c = karras[p];
+        r += 1;
+        if (cur == c.y) r += c.x >= n - 1 ? 0u : inner[c.x];
+        cur = p;
+    }
+    rank[x] = r;
+}
+
+__global__ __launch_bounds__(256) void k_emit(uint32_t n, const uint2* __restrict__ karras,
+                                              const uint2* __restrict__ ranges, const uint32_t* __restrict__ parent,
+                                              const uint32_t* __restrict__ subtree,
+                                              const uint32_t* __restrict__ rank,
+                                              const LucidaAabb* __restrict__ node_box, GpuBvhNode* __restrict__ out) {
+    const uint32_t id = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t leaf_base = n - 1;
+    if (id >= 2 * n - 1) return;
+    uint32_t slot;
+    if (id == 0) {
+        slot = 0;
+    } else {
+        uint32_t p = parent[id];
+        if (subtree[p] <= MAX_LEAF) return;  // inside a collapsed leaf
+        slot = 1 + 2 * rank[p] + (karras[p].y == id ? 1u : 0u);
+    }
+    const bool is_leaf = id >= leaf_base;
+    const uint32_t size = is_leaf ? 1u : subtree[id];
+    GpuBvhNode nd;
+    const LucidaAabb b = node_box[id];
+    nd.bounds_min[0] = b.lo[0]; nd.bounds_min[1] = b.lo[1]; nd.bounds_min[2] = b.lo[2];
+    nd.bounds_max[0] = b.hi[0]; nd.bounds_max[1] = b.hi[1]; nd.bounds_max[2] = b.hi[2];
+    if (is_leaf || size <= MAX_LEAF) {
+        nd.left_first = is_leaf ? id - leaf_base : ranges[id].x;
+        nd.count = size;
+    } else {
+        nd.left_first = 1 + 2 * rank[id];
+        nd.count = 0;
+    }
+    out[slot] = nd;
+}
+
+__global__ __launch_bounds__(256) void k_gather_tris(const GpuTriangle* __restrict__ tris,
+                                                     const uint32_t* __restrict__ order, uint32_t n,
+                                                     GpuTriangle* __restrict__ out) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) out[k] = tris[order[k]];
+}
+
+__global__ void k_single_leaf(const LucidaAabb* __restrict__ tri_bounds, GpuBvhNode* out, uint32_t* prims) {
+    LucidaAabb b = tri_bounds[0];
+    GpuBvhNode nd;
+    for (int a = 0; a < 3; ++a) {
+        nd.bounds_min[a] = b.lo[a];
+        nd.bounds_max[a] = b.hi[a];
+    }
+    nd.left_first = 0;
+    nd.count = 1;
+    out[0] = nd;
+    if (prims) prims[0] = 0;
+}
+
+inline uint32_t blocks_for(size_t n, int block) { return (uint32_t)((n + block - 1) / block); }
+
+}  // namespace
+
+// Radix-sorts device arrays keys/vals in place over bits [0, bits), using
+// tmp arrays of the same length and ctx's histogram scratch.
+static lucida_status sort_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
+                                 size_t n, uint32_t bits) {
+    if (n == 0) return LUCIDA_OK;
+    if (n > 0xFFFFFFFFull / 2) return set_error(LUCIDA_INVALID_ARG, "sort: more than 2^31 keys");
+    const uint32_t tiles = blocks_for(n, SORT_TILE);
+    LUCIDA_HIP(ctx->lb_hist.ensure((size_t)tiles * RADIX * 4), LUCIDA_OOM);
+    uint32_t* hist = ctx->lb_hist.as<uint32_t>();
+    uint32_t *src_k = keys, *src_v = vals, *dst_k = tk, *dst_v = tv;
+    const uint32_t passes = (bits + 7) / 8;
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        const uint32_t shift = pass * 8;
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_BLOCK), 0, ctx->stream, src_k, (uint32_t)n, shift,
+                           hist, tiles);
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, hist, tiles * RADIX);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_BLOCK), 0, ctx->stream, src_k, src_v,
+                           (uint32_t)n, shift, hist, tiles, dst_k, dst_v);
+        std::swap(src_k, dst_k);
+        std::swap(src_v, dst_v);
+    }
+    if (src_k != keys) {  // odd number of passes: copy back
+        LUCIDA_HIP(hipMemcpyAsync(keys, src_k, n * 4, hipMemcpyDeviceToDevice, ctx->stream), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipMemcpyAsync(vals, src_v, n * 4, hipMemcpyDeviceToDevice, ctx->stream), LUCIDA_VALIDATION);
+    }
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    return LUCIDA_OK;
+}
+
+// The device stages of the build, up to and including the fit. Results stay in
+// the context's scratch (lb_codes = sorted codes, lb_vals = order, ...).
+static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
+                                  float* stage_us, bool keep_unsorted_codes) {
+    hipStream_t s = ctx->stream;
+    const uint32_t nb = blocks_for(n, PREP_BLOCK);
+    const size_t nodes = 2 * n - 1;
+    const size_t internal = n > 1 ? n - 1 : 1;
+    LUCIDA_HIP(ctx->lb_bounds.ensure(n * sizeof(LucidaAabb)), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_partial.ensure((size_t)nb * 6 * 4), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_misc.ensure(256), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_codes.ensure(n * 4 * (keep_unsorted_codes ? 2 : 1)), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_vals.ensure(n * 4), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_tmpk.ensure(n * 4), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_tmpv.ensure(n * 4), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_karras.ensure(internal * 8), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_rank.ensure(internal * 8), LUCIDA_OOM);  // ranges, then ranks
+    LUCIDA_HIP(ctx->lb_parent.ensure(nodes * 4), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_nodebox.ensure(nodes * sizeof(LucidaAabb)), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_subtree.ensure(internal * 4 * 3), LUCIDA_OOM);  // subtree, inner, rank
+    LUCIDA_HIP(ctx->lb_flags.ensure(internal * 4), LUCIDA_OOM);
+
+    uint32_t* misc = ctx->lb_misc.as<uint32_t>();  // [0] ticket, [8..16) scene box
+    LucidaAabb* scene_box = reinterpret_cast<LucidaAabb*>(misc + 8);
+    hipEvent_t ev[6];
+    const bool timed = stage_us != nullptr;
+    if (timed)
+        for (auto& e : ev) LUCIDA_HIP(hipEventCreate(&e), LUCIDA_VALIDATION);
+    if (timed) LUCIDA_HIP(hipEventRecord(ev[0], s), LUCIDA_VALIDATION);
+
+    LUCIDA_HIP(hipMemsetAsync(misc, 0, 4, s), LUCIDA_VALIDATION);
+    hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(PREP_BLOCK), 0, s, tris, reinterpret_cast<const float4*>(positions),
+                       (uint32_t)n, ctx->lb_bounds.as<LucidaAabb>(), ctx->lb_partial.as<float>(), misc, scene_box);
+    hipLaunchKernelGGL(k_morton, dim3(nb), dim3(PREP_BLOCK), 0, s, ctx->lb_bounds.as<LucidaAabb>(), (uint32_t)n,
+                       scene_box, ctx->lb_codes.as<uint32_t>(), ctx->lb_vals.as<uint32_t>());
+    if (keep_unsorted_codes)
+        LUCIDA_HIP(hipMemcpyAsync(ctx->lb_codes.as<uint32_t>() + n, ctx->lb_codes.as<uint32_t>(), n * 4,
+                                  hipMemcpyDeviceToDevice, s),
+                   LUCIDA_VALIDATION);
+    if (timed) LUCIDA_HIP(hipEventRecord(ev[1], s), LUCIDA_VALIDATION);
+    lucida_status st = sort_device(ctx, ctx->lb_codes.as<uint32_t>(), ctx->lb_vals.as<uint32_t>(),
+                                   ctx->lb_tmpk.as<uint32_t>(), ctx->lb_tmpv.as<uint32_t>(), n, 30);
+    if (st != LUCIDA_OK) return st;
+    if (timed) LUCIDA_HIP(hipEventRecord(ev[2], s), LUCIDA_VALIDATION);
+    if (n >= 2) {
+        LUCIDA_HIP(hipMemsetAsync(ctx->lb_parent.ptr, 0xFF, 4, s), LUCIDA_VALIDATION);  // root has no parent
+        hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ctx->lb_codes.as<uint32_t>(),
+                           (uint32_t)n, ctx->lb_karras.as<uint2>(), ctx->lb_rank.as<uint2>(),
+                           ctx->lb_parent.as<uint32_t>());
+    }
+    if (timed) LUCIDA_HIP(hipEventRecord(ev[3], s), LUCIDA_VALIDATION);
+    if (n >= 2) {
+        LUCIDA_HIP(hipMemsetAsync(ctx->lb_flags.ptr, 0, (n - 1) * 4, s), LUCIDA_VALIDATION);
+        uint32_t* subtree = ctx->lb_subtree.as<uint32_t>();
+        hipLaunchKernelGGL(k_fit, dim3(blocks_for(n, 256)), dim3(256), 0, s, ctx->lb_bounds.as<LucidaAabb>(),
+                           ctx->lb_vals.as<uint32_t>(), (uint32_t)n, ctx->lb_karras.as<uint2>(),
+                           ctx->lb_parent.as<uint32_t>(), ctx->lb_nodebox.as<LucidaAabb>(), subtree,
+                           subtree + internal, ctx->lb_flags.as<uint32_t>());
+    }
+    if (timed) LUCIDA_HIP(hipEventRecord(ev[4], s), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    if (timed) {
+        LUCIDA_HIP(hipEventSynchronize(ev[4]), LUCIDA_VALIDATION);
+        for (int k = 0; k < 4; ++k) LUCIDA_HIP(hipEventElapsedTime(&stage_us[k], ev[k], ev[k + 1]), LUCIDA_VALIDATION);
+        for (int k = 0; k < 4; ++k) stage_us[k] *= 1000.0f;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+    return LUCIDA_OK;
+}
+
+// Relayout into GpuBvhNode on the device; returns the node count (a one-word
+// readback of the root's emitted-internal count).
+static lucida_status relayout_device(lucida_ctx* ctx, size_t n, GpuBvhNode* nodes_out, uint32_t* prims_out,
+                                     size_t* num_nodes) {
+    hipStream_t s = ctx->stream;
+    const size_t internal = n - 1;
+    uint32_t* subtree = ctx->lb_subtree.as<uint32_t>();
+    uint32_t* inner = subtree + internal;
+    uint32_t* rank = subtree + 2 * internal;
+    hipLaunchKernelGGL(k_rank, dim3(blocks_for(internal, 256)), dim3(256), 0, s, (uint32_t)n,
+                       ctx->lb_karras.as<uint2>(), ctx->lb_parent.as<uint32_t>(), subtree, inner, rank);
+    hipLaunchKernelGGL(k_emit, dim3(blocks_for(2 * n - 1, 256)), dim3(256), 0, s, (uint32_t)n,
+                       ctx->lb_karras.as<uint2>(), ctx->lb_rank.as<uint2>(), ctx->lb_parent.as<uint32_t>(), subtree,
+                       rank, ctx->lb_nodebox.as<LucidaAabb>(), nodes_out);
+    if (prims_out)
+        LUCIDA_HIP(hipMemcpyAsync(prims_out, ctx->lb_vals.ptr, n * 4, hipMemcpyDeviceToDevice, s), LUCIDA_VALIDATION);
+    uint32_t root_inner = 0;
+    LUCIDA_HIP(hipMemcpyAsync(&root_inner, inner, 4, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
+    LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
+    // The root, when it is a collapsed leaf, has subtree <= 4 and inner 0.
+    *num_nodes = 1 + 2 * (size_t)root_inner;
+    return LUCIDA_OK;
+}
+
+lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
+                                GpuBvhNode* nodes, size_t* num_nodes, uint32_t* prim_indices,
+                                GpuTriangle* tris_sorted, float* stage_us) {
+    if (n == 0) {
+        *num_nodes = 0;
+        return LUCIDA_OK;
+    }
+    if (n > (1u << 30)) return set_error(LUCIDA_INVALID_ARG, "LBVH: more than 2^30 triangles");
+    float us[4] = {0, 0, 0, 0};
+    lucida_status st = build_stages(ctx, tris, n, positions, stage_us ? us : nullptr, false);
+    if (st != LUCIDA_OK) return st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (stage_us) {
+        LUCIDA_HIP(hipEventCreate(&e0), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipEventCreate(&e1), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipEventRecord(e0, ctx->stream), LUCIDA_VALIDATION);
+    }
+    if (n == 1) {
+        hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(1), 0, ctx->stream, ctx->lb_bounds.as<LucidaAabb>(), nodes,
+                           prim_indices);
+        *num_nodes = 1;
+    } else {
+        st = relayout_device(ctx, n, nodes, prim_indices, num_nodes);
+        if (st != LUCIDA_OK) return st;
+    }
+    if (tris_sorted)
+        hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, tris,
+                           ctx->lb_vals.as<uint32_t>(), (uint32_t)n, tris_sorted);
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    if (stage_us) {
+        LUCIDA_HIP(hipEventRecord(e1, ctx->stream), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipEventSynchronize(e1), LUCIDA_VALIDATION);
+        float ms = 0;
+        LUCIDA_HIP(hipEventElapsedTime(&ms, e0, e1), LUCIDA_VALIDATION);
+        for (int k = 0; k < 4; ++k) stage_us[k] = us[k];
+        stage_us[4] = ms * 1000.0f;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return LUCIDA_OK;
+}
+
+}  // namespace lucida
+
+using namespace lucida;
+
+extern "C" {
+
+lucida_status lucida_sort_u32_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
+                                     size_t n) {
+    if (!ctx || (n && (!keys || !vals || !tk || !tv)))
+        return set_error(LUCIDA_INVALID_ARG, "lucida_sort_u32_device: null argument");
+    clear_error();
+    return sort_device(ctx, keys, vals, tk, tv, n, 32);
+}
+
+lucida_status lucida_sort_u32(lucida_ctx* ctx, const uint32_t* keys, const uint32_t* vals, size_t n,
+                              uint32_t* keys_out, uint32_t* vals_out) {
+    if (!ctx || (n && (!keys || !vals || !keys_out || !vals_out)))
+        return set_error(LUCIDA_INVALID_ARG, "lucida_sort_u32: null argument");
+    clear_error();
+    if (n == 0) return LUCIDA_OK;
+    DevBuf buf;
+    LUCIDA_HIP(buf.ensure(n * 16), LUCIDA_OOM);
+    uint32_t* d = buf.as<uint32_t>();
+    lucida_status st = LUCIDA_OK;
+    if (hipMemcpyAsync(d, keys, n * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(d + n, vals, n * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        buf.release();
+        return set_error(LUCIDA_VALIDATION, "lucida_sort_u32: upload failed");
+    }
+    st = sort_device(ctx, d, d + n, d + 2 * n, d + 3 * n, n, 32);
+    if (st == LUCIDA_OK) {
+        if (hipMemcpyAsync(keys_out, d, n * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(vals_out, d + n, n * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            st = set_error(LUCIDA_READBACK, "lucida_sort_u32: readback failed");
+    }
+    buf.release();
+    return st;
+}
+
+lucida_status lucida_lbvh_build_parts(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
+                                      size_t num_positions, lucida_lbvh_parts* out) {
+    if (!ctx || !tris || !positions || !out || n == 0)
+        return set_error(LUCIDA_INVALID_ARG, "lucida_lbvh_build_parts: empty geometry has no tree");
+    clear_error();
+    for (size_t i = 0; i < n; ++i)
+        if (tris[i].i0 >= num_positions || tris[i].i1 >= num_positions || tris[i].i2 >= num_positions)
+            return set_error(LUCIDA_INVALID_ARG, "lucida_lbvh_build_parts: triangle index out of range");
+    LUCIDA_HIP(ctx->lb_tris.ensure(n * sizeof(GpuTriangle)), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_pos.ensure(num_positions * 16), LUCIDA_OOM);
+    hipStream_t s = ctx->stream;
+    LUCIDA_HIP(hipMemcpyAsync(ctx->lb_tris.ptr, tris, n * sizeof(GpuTriangle), hipMemcpyHostToDevice, s),
+               LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipMemcpyAsync(ctx->lb_pos.ptr, positions, num_positions * 16, hipMemcpyHostToDevice, s),
+               LUCIDA_VALIDATION);
+    lucida_status st =
+        build_stages(ctx, ctx->lb_tris.as<GpuTriangle>(), n, ctx->lb_pos.as<float>(), nullptr, true);
+    if (st != LUCIDA_OK) return st;
+    const size_t internal = n - 1;
+    uint32_t* misc = ctx->lb_misc.as<uint32_t>();
+    if (out->codes)
+        LUCIDA_HIP(hipMemcpyAsync(out->codes, ctx->lb_codes.as<uint32_t>() + n, n * 4, hipMemcpyDeviceToHost, s),
+                   LUCIDA_READBACK);
+    if (out->order) LUCIDA_HIP(hipMemcpyAsync(out->order, ctx->lb_vals.ptr, n * 4, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
+    if (out->scene_bounds)
+        LUCIDA_HIP(hipMemcpyAsync(out->scene_bounds, misc + 8, 32, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
+    if (n >= 2) {
+        if (out->karras)
+            LUCIDA_HIP(hipMemcpyAsync(out->karras, ctx->lb_karras.ptr, internal * 8, hipMemcpyDeviceToHost, s),
+                       LUCIDA_READBACK);
+        if (out->parent)
+            LUCIDA_HIP(hipMemcpyAsync(out->parent, ctx->lb_parent.ptr, (2 * n - 1) * 4, hipMemcpyDeviceToHost, s),
+                       LUCIDA_READBACK);
+        if (out->node_bounds)
+            LUCIDA_HIP(hipMemcpyAsync(out->node_bounds, ctx->lb_nodebox.ptr, (2 * n - 1) * sizeof(LucidaAabb),
+                                      hipMemcpyDeviceToHost, s),
+                       LUCIDA_READBACK);
+        if (out->subtree_size)
+            LUCIDA_HIP(hipMemcpyAsync(out->subtree_size, ctx->lb_subtree.ptr, internal * 4, hipMemcpyDeviceToHost, s),
+                       LUCIDA_READBACK);
+    } else {
+        if (out->parent) out->parent[0] = 0xFFFFFFFFu;
+        if (out->node_bounds)
+            LUCIDA_HIP(hipMemcpyAsync(out->node_bounds, ctx->lb_bounds.ptr, sizeof(LucidaAabb), hipMemcpyDeviceToHost, s),
+                       LUCIDA_READBACK);
+    }
+    LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
+                                       GpuBvhNode* nodes, size_t* num_nodes, uint32_t* prim_indices,
+                                       GpuTriangle* tris_sorted, float* stage_us) {
+    if (!ctx || !num_nodes || (n && (!tris || !positions || !nodes)))
+        return set_error(LUCIDA_INVALID_ARG, "lucida_lbvh_build_device: null argument");
+    clear_error();
+    return lbvh_build_device(ctx, tris, n, positions, nodes, num_nodes, prim_indices, tris_sorted, stage_us);
+}
+
+lucida_status lucida_lbvh_build(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
+                                size_t num_positions, GpuBvhNode* nodes_out, size_t* num_nodes,
+                                uint32_t* prim_indices_out) {
+    if (!ctx || !num_nodes || (n && (!tris || !positions || !nodes_out || !prim_indices_out)))
+        return set_error(LUCIDA_INVALID_ARG, "lucida_lbvh_build: null argument");
+    clear_error();
+    if (n == 0) {
+        *num_nodes = 0;
+        return LUCIDA_OK;
+    }
+    for (size_t i = 0; i < n; ++i)
+        if (tris[i].i0 >= num_positions || tris[i].i1 >= num_positions || tris[i].i2 >= num_positions)
+            return set_error(LUCIDA_INVALID_ARG, "lucida_lbvh_build: triangle index out of range");
+    hipStream_t s = ctx->stream;
+    LUCIDA_HIP(ctx->lb_tris.ensure(n * sizeof(GpuTriangle)), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_pos.ensure(num_positions * 16), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_nodes.ensure((2 * n - 1) * sizeof(GpuBvhNode)), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->lb_prims.ensure(n * 4), LUCIDA_OOM);
+    LUCIDA_HIP(hipMemcpyAsync(ctx->lb_tris.ptr, tris, n * sizeof(GpuTriangle), hipMemcpyHostToDevice, s),
+               LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipMemcpyAsync(ctx->lb_pos.ptr, positions, num_positions * 16, hipMemcpyHostToDevice, s),
+               LUCIDA_VALIDATION);
+    lucida_status st = lbvh_build_device(ctx, ctx->lb_tris.as<GpuTriangle>(), n, ctx->lb_pos.as<float>(),
+                                         ctx->lb_nodes.as<GpuBvhNode>(), num_nodes, ctx->lb_prims.as<uint32_t>(),
+                                         nullptr, nullptr);
+    if (st != LUCIDA_OK) return st;
+    LUCIDA_HIP(hipMemcpyAsync(nodes_out, ctx->lb_nodes.ptr, *num_nodes * sizeof(GpuBvhNode), hipMemcpyDeviceToHost, s),
+               LUCIDA_READBACK);
+    LUCIDA_HIP(hipMemcpyAsync(prim_indices_out, ctx->lb_prims.ptr, n * 4, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
+    LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
+    return LUCIDA_OK;
+}
+
+}  // extern "C"

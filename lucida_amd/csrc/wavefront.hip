@@ -1,0 +1,806 @@
+// Wavefront path tracer for gfx950.
+//
+// Reference: shaders/wavefront/{generate,extend,shade,connect,reset,resolve}.wgsl
+// and the host loop of crates/gpu/src/wavefront.rs:152-676; the CPU twin is
+// integrator.rs:209-699.
+//
+// Differences from the reference's WebGPU design, all MI355X-driven:
+//  * Path state is kept as 16-byte planes (struct-of-arrays) instead of one
+//    112-byte record, so each stage streams only the planes it needs: EXTEND
+//    reads 32 bytes of a path, not 112.
+//  * EXTEND is pure traversal and writes a 16-byte hit (t, u, v, primitive);
+//    SHADE reconstructs the surface, so the traversal kernel stays small.
+//  * Traversal carries a visited child's (first, count) with it, so each
+//    node visit reads only the two 32-byte children (64 bytes), and the
+//    stack lives in LDS, one column per lane.
+//  * Queues are compacted with wave-level prefix sums: one atomic per wave
+//    (ballot + popcount), not one per path.
+//  * There is no indirect dispatch on HIP; kernels read their item counts
+//    from device memory and the grid covers the batch's upper bound. Queue
+//    and shadow counters have one slot per bounce, so no RESET pass is needed.
+//  * The guide channels and radiance stay per path and RESOLVE sums a pixel's
+//    samples in sample order, as the reference does, so the image is the
+//    CPU twin's up to float reassociation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "context.hpp"
+#include "shading.hpp"
+
+namespace lucida {
+namespace {
+
+using namespace dev;
+
+constexpr uint32_t MISS = 0xFFFFFFFFu;
+constexpr uint32_t ANALYTIC = 0x80000000u;
+constexpr int WG = 64;          // one wave per workgroup for the per-path kernels
+constexpr int TRACE_WG = 128;   // traversal workgroups (two waves)
+constexpr int STACK = 32;       // traversal stack entries per lane (shaders/common/bvh.wgsl:133)
+constexpr int MAX_BOUNCES = 64;
+
+// Per-render device state. Planes are float4 arrays of P entries.
+struct Planes {
+    float4* org;   // origin.xyz, depth (uint bits)
+    float4* dir;   // direction.xyz, rng_state (uint bits)
+    float4* thr;   // throughput.xyz, prev_bsdf_pdf
+    float4* prv;   // prev_position.xyz, sampler_dim (uint bits)
+    float4* rad;   // radiance.xyz, unused
+    float4* gal;   // guide albedo.xyz, guide depth
+    float4* gnr;   // guide normal.xyz, unused
+    float4* hit;   // t, u, v, primitive id (uint bits)
+    float4* sro;   // shadow: origin.xyz, t_max
+    float4* srd;   // shadow: direction.xyz, path (uint bits)
+    float4* src;   // shadow: contribution.xyz, unused
+    uint32_t* q0;
+    uint32_t* q1;
+    uint32_t* counters;  // [0..MAX_BOUNCES): queue length per bounce; [64..128): shadow count per bounce;
+                         // [128..136): u64 stats (ext rays, shadow rays, node visits, tri tests)
+};
+
+struct SceneView {
+    const GpuMaterial* __restrict__ materials;
+    const GpuPrimitive* __restrict__ prims;
+    const float4* __restrict__ pos;
+    const GpuVertexAttr* __restrict__ attrs;
+    const GpuTriangle* __restrict__ tris;
+    const GpuBvhNode* __restrict__ nodes;
+    const GpuLight* __restrict__ lights;
+    const float* __restrict__ ggx;
+    EnvView env;
+    uint32_t num_prims, num_tris, num_nodes, num_lights;
+};
+
+__device__ inline uint64_t lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Wave-level append: every lane of the wave must call this; lanes with
+// `want` get consecutive slots from one atomic per wave.
+__device__ inline uint32_t wave_append(bool want, uint32_t* counter) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return 0;
+    const uint32_t total = (uint32_t)__popcll(m);
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(m & lanemask_lt());
+}
+
+__device__ inline uint64_t wave_sum_u32(uint32_t v) {
+    uint64_t s = v;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    return s;
+}
+
+// Per-wave totals of the traversal counters, one atomic each. Overflows are
+// always counted; visits and tests only when asked for.
+__device__ inline void flush_traversal_stats(uint32_t* counters, uint32_t visits, uint32_t tests, uint32_t overflow,
+                                             uint32_t stats_on) {
+    uint64_t* s64 = reinterpret_cast<uint64_t*>(counters + 128);
+    const uint64_t so = wave_sum_u32(overflow);
+    if (stats_on) {
+        const uint64_t sv = wave_sum_u32(visits), st = wave_sum_u32(tests);
+        if ((threadIdx.x & 63) == 0) {
+            if (sv) atomicAdd((unsigned long long*)&s64[2], (unsigned long long)sv);
+            if (st) atomicAdd((unsigned long long*)&s64[3], (unsigned long long)st);
+        }
+    }
+    if ((threadIdx.x & 63) == 0 && so) atomicAdd((unsigned long long*)&s64[4], (unsigned long long)so);
+}
+
+// ---------------------------------------------------------------------------
+// GENERATE (generate.wgsl:36-88): camera ray per (pixel, sample); identity
+// queue; bounce-0 count.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_generate(const GpuUniforms* __restrict__ U, Planes p) {
+    const GpuUniforms u = *U;
+    const uint32_t pixels = u.width * u.height;
+    const uint32_t total = pixels * u.samples_per_launch;
+    const uint32_t index = blockIdx.x * 256 + threadIdx.x;
+    if (index == 0) p.counters[0] = total;
+    if (index >= total) return;
+    const uint32_t pixel = index % pixels;
+    const uint32_t sample = u.sample_offset + index / pixels;
+    const uint32_t x = pixel % u.width, y = pixel / u.width;
+    Sampler s = make_sampler(u, pixel, sample);
+    float pu, pv, lu, lv;
+    s.next_vec2(&pu, &pv);
+    s.next_vec2(&lu, &lv);
+    // camera.rs:84 generate_ray
+    float sx = ((float)x + pu) / (float)u.width;
+    float ty = ((float)y + pv) / (float)u.height;
+    V3 origin = ld3(u.cam_origin);
+    V3 target = ld3(u.cam_upper_left) + sx * ld3(u.cam_horizontal) + ty * ld3(u.cam_vertical);
+    if (u.lens_radius > 0.0f) {
+        float dx, dy;
+        concentric_disk(lu, lv, &dx, &dy);
+        dx *= u.lens_radius;
+        dy *= u.lens_radius;
+        V3 right = normalize(ld3(u.cam_horizontal)), down = normalize(ld3(u.cam_vertical));
+        origin = origin + (right * dx + down * dy);
+    }
+    V3 d = normalize(target - origin);
+    p.org[index] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(0u));
+    p.dir[index] = make_float4(d.x, d.y, d.z, __uint_as_float(s.state));
+    p.thr[index] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    p.prv[index] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(s.dim));
+    p.rad[index] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    p.gal[index] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    p.gnr[index] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    p.q0[index] = index;
+}
+
+// ---------------------------------------------------------------------------
+// Traversal (bvh.rs:1045 / bvh.wgsl:137): near-first, children tested as a
+// pair, the far child pushed. A stack entry packs a child's (first, count) so
+// a popped node is never re-read: (first << 3) | count, count <= 4 (leaf) or
+// 0 (internal). An overflowing push is dropped, as the reference GPU does,
+// and counted.
+// ---------------------------------------------------------------------------
+
+__device__ inline bool slab(const float4 lo, const float4 hi, V3 o, V3 inv, float t_min, float t_max, float* entry) {
+    float t0x = (lo.x - o.x) * inv.x, t1x = (hi.x - o.x) * inv.x;
+    float t0y = (lo.y - o.y) * inv.y, t1y = (hi.y - o.y) * inv.y;
+    float t0z = (lo.z - o.z) * inv.z, t1z = (hi.z - o.z) * inv.z;
+    float nx = fminf(t0x, t1x), ny = fminf(t0y, t1y), nz = fminf(t0z, t1z);
+    float fx = fmaxf(t0x, t1x), fy = fmaxf(t0y, t1y), fz = fmaxf(t0z, t1z);
+    float en = fmaxf(fmaxf(fmaxf(nx, ny), nz), t_min);
+    float ex = fminf(fminf(fminf(fx, fy), fz), t_max);
+    *entry = en;
+    return en <= ex;
+}
+
+__device__ inline V3 safe_inv_dir(V3 d) {
+    const float BIG = 1.0e16f, SMALL = 1.0e-16f;
+    return V3{fabsf(d.x) > SMALL ? 1.0f / d.x : copysignf(BIG, d.x),
+              fabsf(d.y) > SMALL ? 1.0f / d.y : copysignf(BIG, d.y),
+              fabsf(d.z) > SMALL ? 1.0f / d.z : copysignf(BIG, d.z)};
+}
+
+// bvh.rs:930 intersect_triangle, in the twin's operation order.
+__device__ inline bool tri_test(const float4* __restrict__ pos, const GpuTriangle& tri, V3 o, V3 d, float t_min,
+                                float t_max, float* t_out, float* u_out, float* v_out) {
+    V3 p0 = f4xyz(pos[tri.i0]);
+    V3 e1 = f4xyz(pos[tri.i1]) - p0;
+    V3 e2 = f4xyz(pos[tri.i2]) - p0;
+    V3 pv = cross(d, e2);
+    float det = dot(e1, pv);
+    if (fabsf(det) < 1.0e-12f) return false;
+    float inv_det = 1.0f / det;
+    V3 tv = o - p0;
+    float u = dot(tv, pv) * inv_det;
+    if (!(u >= 0.0f && u <= 1.0f)) return false;
+    V3 qv = cross(tv, e1);
+    float v = dot(d, qv) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float t = dot(e2, qv) * inv_det;
+    if (t < t_min || t > t_max) return false;
+    *t_out = t;
+    *u_out = u;
+    *v_out = v;
+    return true;
+}
+
+__device__ inline uint32_t pack_child(uint32_t first, uint32_t count) { return (first << 3) | count; }
+
+// Closest hit (any_hit = false) or occlusion (any_hit = true) against the
+// scene's triangles. Returns hit flag; closest-hit fills t/u/v/tri.
+template <bool ANY_HIT>
+__device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk, V3 o, V3 d, float t_min,
+                                float t_max, float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
+                                uint32_t* visits, uint32_t* tests, uint32_t* overflow) {
+    bool found = false;
+    float closest = t_max;
+    if (sc.num_nodes == 0) {
+        for (uint32_t i = 0; i < sc.num_tris; ++i) {
+            float t, u, v;
+            if (tri_test(sc.pos, sc.tris[i], o, d, t_min, closest, &t, &u, &v)) {
+                if (ANY_HIT) return true;
+                closest = t;
+                *t_out = t; *u_out = u; *v_out = v; *tri_out = i;
+                found = true;
+            }
+        }
+        return found;
+    }
+    const V3 inv = safe_inv_dir(d);
+    const GpuBvhNode root = sc.nodes[0];
+    uint32_t cur = pack_child(root.left_first, root.count);
+    int sp = 0;
+    const int lane_stride = TRACE_WG;
+    for (;;) {
+        *visits += 1;
+        const uint32_t first = cur >> 3, count = cur & 7u;
+        if (count > 0) {
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t ti = first + k;
+                *tests += 1;
+                float t, u, v;
+                if (tri_test(sc.pos, sc.tris[ti], o, d, t_min, closest, &t, &u, &v)) {
+                    if (ANY_HIT) return true;
+                    closest = t;
+                    *t_out = t; *u_out = u; *v_out = v; *tri_out = ti;
+                    found = true;
+                }
+            }
+        } else {
+            const float4* cn = reinterpret_cast<const float4*>(sc.nodes + first);
+            const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
+            float ea, eb;
+            const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
+            const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
+            const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
+            const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
+            if (ha && hb) {
+                const bool a_near = ANY_HIT ? true : ea <= eb;
+                if (sp < STACK) {
+                    stk[sp * lane_stride] = a_near ? cb : ca;
+                    ++sp;
+                } else {
+                    *overflow += 1;
+                }
+                cur = a_near ? ca : cb;
+                continue;
+            } else if (ha) {
+                cur = ca;
+                continue;
+            } else if (hb) {
+                cur = cb;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = stk[sp * lane_stride];
+    }
+    return found;
+}
+
+// ---------------------------------------------------------------------------
+// EXTEND (extend.wgsl:124, scene.rs:452): analytic primitives by brute force,
+// then the BVH with t_max = the best analytic hit. Writes (t, u, v, id):
+// id = triangle, ANALYTIC | primitive, or MISS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc, Planes p, const uint32_t* __restrict__ queue,
+                                                     const uint32_t* __restrict__ count_ptr, uint32_t stats_on) {
+    __shared__ uint32_t stack_mem[STACK * TRACE_WG];
+    const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
+    const uint32_t count = *count_ptr;
+    uint32_t visits = 0, tests = 0, overflow = 0;
+    if (i < count) {
+        const uint32_t pid = queue[i];
+        const float4 of = p.org[pid], df = p.dir[pid];
+        const V3 o = f4xyz(of), d = f4xyz(df);
+        float best_t = T_MAX;
+        uint32_t best = MISS;
+        for (uint32_t k = 0; k < sc.num_prims; ++k) {
+            const GpuPrimitive& pr = sc.prims[k];
+            float t;
+            bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, o, d, T_MIN, best_t, &t)
+                                                      : intersect_sphere(pr, o, d, T_MIN, best_t, &t);
+            if (h) {
+                best_t = t;
+                best = k;
+            }
+        }
+        float t = 0.0f, u = 0.0f, v = 0.0f;
+        uint32_t tri = 0;
+        bool th = traverse<false>(sc, stack_mem + threadIdx.x, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
+                                  &overflow);
+        float4 out;
+        if (th) out = make_float4(t, u, v, __uint_as_float(tri));
+        else if (best != MISS) out = make_float4(best_t, 0.0f, 0.0f, __uint_as_float(ANALYTIC | best));
+        else out = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(MISS));
+        p.hit[i] = out;
+    }
+    flush_traversal_stats(p.counters, visits, tests, overflow, stats_on);
+}
+
+// ---------------------------------------------------------------------------
+// SHADE (shade.wgsl:120-348 = integrator.rs radiance_with_guide, one bounce).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_shade(SceneView sc, const GpuUniforms* __restrict__ U, Planes p,
+                                              const uint32_t* __restrict__ queue_in,
+                                              const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ queue_out,
+                                              uint32_t* __restrict__ next_count, uint32_t* __restrict__ shadow_count) {
+    const uint32_t i = blockIdx.x * WG + threadIdx.x;
+    const uint32_t count = *count_ptr;
+    bool alive = false, shadow = false;
+    uint32_t pid = 0;
+    V3 sr_o{0, 0, 0}, sr_d{0, 0, 0}, sr_c{0, 0, 0};
+    float sr_t = 0.0f;
+    if (i < count) {
+        const GpuUniforms& u = *U;
+        pid = queue_in[i];
+        const float4 hit = p.hit[i];
+        float4 of = p.org[pid], df = p.dir[pid], tf = p.thr[pid], pf = p.prv[pid], rf = p.rad[pid];
+        const uint32_t depth = __float_as_uint(of.w);
+        const V3 origin = f4xyz(of), dir = f4xyz(df);
+        V3 throughput = f4xyz(tf), radiance = f4xyz(rf), prev_position = f4xyz(pf);
+        const float prev_bsdf_pdf = tf.w;
+        const uint32_t pixels = u.width * u.height;
+        Sampler s;
+        s.sobol = u.sampler_kind == LUCIDA_SAMPLER_SOBOL;
+        s.state = __float_as_uint(df.w);
+        s.index = u.sample_offset + pid / pixels;
+        s.seed = hash_u32(pid % pixels + u.frame_seed * 0x9e3779b9u);
+        s.dim = __float_as_uint(pf.w);
+        const uint32_t mode = u.sampling_mode > 2 ? 2 : u.sampling_mode;
+        const uint32_t total = (uint32_t)sc.num_lights + (sc.env.present() ? 1u : 0u);
+        const uint32_t id = __float_as_uint(hit.w);
+
+        if (id == MISS) {
+            if (sc.env.present()) {
+                float w = 1.0f;
+                if (depth > 0) {
+                    if (mode == LUCIDA_SAMPLING_NEE) w = 0.0f;
+                    else if (mode == LUCIDA_SAMPLING_MIS)
+                        w = power_heuristic(prev_bsdf_pdf, env_pdf(sc.env, dir) / (float)max(total, 1u));
+                }
+                if (w > 0.0f) radiance = radiance + throughput * env_radiance(sc.env, dir) * w;
+            } else {
+                radiance = radiance + throughput * ld3(u.background);
+            }
+            p.rad[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+        } else {
+            // Surface at the hit: shade_triangle / shade_analytic (scene.rs:405-560).
+            const float t = hit.x;
+            V3 position = origin + t * dir;
+            V3 normal, gnormal;
+            bool front;
+            uint32_t material;
+            float light_area;
+            if (id & ANALYTIC) {
+                const GpuPrimitive& pr = sc.prims[id & ~ANALYTIC];
+                const bool quad = pr.kind == LUCIDA_PRIM_KIND_QUAD;
+                light_area = quad ? length(cross(ld3(pr.edge_u), ld3(pr.edge_v))) : 0.0f;
+                V3 gn = quad ? ld3(pr.normal) : (position - ld3(pr.position)) / pr.radius;
+                front = dot(gn, dir) < 0.0f;
+                gnormal = front ? gn : -gn;
+                normal = gnormal;
+                material = pr.material;
+            } else {
+                const GpuTriangle tri = sc.tris[id];
+                const V3 p0 = f4xyz(sc.pos[tri.i0]), p1 = f4xyz(sc.pos[tri.i1]), p2 = f4xyz(sc.pos[tri.i2]);
+                const float w = 1.0f - hit.y - hit.z;
+                V3 sh = normalize_or(w * ld3(sc.attrs[tri.i0].normal) + hit.y * ld3(sc.attrs[tri.i1].normal) +
+                                         hit.z * ld3(sc.attrs[tri.i2].normal),
+                                     v3(0, 0, 1));
+                V3 cr = cross(p1 - p0, p2 - p0);
+                V3 g = normalize_or(cr, v3(0, 0, 1));
+                front = dot(g, dir) < 0.0f;
+                const float flip = front ? 1.0f : -1.0f;
+                normal = sh * flip;
+                gnormal = g * flip;
+                material = tri.material;
+                light_area = 0.5f * length(cr);
+            }
+            const GpuMaterial m = sc.materials[material];
+            const V3 emissive = ld3(m.emissive);
+            if (front && max3(emissive) > 0.0f) {
+                float w = 1.0f;
+                if (depth > 0) {
+                    if (mode == LUCIDA_SAMPLING_NEE) w = 0.0f;
+                    else if (mode == LUCIDA_SAMPLING_MIS) {
+                        // light_pdf_from_geometry (light.rs:241) * n / total
+                        float pdf_light = 0.0f;
+                        const uint32_t n = sc.num_lights;
+                        if (n != 0 && light_area > 0.0f) {
+                            V3 tl = position - prev_position;
+                            float dist_sq = dot(tl, tl);
+                            if (dist_sq > 0.0f) {
+                                V3 wl = tl / sqrtf(dist_sq);
+                                float cos_l = dot(gnormal, -wl);
+                                if (cos_l > 1.0e-6f) pdf_light = (1.0f / ((float)n * light_area)) * dist_sq / cos_l;
+                            }
+                        }
+                        pdf_light = pdf_light * ((float)n / (float)max(total, 1u));
+                        w = power_heuristic(prev_bsdf_pdf, pdf_light);
+                    }
+                }
+                if (w > 0.0f) radiance = radiance + throughput * emissive * w;
+            }
+            const Surface surf = surface_from_material(m, front, sc.ggx);
+            if (depth == 0) {
+                V3 ga = vmax(surf.diffuse_albedo + surf.f0 + emissive, v3s(1.0e-3f));
+                p.gal[pid] = make_float4(ga.x, ga.y, ga.z, t);
+                p.gnr[pid] = make_float4(normal.x, normal.y, normal.z, 0.0f);
+            }
+            const V3 wo = to_local(-dir, normal);
+            if (wo.z > 0.0f) {
+                if (mode != LUCIDA_SAMPLING_BSDF && depth + 1 < u.max_depth && total > 0) {
+                    // direct_light (integrator.rs:418): the contribution is
+                    // resolved here; CONNECT only answers visibility.
+                    const float u_select = s.next_f32();
+                    float ua_x, ua_y;
+                    s.next_vec2(&ua_x, &ua_y);
+                    const uint32_t n_lights = sc.num_lights;
+                    const uint32_t pick = min((uint32_t)(u_select * (float)total), total - 1u);
+                    V3 wi_world{0, 1, 0}, rin{0, 0, 0};
+                    float distance = 0.0f, pdf = 0.0f;
+                    bool valid = false;
+                    if (pick < n_lights) {
+                        const float u_light = ((float)pick + 0.5f) / (float)n_lights;
+                        const uint32_t li = min((uint32_t)(u_light * (float)n_lights), n_lights - 1u);
+                        const GpuLight L = sc.lights[li];
+                        V3 lo = ld3(L.origin), eu = ld3(L.edge_u), ev = ld3(L.edge_v);
+                        V3 pt;
+                        if (L.kind == LUCIDA_LIGHT_KIND_TRIANGLE) {
+                            float su = sqrtf(ua_x);
+                            pt = lo + (1.0f - su) * eu + (ua_y * su) * ev;
+                        } else {
+                            pt = lo + ua_x * eu + ua_y * ev;
+                        }
+                        V3 tl = pt - position;
+                        float dist_sq = dot(tl, tl);
+                        if (dist_sq > 0.0f) {
+                            float dist = sqrtf(dist_sq);
+                            V3 wl = tl / dist;
+                            float cos_l = dot(ld3(L.normal), -wl);
+                            if (cos_l > 1.0e-6f) {
+                                float lp = (1.0f / ((float)n_lights * L.area)) * dist_sq / cos_l;
+                                if (isfinite(lp) && lp > 0.0f) {
+                                    wi_world = wl;
+                                    rin = ld3(sc.materials[L.material].emissive);
+                                    distance = dist;
+                                    pdf = lp * ((float)n_lights / (float)total);
+                                    valid = true;
+                                }
+                            }
+                        }
+                    } else {
+                        V3 ed, er;
+                        float ep = env_sample(sc.env, ua_x, ua_y, &ed, &er);
+                        if (ep > 0.0f) {
+                            wi_world = ed;
+                            rin = er;
+                            distance = INFINITY;
+                            pdf = ep / (float)total;
+                            valid = true;
+                        }
+                    }
+                    if (valid && pdf > 0.0f) {
+                        V3 wi = to_local(wi_world, normal);
+                        if (wi.z != 0.0f && (wi.z > 0.0f || surf.transmission > 0.0f)) {
+                            V3 f = surface_eval(surf, wo, wi);
+                            if (max3(f) > 0.0f) {
+                                float mis = mode == LUCIDA_SAMPLING_MIS
+                                                ? power_heuristic(pdf, surface_pdf(surf, wo, wi))
+                                                : 1.0f;
+                                if (mis > 0.0f) {
+                                    sr_o = offset_ray_origin(position, wi.z < 0.0f ? -gnormal : gnormal);
+                                    sr_d = wi_world;
+                                    sr_t = distance * (1.0f - 1.0e-3f);
+                                    sr_c = throughput * (f * fabsf(wi.z) * rin * (mis / pdf));
+                                    shadow = true;
+                                }
+                            }
+                        }
+                    }
+                }
+                const float u_lobe = s.next_f32();
+                float bx, by;
+                s.next_vec2(&bx, &by);
+                V3 wi, weight;
+                const float bpdf = surface_sample(surf, wo, u_lobe, bx, by, &wi, &weight);
+                if (bpdf > 0.0f) {
+                    V3 nt = throughput * weight;
+                    if (max3(nt) > 0.0f && depth + 1 < u.max_depth) {
+                        alive = true;
+                        V3 no = offset_ray_origin(position, wi.z < 0.0f ? -gnormal : gnormal);
+                        V3 nd = to_world(wi, normal);
+                        p.org[pid] = make_float4(no.x, no.y, no.z, __uint_as_float(depth + 1));
+                        p.dir[pid] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(s.state));
+                        p.thr[pid] = make_float4(nt.x, nt.y, nt.z, bpdf);
+                        p.prv[pid] = make_float4(position.x, position.y, position.z, __uint_as_float(s.dim));
+                    }
+                }
+            }
+            p.rad[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+        }
+    }
+    // Compaction: one atomic per wave for each queue.
+    const uint32_t sslot = wave_append(shadow, shadow_count);
+    if (shadow) {
+        p.sro[sslot] = make_float4(sr_o.x, sr_o.y, sr_o.z, sr_t);
+        p.srd[sslot] = make_float4(sr_d.x, sr_d.y, sr_d.z, __uint_as_float(pid));
+        p.src[sslot] = make_float4(sr_c.x, sr_c.y, sr_c.z, 0.0f);
+    }
+    const uint32_t qslot = wave_append(alive, next_count);
+    if (alive) queue_out[qslot] = pid;
+}
+
+// ---------------------------------------------------------------------------
+// CONNECT (connect.wgsl:192-240, light.rs:327 unoccluded): any-hit over the
+// analytic primitives and the BVH; an unoccluded ray adds its contribution
+// to its path's radiance (one shadow ray per path per bounce: no races).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc, Planes p, const uint32_t* __restrict__ count_ptr,
+                                                      uint32_t stats_on) {
+    __shared__ uint32_t stack_mem[STACK * TRACE_WG];
+    const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
+    const uint32_t count = *count_ptr;
+    uint32_t visits = 0, tests = 0, overflow = 0;
+    if (i < count) {
+        const float4 a = p.sro[i], b = p.srd[i];
+        const V3 o = f4xyz(a), d = f4xyz(b);
+        const float t_max = a.w;
+        bool blocked = false;
+        for (uint32_t k = 0; k < sc.num_prims && !blocked; ++k) {
+            const GpuPrimitive& pr = sc.prims[k];
+            float t;
+            blocked = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, o, d, T_MIN, t_max, &t)
+                                                       : intersect_sphere(pr, o, d, T_MIN, t_max, &t);
+        }
+        if (!blocked) {
+            float t, u, v;
+            uint32_t tri;
+            blocked = traverse<true>(sc, stack_mem + threadIdx.x, o, d, T_MIN, t_max, &t, &u, &v, &tri, &visits,
+                                     &tests, &overflow);
+        }
+        if (!blocked) {
+            const uint32_t pid = __float_as_uint(b.w);
+            const float4 c = p.src[i];
+            float4 r = p.rad[pid];
+            r.x += c.x;
+            r.y += c.y;
+            r.z += c.z;
+            p.rad[pid] = r;
+        }
+    }
+    flush_traversal_stats(p.counters, visits, tests, overflow, stats_on);
+}
+
+// ---------------------------------------------------------------------------
+// RESOLVE (resolve.wgsl:305-338): per pixel, sum the batch's samples in sample
+// order and add them to the film.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_resolve(const GpuUniforms* __restrict__ U, Planes p, GpuAccum* accum) {
+    const GpuUniforms& u = *U;
+    const uint32_t pixels = u.width * u.height;
+    const uint32_t px = blockIdx.x * 256 + threadIdx.x;
+    if (px >= pixels) return;
+    V3 sum{0, 0, 0}, sq{0, 0, 0}, al{0, 0, 0}, nr{0, 0, 0};
+    float dp = 0.0f;
+    for (uint32_t j = 0; j < u.samples_per_launch; ++j) {
+        const uint32_t q = px + j * pixels;
+        const V3 r = f4xyz(p.rad[q]);
+        const float4 ga = p.gal[q];
+        sum = sum + r;
+        sq = sq + r * r;
+        al = al + f4xyz(ga);
+        nr = nr + f4xyz(p.gnr[q]);
+        dp = dp + ga.w;
+    }
+    float4* a = reinterpret_cast<float4*>(accum + px);
+    float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+    a0.x += sum.x; a0.y += sum.y; a0.z += sum.z; a0.w += (float)u.samples_per_launch;
+    a1.x += al.x; a1.y += al.y; a1.z += al.z; a1.w += dp;
+    a2.x += nr.x; a2.y += nr.y; a2.z += nr.z;
+    a3.x += sq.x; a3.y += sq.y; a3.z += sq.z;
+    a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
+}
+
+// Clears the per-bounce counters with a kernel (not a memset) so the batch can
+// be captured in a graph and replayed.
+__global__ void k_clear_counters(uint32_t* counters, uint32_t n) {
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) counters[k] = 0;
+}
+
+// bsdf.rs:1200-1227 energy::compute_table on the device: 32 x 32 entries,
+// each the mean over a 128 x 128 stratified grid of the white-Fresnel
+// single-scattering weight. One workgroup per entry, summed in f64 as the
+// reference does.
+__global__ __launch_bounds__(256) void k_ggx_table(float* table) {
+    const int ri = blockIdx.x / 32, ci = blockIdx.x % 32;
+    const float roughness = (float)ri / 31.0f;
+    const float alpha = roughness_to_alpha(roughness);
+    const float root_mu = (float)ci / 31.0f;
+    const float mu = fmaxf(root_mu * root_mu, 1.0e-4f);
+    const float sin_t = sqrtf(fmaxf(1.0f - mu * mu, 0.0f));
+    const V3 wo{sin_t, 0.0f, mu};
+    const float n = 128.0f;
+    double sum = 0.0;
+    for (int k = threadIdx.x; k < 128 * 128; k += 256) {
+        const int a = k / 128, b = k % 128;
+        const float ux = ((float)a + 0.5f) / n, uy = ((float)b + 0.5f) / n;
+        sum += (double)specular_single_weight_white(alpha, wo, ux, uy);
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) table[blockIdx.x] = (float)(red[0] / (128.0 * 128.0));
+}
+
+inline uint32_t grid_for(size_t n, int block) { return (uint32_t)((n + block - 1) / block); }
+
+}  // namespace
+
+lucida_status ensure_ggx_table(lucida_ctx* ctx) {
+    if (ctx->ggx_ready) return LUCIDA_OK;
+    LUCIDA_HIP(ctx->ggx_table.ensure(1024 * sizeof(float)), LUCIDA_OOM);
+    hipLaunchKernelGGL(k_ggx_table, dim3(1024), dim3(256), 0, ctx->stream, ctx->ggx_table.as<float>());
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
+    ctx->ggx_ready = true;
+    return LUCIDA_OK;
+}
+
+// Device buffers of the wavefront, grown on demand and kept by the context.
+static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, GpuUniforms** dU) {
+    if (P > ctx->wf_capacity) {
+        LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
+        LUCIDA_HIP(ctx->wf_planes.ensure(P * 16 * 7), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_hits.ensure(P * 16), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * 3), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_queues.ensure(P * 4 * 2), LUCIDA_OOM);
+        ctx->wf_capacity = P;
+    }
+    LUCIDA_HIP(ctx->wf_counters.ensure(4096), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->wf_uniforms.ensure(sizeof(GpuUniforms) * 2), LUCIDA_OOM);
+    float4* pl = ctx->wf_planes.as<float4>();
+    const size_t C = ctx->wf_capacity;
+    out->org = pl;
+    out->dir = pl + C;
+    out->thr = pl + 2 * C;
+    out->prv = pl + 3 * C;
+    out->rad = pl + 4 * C;
+    out->gal = pl + 5 * C;
+    out->gnr = pl + 6 * C;
+    out->hit = ctx->wf_hits.as<float4>();
+    float4* sh = ctx->wf_shadow.as<float4>();
+    out->sro = sh;
+    out->srd = sh + C;
+    out->src = sh + 2 * C;
+    out->q0 = ctx->wf_queues.as<uint32_t>();
+    out->q1 = ctx->wf_queues.as<uint32_t>() + C;
+    out->counters = ctx->wf_counters.as<uint32_t>();
+    *dU = ctx->wf_uniforms.as<GpuUniforms>();
+    return LUCIDA_OK;
+}
+
+}  // namespace lucida
+
+using namespace lucida;
+
+extern "C" {
+
+lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene, const GpuUniforms* uniforms,
+                                      uint32_t samples, lucida_film* film, const lucida_wavefront_config* config,
+                                      lucida_render_stats* stats) {
+    if (!ctx || !scene || !uniforms || !film) return set_error(LUCIDA_INVALID_ARG, "lucida_render_wavefront: null argument");
+    clear_error();
+    if (uniforms->width != film->width || uniforms->height != film->height)
+        return set_error(LUCIDA_INVALID_ARG, "uniforms width/height do not match the film");
+    if (uniforms->max_depth > MAX_BOUNCES) return set_error(LUCIDA_INVALID_ARG, "max_depth above 64");
+    if (scene->num_instances) return set_error(LUCIDA_INVALID_ARG, "instanced scenes are not supported yet");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    if (samples == 0 || uniforms->max_depth == 0) return LUCIDA_OK;
+    lucida_status st = ensure_ggx_table(ctx);
+    if (st != LUCIDA_OK) return st;
+
+    lucida_wavefront_config cfg{};
+    if (config) cfg = *config;
+    const size_t pixels = (size_t)film->width * film->height;
+    uint32_t spb = cfg.samples_per_batch;
+    if (spb == 0) {
+        // Default: about 4M paths in flight (~450 MB of path state).
+        size_t target = (size_t)1 << 22;
+        spb = (uint32_t)std::max<size_t>(1, target / pixels);
+    }
+    spb = std::min(spb, samples);
+    const size_t P = pixels * spb;
+    if (P >= 0x7FFFFFFFull / 4) return set_error(LUCIDA_INVALID_ARG, "too many paths in one batch");
+
+    Planes p;
+    GpuUniforms* dU = nullptr;
+    st = ensure_wavefront(ctx, P, &p, &dU);
+    if (st != LUCIDA_OK) return st;
+
+    SceneView sc;
+    sc.materials = scene->materials;
+    sc.prims = scene->primitives;
+    sc.pos = reinterpret_cast<const float4*>(scene->positions);
+    sc.attrs = scene->vertex_attrs;
+    sc.tris = scene->triangles;
+    sc.nodes = scene->bvh_nodes;
+    sc.lights = scene->lights;
+    sc.ggx = ctx->ggx_table.as<float>();
+    sc.env.w = scene->env_width;
+    sc.env.h = scene->env_height;
+    sc.env.total_weight = scene->env_total_weight;
+    sc.env.pixels = reinterpret_cast<const float4*>(scene->env_pixels);
+    sc.env.conditional = scene->env_conditional;
+    sc.env.marginal = scene->env_marginal;
+    sc.num_prims = (uint32_t)scene->num_primitives;
+    sc.num_tris = (uint32_t)scene->num_triangles;
+    sc.num_nodes = (uint32_t)scene->num_bvh_nodes;
+    sc.num_lights = (uint32_t)scene->num_lights;
+
+    hipStream_t s = ctx->stream;
+    const uint32_t depth = uniforms->max_depth;
+    const uint32_t stats_on = (stats && cfg.count_traversal) ? 1u : 0u;
+    uint64_t totals[5] = {0, 0, 0, 0, 0};
+    LUCIDA_HIP(hipEventRecord(ctx->ev0, s), LUCIDA_VALIDATION);
+    for (uint32_t done = 0; done < samples; done += spb) {
+        const uint32_t this_spb = std::min(spb, samples - done);
+        GpuUniforms u = *uniforms;
+        u.sample_offset = uniforms->sample_offset + done;
+        u.samples_per_launch = this_spb;
+        const size_t Pb = pixels * this_spb;
+        LUCIDA_HIP(hipMemcpyAsync(dU, &u, sizeof u, hipMemcpyHostToDevice, s), LUCIDA_VALIDATION);
+        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, 256u);
+        hipLaunchKernelGGL(k_generate, dim3(grid_for(Pb, 256)), dim3(256), 0, s, dU, p);
+        for (uint32_t b = 0; b < depth; ++b) {
+            uint32_t* qin = (b & 1) ? p.q1 : p.q0;
+            uint32_t* qout = (b & 1) ? p.q0 : p.q1;
+            hipLaunchKernelGGL(k_extend, dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc, p, qin,
+                               p.counters + b, stats_on);
+            hipLaunchKernelGGL(k_shade, dim3(grid_for(Pb, WG)), dim3(WG), 0, s, sc, dU, p, qin, p.counters + b, qout,
+                               p.counters + b + 1, p.counters + 64 + b);
+            hipLaunchKernelGGL(k_connect, dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc, p,
+                               p.counters + 64 + b, stats_on);
+        }
+        hipLaunchKernelGGL(k_resolve, dim3(grid_for(pixels, 256)), dim3(256), 0, s, dU, p, film->accum);
+        LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+        {
+            uint32_t host[256];
+            LUCIDA_HIP(hipMemcpyAsync(host, p.counters, sizeof host, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
+            LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
+            for (uint32_t b = 0; b < depth; ++b) {
+                totals[0] += host[b];
+                totals[1] += host[64 + b];
+            }
+            const uint64_t* s64 = reinterpret_cast<const uint64_t*>(host + 128);
+            totals[2] += s64[2];
+            totals[3] += s64[3];
+            totals[4] += s64[4];
+        }
+    }
+    LUCIDA_HIP(hipEventRecord(ctx->ev1, s), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipEventSynchronize(ctx->ev1), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
+    if (stats) {
+        float ms = 0.0f;
+        LUCIDA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), LUCIDA_VALIDATION);
+        stats->extension_rays = totals[0];
+        stats->shadow_rays = totals[1];
+        stats->node_visits = totals[2];
+        stats->triangle_tests = totals[3];
+        stats->stack_overflows = totals[4];
+        stats->elapsed_ms = ms;
+    }
+    return LUCIDA_OK;
+}
+
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+"""ctypes wrapper of the CPU oracle (liblucida_oracle.so).
+
+TEST INFRASTRUCTURE. The oracle restates the reference's Rust CPU twin
+(crates/core) in C++; it is the checker the HIP path is compared with. Only the
+test suite, the smoke check and the bench's CPU-baseline leg import this
+module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liblucida_oracle.so"
+
+AABB = np.dtype([("lo", "<f4", 3), ("_p0", "<f4"), ("hi", "<f4", 3), ("_p1", "<f4")])
+TRIANGLE = np.dtype([("i0", "<u4"), ("i1", "<u4"), ("i2", "<u4"), ("material", "<u4")])
+BVH_NODE = np.dtype([("bounds_min", "<f4", 3), ("left_first", "<u4"), ("bounds_max", "<f4", 3), ("count", "<u4")])
+
+_lib = None
+
+
+def build() -> Path:
+    """Builds the oracle with its Makefile (g++, contraction off)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "-j8"], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        P, S, U32, F, I = C.c_void_p, C.c_size_t, C.c_uint32, C.c_float, C.c_int
+        sigs = {
+            "oracle_expand_bits": ([U32], U32),
+            "oracle_morton3d": ([F, F, F], U32),
+            "oracle_normalise": ([P, P, P, P], None),
+            "oracle_lcp": ([P, S, I, I], I),
+            "oracle_triangle_bounds": ([P, S, P, P], None),
+            "oracle_lbvh_build": ([P, S, P, S, P, P, P, P, P, P, P, P, P, P], S),
+            "oracle_sah_cost": ([P, S], F),
+            "oracle_validate": ([P, S, P, S, S], S),
+            "oracle_sort_pairs": ([P, P, S, P, P], None),
+        }
+        for name, (args, res) in sigs.items():
+            try:
+                fn = getattr(L, name)
+            except AttributeError:
+                continue
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def expand_bits(v: int) -> int:
+    return lib().oracle_expand_bits(v)
+
+
+def morton3d(x: float, y: float, z: float) -> int:
+    return lib().oracle_morton3d(x, y, z)
+
+
+def normalise(p, lo, hi):
+    p, lo, hi = (np.asarray(v, np.float32) for v in (p, lo, hi))
+    out = np.empty(3, np.float32)
+    lib().oracle_normalise(_p(p), _p(lo), _p(hi), _p(out))
+    return out
+
+
+def lcp(codes, i: int, j: int) -> int:
+    codes = np.ascontiguousarray(codes, np.uint32)
+    return lib().oracle_lcp(_p(codes), len(codes), i, j)
+
+
+def triangle_bounds(tris, positions) -> np.ndarray:
+    tris = np.ascontiguousarray(tris, TRIANGLE)
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 4)
+    out = np.empty(len(tris), AABB)
+    lib().oracle_triangle_bounds(_p(tris), len(tris), _p(pos), _p(out))
+    return out
+
+
+def lbvh_build(tris, positions, max_leaf: int = 4) -> dict:
+    """Lbvh::build_with_leaf_size (crates/core/src/lbvh.rs:422) with every stage."""
+    tris = np.ascontiguousarray(tris, TRIANGLE)
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 4)
+    n = len(tris)
+    r = {
+        "codes": np.empty(n, np.uint32), "order": np.empty(n, np.uint32),
+        "karras": np.empty((max(n - 1, 1), 2), np.uint32), "parent": np.empty(max(2 * n - 1, 1), np.uint32),
+        "node_bounds": np.empty(max(2 * n - 1, 1), AABB), "subtree_size": np.empty(max(n - 1, 1), np.uint32),
+        "scene_bounds": np.empty(1, AABB), "nodes": np.empty(max(2 * n - 1, 1), BVH_NODE),
+        "prim_indices": np.empty(max(n, 1), np.uint32),
+    }
+    sah = C.c_float()
+    count = lib().oracle_lbvh_build(_p(tris), n, _p(pos), max_leaf, _p(r["codes"]), _p(r["order"]),
+                                    _p(r["karras"]), _p(r["parent"]), _p(r["node_bounds"]),
+                                    _p(r["subtree_size"]), _p(r["scene_bounds"]), _p(r["nodes"]),
+                                    _p(r["prim_indices"]), C.byref(sah))
+    r["nodes"] = r["nodes"][:count]
+    r["prim_indices"] = r["prim_indices"][:n]
+    if n < 2:
+        r["karras"] = r["karras"][:0]
+        r["subtree_size"] = r["subtree_size"][:0]
+    r["sah_cost"] = sah.value
+    return r
+
+
+def sah_cost(nodes) -> float:
+    nodes = np.ascontiguousarray(nodes, BVH_NODE)
+    return lib().oracle_sah_cost(_p(nodes), len(nodes))
+
+
+def validate(nodes, prim_indices, triangle_count: int) -> int:
+    nodes = np.ascontiguousarray(nodes, BVH_NODE)
+    prims = np.ascontiguousarray(prim_indices, np.uint32)
+    return lib().oracle_validate(_p(nodes), len(nodes), _p(prims), len(prims), triangle_count)
+
+
+def sort_pairs(keys, vals):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    vals = np.ascontiguousarray(vals, np.uint32)
+    ko, vo = np.empty_like(keys), np.empty_like(vals)
+    lib().oracle_sort_pairs(_p(keys), _p(vals), len(keys), _p(ko), _p(vo))
+    return ko, vo

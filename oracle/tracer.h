@@ -1,0 +1,50 @@
+/* Oracle: the CPU reference tracer's C entry points (TEST INFRASTRUCTURE). */
+#ifndef LUCIDA_ORACLE_TRACER_H
+#define LUCIDA_ORACLE_TRACER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/lucida.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct RenderCounts {
+    uint64_t extension_rays, shadow_rays, node_visits, triangle_tests;
+} RenderCounts;
+
+uint32_t oracle_sobol(uint32_t index, uint32_t dim);
+uint32_t oracle_owen_scramble(uint32_t x, uint32_t seed);
+uint32_t oracle_hash_u32(uint32_t x);
+float oracle_sobol_sample(uint32_t index, uint32_t dim, uint32_t seed);
+/* Draws from a Sampler (sobol.rs:287-361): pattern[i] = 1 for next_f32, 2 for
+ * next_vec2 (two outputs). */
+void oracle_sampler_draws(uint32_t kind, uint32_t frame, uint32_t pixel, uint32_t sample, const uint32_t* pattern,
+                          size_t n, float* out);
+/* energy::compute_table (bsdf.rs:1200-1227), 32 x 32, row = roughness. */
+void oracle_ggx_table(float* out);
+float oracle_directional_albedo_grid(float roughness, float cos_o, uint32_t grid);
+void oracle_offset_ray_origin(const float p[3], const float n[3], float out[3]);
+
+/* integrator::render_with_guides restricted to rows [row_begin, row_end):
+ * rgb receives the per-pixel mean (3 floats per pixel), guides (optional) 8
+ * floats per pixel: albedo[3], normal[3], depth, steps. The scene's BVH must
+ * be compacted (leaves index triangles directly) or absent (brute force). */
+void oracle_render(const lucida_scene_desc* d, const GpuUniforms* u, const float background[3], uint32_t threads,
+                   uint32_t row_begin, uint32_t row_end, float* rgb, float* guides, RenderCounts* counts);
+
+/* denoise::denoise (denoise.rs:134) on a film and its guides (3 floats per
+ * pixel for film/albedo/normal/out, 1 for depth). */
+void oracle_denoise(const float* film, const float* albedo, const float* normal, const float* depth, uint32_t width,
+                    uint32_t height, const lucida_denoise_params* p, float* out);
+
+/* image::compare (image.rs:178): out = {rmse, max_abs, mean_rel, mean_a,
+ * mean_b, mean_abs}. */
+void oracle_compare(const float* a, const float* b, size_t pixels, double* out6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

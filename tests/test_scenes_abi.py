@@ -1,0 +1,132 @@
+"""Host-side inputs and the C ABI, on the CPU.
+
+The scene generators are checked byte for byte against the packed scenes the
+reference ships (web/public/scenes/*.bin, via the fixtures in tests/golden),
+and the camera against the reference's CAMERA_FIXTURES
+(web/src/generated/scenes.ts:283). The library must load and export every
+entry point the headers declare; no GPU call is made here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import json
+
+import numpy as np
+import pytest
+
+import lucida_amd as L
+from conftest import GOLDEN
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    names = L.exported_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_no_gpu_is_reported_as_no_adapter_or_works():
+    # Without a GPU, Gpu() must fail loudly (NoAdapter), never fall back.
+    import os
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible here")
+    with pytest.raises(L.GpuError) as e:
+        L.Gpu(0)
+    assert e.value.kind in ("NoAdapter", "NoDevice")
+
+
+def test_cornell_box_matches_reference_asset():
+    ref = json.loads((GOLDEN / "cornell_box_sections.json").read_text())
+    a = L.Blob("cornell-box").arrays()
+    assert a["materials"].view(np.uint32).ravel().tolist() == ref["materials"]
+    assert a["primitives"].view(np.uint32).ravel().tolist() == ref["primitives"]
+    assert a["lights"].view(np.uint32).ravel().tolist() == ref["lights"]
+
+
+def test_cornell_mesh_matches_reference_asset():
+    ref = json.loads((GOLDEN / "cornell_mesh_sections.json").read_text())
+    a = L.Blob("cornell-mesh").arrays()
+    assert len(a["positions"]) == ref["counts"]["positions"]
+    assert len(a["triangles"]) == ref["counts"]["triangles"]
+    sha = lambda arr: hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest()
+    assert sha(a["materials"]) == ref["sha256_materials"]
+    assert sha(a["positions"]) == ref["sha256_positions"]
+    # Vertex normals are exact; the uv channel goes through atan2/asin, whose
+    # last bits depend on the platform libm the asset was generated with (the
+    # reference's was Apple's), so the attribute section is compared with the
+    # uvs rounded off. uv is not read by any shading on this path.
+    va = a["vertex_attrs"].copy()
+    ref_uv_free = ref.get("sha256_vertexAttrs_no_uv")
+    if ref_uv_free:
+        va["uv"] = 0
+        assert sha(va) == ref_uv_free
+    s = ref["uv_samples"]
+    np.testing.assert_allclose(a["vertex_attrs"]["uv"][s["index"]], np.array(s["values"], np.float32), atol=2e-7)
+    tris = a["triangles"].view(np.uint32).reshape(-1, 4)
+    order = np.lexsort(tris.T[::-1])
+    assert sha(tris[order]) == ref["sha256_triangles_sorted"]
+    bits = a["positions"].view(np.uint32)[ref["position_samples"]["index"]]
+    assert bits.tolist() == ref["position_samples"]["bits"]
+    # Two triangle lights from the emissive quad; the reference lists them in
+    # its BVH leaf order, so either order of the pair is accepted.
+    assert len(a["lights"]) == ref["counts"]["lights"] == 2
+    assert ref["sha256_lights"] in (sha(a["lights"]), sha(a["lights"][::-1]))
+
+
+@pytest.mark.parametrize("idx", range(7))
+def test_camera_matches_reference_fixtures(idx):
+    cams = json.loads((GOLDEN / "camera_fixtures.json").read_text())
+    c = cams[idx]
+    if c["aperture"] != 0 or c["focus_distance"] != pytest.approx(
+            float(np.linalg.norm(np.subtract(c["look_at"], c["eye"]))), rel=1e-6):
+        pytest.skip("fixture uses a focus distance the look_at constructor does not produce")
+    # Build a one-off blob with this camera through the generic icosphere scene
+    # and overwrite its camera via the uniforms path is not exposed; instead
+    # compare with a direct restatement through the library's cornell camera
+    # when the fixture is the Cornell one.
+    if c["label"] not in ("cornell", "cornell-wide", "cornell-tall"):
+        pytest.skip("the library exposes the Cornell camera only through cornell-box")
+    u = L.Blob("cornell-box").uniforms(c["width"], c["height"])
+    for key, field in (("cam_origin", "cam_origin"), ("cam_upper_left", "cam_upper_left"),
+                       ("cam_horizontal", "cam_horizontal"), ("cam_vertical", "cam_vertical")):
+        got = np.array(getattr(u, field)[:], np.float32)
+        exp = np.array(c[key], np.float32)
+        # tan() goes through the platform libm (the fixtures were made with
+        # Apple's): agreement to 2 ulp, not bits.
+        np.testing.assert_allclose(got, exp, rtol=2.5e-7, atol=0)
+
+
+def test_uniform_counts_follow_the_blob():
+    b = L.Blob("box-c1")
+    a = b.arrays()
+    u = b.uniforms(256, 256, samples=64)
+    assert u.num_primitives == len(a["primitives"]) == 3
+    assert u.num_triangles == len(a["triangles"]) == 10
+    assert u.num_lights == 1 and a["lights"][0]["area"] == np.float32(0.0625)
+    assert u.width == 256 and u.samples_per_launch == 64
+
+
+def test_c2_scene_sizes():
+    b = L.Blob("sphere-c2", frequency=16)
+    a = b.arrays()
+    assert len(a["triangles"]) == 20 * 16 * 16 + 1024
+    assert len(a["positions"]) == 10 * 16 * 16 + 2 + 512 * 4
+    assert len(a["lights"]) == 16
+    # every vertex of the sphere lies within the displacement band
+    sphere = a["positions"][512 * 4:, :3]
+    r = np.linalg.norm(sphere, axis=1)
+    assert r.min() > 0.94 and r.max() < 1.06
+    # closed surface: every edge shared by exactly two sphere triangles
+    t = a["triangles"][1024:]
+    e = np.stack([np.sort(np.stack([t["i0"], t["i1"]], 1), 1), np.sort(np.stack([t["i1"], t["i2"]], 1), 1),
+                  np.sort(np.stack([t["i2"], t["i0"]], 1), 1)]).reshape(-1, 2)
+    _, counts = np.unique(e, axis=0, return_counts=True)
+    assert np.all(counts == 2)
+
+
+def test_unknown_scene_is_an_error():
+    with pytest.raises(L.GpuError) as e:
+        L.Blob("no-such-scene")
+    assert e.value.kind == "InvalidArg"
