@@ -1,0 +1,218 @@
+"""Benchmark: the wavefront path tracer on the 1.05M-triangle scene.
+
+Workload (one "step"): render the noise-displaced geodesic sphere (1,048,820
+triangles) in its 1,024-triangle room lit by 16 emissive quads, at 1024x1024,
+16 samples per pixel, 8 bounces, NEE + MIS, Owen-scrambled Sobol, into a
+cleared film. The scene, its GPU-built LBVH and the film are resident in HBM
+before timing starts. value = paths per second over the timed steps.
+
+Prints one JSON line. Extra figures (Mrays/s, LBVH Mtris/s, a-trous time,
+per-kernel times) ride in "extra".
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--cpu-seconds 15] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mpaths/s @1M tris 8 bounces; time-to-relMSE 0.01: denoised vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def lbvh_build_rate(L, gpu, blob, runs: int = 5) -> dict:
+    """Device LBVH build of the scene's triangles (inputs already on the GPU)."""
+    a = blob.arrays()
+    tris = np.ascontiguousarray(a["triangles"])
+    pos = np.ascontiguousarray(a["positions"])
+    n = len(tris)
+    lib = L.lib()
+    ptrs = {}
+    sizes = {"tris": tris.nbytes, "pos": pos.nbytes, "nodes": (2 * n - 1) * 32, "prims": n * 4,
+             "sorted": tris.nbytes}
+    for k, b in sizes.items():
+        p = C.c_void_p()
+        L._check(lib.lucida_device_alloc(gpu._h, b, C.byref(p)))
+        ptrs[k] = p
+    L._check(lib.lucida_copy_to_device(gpu._h, ptrs["tris"], tris.ctypes.data, tris.nbytes))
+    L._check(lib.lucida_copy_to_device(gpu._h, ptrs["pos"], pos.ctypes.data, pos.nbytes))
+    times, stages = [], []
+    count = C.c_size_t()
+    for i in range(runs + 1):
+        st = np.zeros(5, np.float32)
+        gpu.check_errors()
+        t0 = time.perf_counter()
+        L._check(lib.lucida_lbvh_build_device(gpu._h, ptrs["tris"], n, ptrs["pos"], ptrs["nodes"], C.byref(count),
+                                              ptrs["prims"], ptrs["sorted"], st.ctypes.data))
+        gpu.check_errors()
+        t1 = time.perf_counter()
+        if i > 0:
+            times.append(t1 - t0)
+            stages.append(st.copy())
+    for p in ptrs.values():
+        lib.lucida_device_free(gpu._h, p)
+    med = float(np.median(times))
+    st = np.median(np.stack(stages), axis=0)
+    device_us = float(st.sum())
+    return {"triangles": n, "nodes": int(count.value), "wall_ms": med * 1e3, "device_us": device_us,
+            "stage_us": dict(zip(("prepare", "sort", "karras", "fit", "relayout"), [round(float(x), 1) for x in st])),
+            "mtris_per_s": n / (device_us * 1e-6) / 1e6, "mtris_per_s_wall": n / med / 1e6,
+            "bytes_per_tri": 436}
+
+
+def cpu_baseline(blob, u, target_s: float) -> dict:
+    """The CPU oracle (a C++ restatement of the reference tracer) on a band of
+    rows of the same workload, threads = the cores this process may use."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    threads = len(os.sched_getaffinity(0))
+    threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+    desc = blob.desc()
+    rows = 4
+    t0 = time.perf_counter()
+    pyoracle.render(desc, u, threads=threads, rows=(0, rows), guides=False)
+    dt = time.perf_counter() - t0
+    # scale the band so the timed sample takes about target_s
+    rows = int(max(4, min(u.height, rows * target_s / max(dt, 1e-3))))
+    r0 = (u.height - rows) // 2
+    t0 = time.perf_counter()
+    pyoracle.render(desc, u, threads=threads, rows=(r0, r0 + rows), guides=False)
+    dt = time.perf_counter() - t0
+    paths = rows * u.width * u.samples_per_launch
+    return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"rows {r0}..{r0 + rows} of the {u.width}x{u.height} image at {u.samples_per_launch} spp, "
+                      f"{u.max_depth} bounces ({paths} paths, {dt:.1f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--height", type=int, default=1024)
+    ap.add_argument("--spp", type=int, default=16)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--frequency", type=int, default=229)
+    ap.add_argument("--samples-per-batch", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              f"(the path shards by sample range with one Accum reduction, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import lucida_amd as L
+
+    gpu = L.Gpu(0)
+    dev_name, cus = gpu.info()
+    blob = L.Blob("sphere-c2", frequency=a.frequency)
+    ntris = blob.desc().num_triangles
+    lbvh = lbvh_build_rate(L, gpu, blob)
+    scene = L.Scene(gpu, blob, build_lbvh=True)
+    nodes, tris_sorted = scene.download_bvh()
+    blob.set_bvh(nodes, tris_sorted)  # the CPU baseline traverses the same tree
+    film = L.Film(gpu, a.width, a.height)
+    u = blob.uniforms(a.width, a.height, samples=a.spp, max_depth=a.depth, sampling=L.SAMPLING_MIS,
+                      sampler=L.SAMPLER_SOBOL)
+    cfg = L.WavefrontConfig(a.samples_per_batch, 0, 0, 0, 1, 1)
+
+    for _ in range(a.warmup):
+        film.clear()
+        L.render_wavefront(gpu, scene, u, a.spp, film, cfg)
+    gpu.check_errors()
+
+    agg = {k: 0.0 for k in L.RenderStats.KERNELS}
+    launches = {k: 0 for k in L.RenderStats.KERNELS}
+    ext_rays = shadow_rays = ext_visits = ext_tests = visits = tests = 0
+    device_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        film.clear()
+        st = L.render_wavefront(gpu, scene, u, a.spp, film, cfg)
+        for i, k in enumerate(L.RenderStats.KERNELS):
+            agg[k] += st.kernel_ms[i]
+            launches[k] += st.kernel_launches[i]
+        ext_rays += st.extension_rays
+        shadow_rays += st.shadow_rays
+        ext_visits += st.extend_node_visits
+        ext_tests += st.extend_triangle_tests
+        visits += st.node_visits
+        tests += st.triangle_tests
+        device_ms += st.elapsed_ms
+        if st.stack_overflows:
+            print(f"warning: {st.stack_overflows} traversal stack overflows", file=sys.stderr)
+    gpu.check_errors()
+    t1 = time.perf_counter()
+    sec = (t1 - t0) / a.steps
+    paths = a.width * a.height * a.spp
+    value = paths / sec / 1e6
+
+    # Roofline of the dominant kernel. EXTEND's algorithmic bytes per ray:
+    # queue index 4 + origin/direction planes 32 + hit write 16, plus 64 per
+    # node visit (the two 32-byte children) and 64 per triangle test (16-byte
+    # triangle + three 16-byte positions).
+    ext_bytes = ext_rays * (4 + 32 + 16) + 64 * (ext_visits + ext_tests)
+    ext_ms = agg["extend"]
+    dominant = max(agg, key=agg.get)
+    achieved = ext_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+
+    # a-trous on the final film (1024^2)
+    den = L.Film(gpu, a.width, a.height)
+    dn = L.Denoiser(gpu)
+    dn.run(film, den)
+    gpu.check_errors()
+    td = time.perf_counter()
+    for _ in range(5):
+        dn.run(film, den)
+    gpu.check_errors()
+    atrous_ms = (time.perf_counter() - td) / 5 * 1e3
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(sec * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scene, seeded)",
+        "config": {"workload": f"sphere-c2: displaced geodesic sphere {ntris - 1024} tris + 1,024-tri room, "
+                               f"16 emissive quads, {a.width}x{a.height}, {a.spp} spp, {a.depth} bounces, "
+                               f"NEE+MIS, Sobol", "triangles": ntris, "width": a.width, "height": a.height,
+                   "spp": a.spp, "max_depth": a.depth, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "extend (closest-hit traversal)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "avg_launch_ms": round(ext_ms / max(launches["extend"], 1), 4),
+                     "bytes_per_launch": int(ext_bytes / max(launches["extend"], 1))},
+        "extra": {
+            "device": dev_name, "compute_units": cus,
+            "mrays_per_s": round((ext_rays + shadow_rays) / a.steps / sec / 1e6, 2),
+            "extension_rays_per_step": ext_rays // a.steps, "shadow_rays_per_step": shadow_rays // a.steps,
+            "node_visits_per_ray": round(visits / max(ext_rays + shadow_rays, 1), 2),
+            "triangle_tests_per_ray": round(tests / max(ext_rays + shadow_rays, 1), 2),
+            "kernel_ms_per_step": {k: round(v / a.steps, 3) for k, v in agg.items()},
+            "dominant_kernel": dominant,
+            "device_ms_per_step": round(device_ms / a.steps, 3),
+            "lbvh": lbvh,
+            "atrous_ms": round(atrous_ms, 3),
+        },
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(blob, u, a.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    gpu.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

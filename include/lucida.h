@@ -375,12 +375,18 @@ void* lucida_film_device_ptr(lucida_film* film);
 
 /* Counters a render reports, all over the whole call. */
 typedef struct lucida_render_stats {
-    uint64_t extension_rays; /* closest-hit rays traced (EXTEND)   */
-    uint64_t shadow_rays;    /* any-hit rays traced (CONNECT)      */
-    uint64_t node_visits;    /* BVH node visits, both ray kinds    */
-    uint64_t triangle_tests; /* triangle tests, both ray kinds     */
-    uint64_t stack_overflows;/* traversal pushes dropped for a full stack (0 expected) */
-    float elapsed_ms;        /* device time of the render, HIP events */
+    uint64_t extension_rays;        /* closest-hit rays traced (EXTEND)          */
+    uint64_t shadow_rays;           /* any-hit rays traced (CONNECT)             */
+    uint64_t node_visits;           /* BVH node visits, both ray kinds           */
+    uint64_t triangle_tests;        /* triangle tests, both ray kinds            */
+    uint64_t extend_node_visits;    /* of which closest-hit                      */
+    uint64_t extend_triangle_tests; /* of which closest-hit                      */
+    uint64_t stack_overflows;       /* pushes dropped for a full stack (0 expected) */
+    float elapsed_ms;               /* device time of the whole render (HIP events) */
+    /* Per-kernel device time and launch count when config->profile_kernels is
+     * set, in the order GENERATE, EXTEND, SHADE, CONNECT, RESOLVE. */
+    float kernel_ms[5];
+    uint32_t kernel_launches[5];
 } lucida_render_stats;
 
 /* Tunables of the wavefront. Zero fields take the defaults. */
@@ -390,6 +396,7 @@ typedef struct lucida_wavefront_config {
     uint32_t blocks_per_cu;     /* persistent workgroups per compute unit     */
     uint32_t use_graph;         /* capture the bounce loop in a hipGraph      */
     uint32_t count_traversal;   /* keep node/triangle counters (costs a little) */
+    uint32_t profile_kernels;   /* time every kernel with HIP events          */
 } lucida_wavefront_config;
 
 /* wavefront::render_native_on (crates/gpu/src/wavefront.rs:134): renders
@@ -400,6 +407,11 @@ typedef struct lucida_wavefront_config {
 lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene, const GpuUniforms* uniforms,
                                       uint32_t samples, lucida_film* film, const lucida_wavefront_config* config,
                                       lucida_render_stats* stats);
+
+/* The GGX directional-albedo table the BSDF's energy compensation reads
+ * (energy::compute_table, crates/core/src/bsdf.rs:1200-1227), built on the
+ * device at first use: 32 x 32 floats, row = roughness, column = sqrt(cos). */
+lucida_status lucida_energy_table(lucida_ctx* ctx, float* out_1024);
 
 /* ------------------------------------------------------------------------ */
 /* À-trous denoiser (crates/gpu/src/denoise.rs)                             */

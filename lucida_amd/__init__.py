@@ -75,13 +75,18 @@ class LbvhParts(C.Structure):
 
 
 class RenderStats(C.Structure):
+    """lucida_render_stats; kernel order GENERATE, EXTEND, SHADE, CONNECT, RESOLVE."""
     _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64),
-                ("triangle_tests", C.c_uint64), ("stack_overflows", C.c_uint64), ("elapsed_ms", C.c_float)]
+                ("triangle_tests", C.c_uint64), ("extend_node_visits", C.c_uint64),
+                ("extend_triangle_tests", C.c_uint64), ("stack_overflows", C.c_uint64), ("elapsed_ms", C.c_float),
+                ("kernel_ms", C.c_float * 5), ("kernel_launches", C.c_uint32 * 5)]
+
+    KERNELS = ("generate", "extend", "shade", "connect", "resolve")
 
 
 class WavefrontConfig(C.Structure):
     _fields_ = [("samples_per_batch", C.c_uint32), ("chunk", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("use_graph", C.c_uint32), ("count_traversal", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("count_traversal", C.c_uint32), ("profile_kernels", C.c_uint32)]
 
 
 class DenoiseParams(C.Structure):
@@ -155,6 +160,7 @@ def lib() -> C.CDLL:
             "lucida_render_wavefront": ([P, P, C.POINTER(Uniforms), U32, P, C.POINTER(WavefrontConfig),
                                          C.POINTER(RenderStats)], I),
             "lucida_denoise_atrous": ([P, P, C.POINTER(DenoiseParams), P], I),
+            "lucida_energy_table": ([P, P], I),
             "lucida_blob_create": ([C.c_char_p, C.POINTER(BlobParams), C.POINTER(P)], I),
             "lucida_blob_destroy": ([P], None),
             "lucida_blob_desc": ([P, C.POINTER(SceneDesc)], I),
@@ -419,6 +425,13 @@ def render_wavefront(gpu: Gpu, scene: Scene, uniforms: Uniforms, samples: int, f
     _check(lib().lucida_render_wavefront(gpu._h, scene._h, C.byref(uniforms), samples, film._h, cfg,
                                          C.byref(st)))
     return st
+
+
+def energy_table(gpu: Gpu) -> np.ndarray:
+    """The device's GGX directional-albedo table (bsdf.rs:1200-1227), 1024 floats."""
+    out = np.empty(1024, np.float32)
+    _check(lib().lucida_energy_table(gpu._h, _ptr(out)))
+    return out
 
 
 def film_mean(accum: np.ndarray, width: int, height: int) -> np.ndarray:

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "context.hpp"
 #include "shading.hpp"
@@ -40,6 +41,13 @@ constexpr int WG = 64;          // one wave per workgroup for the per-path kerne
 constexpr int TRACE_WG = 128;   // traversal workgroups (two waves)
 constexpr int STACK = 32;       // traversal stack entries per lane (shaders/common/bvh.wgsl:133)
 constexpr int MAX_BOUNCES = 64;
+// Counter words: queue length per bounce at [0, 72) (bounce b+1's count is
+// written by bounce b's SHADE), shadow-ray count per bounce at [SHADOW_AT,
+// +64), this batch's traversal counters (u64) at STATS_AT, the render's
+// running totals (u64) at TOTALS_AT.
+constexpr uint32_t SHADOW_AT = 80;
+constexpr uint32_t STATS_AT = 160;
+constexpr uint32_t TOTALS_AT = 256;
 
 // Per-render device state. Planes are float4 arrays of P entries.
 struct Planes {
@@ -56,8 +64,7 @@ struct Planes {
     float4* src;   // shadow: contribution.xyz, unused
     uint32_t* q0;
     uint32_t* q1;
-    uint32_t* counters;  // [0..MAX_BOUNCES): queue length per bounce; [64..128): shadow count per bounce;
-                         // [128..136): u64 stats (ext rays, shadow rays, node visits, tri tests)
+    uint32_t* counters;  // see SHADOW_AT / STATS_AT / TOTALS_AT
 };
 
 struct SceneView {
@@ -99,15 +106,18 @@ __device__ inline uint64_t wave_sum_u32(uint32_t v) {
 
 // Per-wave totals of the traversal counters, one atomic each. Overflows are
 // always counted; visits and tests only when asked for.
+// Per-batch counters live at word STATS_AT as u64: closest-hit visits and
+// tests, any-hit visits and tests, stack overflows. `slot` is 0 for EXTEND and
+// 2 for CONNECT.
 __device__ inline void flush_traversal_stats(uint32_t* counters, uint32_t visits, uint32_t tests, uint32_t overflow,
-                                             uint32_t stats_on) {
-    uint64_t* s64 = reinterpret_cast<uint64_t*>(counters + 128);
+                                             uint32_t stats_on, int slot) {
+    uint64_t* s64 = reinterpret_cast<uint64_t*>(counters + STATS_AT);
     const uint64_t so = wave_sum_u32(overflow);
     if (stats_on) {
         const uint64_t sv = wave_sum_u32(visits), st = wave_sum_u32(tests);
         if ((threadIdx.x & 63) == 0) {
-            if (sv) atomicAdd((unsigned long long*)&s64[2], (unsigned long long)sv);
-            if (st) atomicAdd((unsigned long long*)&s64[3], (unsigned long long)st);
+            if (sv) atomicAdd((unsigned long long*)&s64[slot], (unsigned long long)sv);
+            if (st) atomicAdd((unsigned long long*)&s64[slot + 1], (unsigned long long)st);
         }
     }
     if ((threadIdx.x & 63) == 0 && so) atomicAdd((unsigned long long*)&s64[4], (unsigned long long)so);
@@ -233,7 +243,10 @@ __device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk,
     uint32_t cur = pack_child(root.left_first, root.count);
     int sp = 0;
     const int lane_stride = TRACE_WG;
-    for (;;) {
+    // A tree visits each node at most once per ray; the cap only guarantees
+    // termination on a malformed (cyclic) node array.
+    const uint32_t cap = 2u * sc.num_nodes + 8u;
+    for (uint32_t iter = 0; iter < cap; ++iter) {
         *visits += 1;
         const uint32_t first = cur >> 3, count = cur & 7u;
         if (count > 0) {
@@ -318,7 +331,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc, Planes p, con
         else out = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(MISS));
         p.hit[i] = out;
     }
-    flush_traversal_stats(p.counters, visits, tests, overflow, stats_on);
+    flush_traversal_stats(p.counters, visits, tests, overflow, stats_on, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -573,7 +586,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc, Planes p, co
             p.rad[pid] = r;
         }
     }
-    flush_traversal_stats(p.counters, visits, tests, overflow, stats_on);
+    flush_traversal_stats(p.counters, visits, tests, overflow, stats_on, 2);
 }
 
 // ---------------------------------------------------------------------------
@@ -604,6 +617,23 @@ __global__ __launch_bounds__(256) void k_resolve(const GpuUniforms* __restrict__
     a2.x += nr.x; a2.y += nr.y; a2.z += nr.z;
     a3.x += sq.x; a3.y += sq.y; a3.z += sq.z;
     a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
+}
+
+// Adds one batch's counters into the render's running totals (u64 at word
+// TOTALS_AT): extension rays, shadow rays, closest-hit visits and tests,
+// any-hit visits and tests, stack overflows.
+__global__ void k_fold_counters(uint32_t* counters, uint32_t depth) {
+    if (threadIdx.x != 0) return;
+    uint64_t* tot = reinterpret_cast<uint64_t*>(counters + TOTALS_AT);
+    const uint64_t* batch = reinterpret_cast<const uint64_t*>(counters + STATS_AT);
+    uint64_t ext = 0, sh = 0;
+    for (uint32_t b = 0; b < depth; ++b) {
+        ext += counters[b];
+        sh += counters[SHADOW_AT + b];
+    }
+    tot[0] += ext;
+    tot[1] += sh;
+    for (int k = 0; k < 5; ++k) tot[2 + k] += batch[k];
 }
 
 // Clears the per-bounce counters with a kernel (not a memset) so the batch can
@@ -694,6 +724,16 @@ using namespace lucida;
 
 extern "C" {
 
+lucida_status lucida_energy_table(lucida_ctx* ctx, float* out) {
+    if (!ctx || !out) return set_error(LUCIDA_INVALID_ARG, "lucida_energy_table: null argument");
+    lucida_status st = ensure_ggx_table(ctx);
+    if (st != LUCIDA_OK) return st;
+    LUCIDA_HIP(hipMemcpyAsync(out, ctx->ggx_table.ptr, 1024 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+               LUCIDA_READBACK);
+    LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_READBACK);
+    return LUCIDA_OK;
+}
+
 lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene, const GpuUniforms* uniforms,
                                       uint32_t samples, lucida_film* film, const lucida_wavefront_config* config,
                                       lucida_render_stats* stats) {
@@ -703,6 +743,8 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         return set_error(LUCIDA_INVALID_ARG, "uniforms width/height do not match the film");
     if (uniforms->max_depth > MAX_BOUNCES) return set_error(LUCIDA_INVALID_ARG, "max_depth above 64");
     if (scene->num_instances) return set_error(LUCIDA_INVALID_ARG, "instanced scenes are not supported yet");
+    if (scene->num_bvh_nodes >= (1u << 28) || scene->num_triangles >= (1u << 28))
+        return set_error(LUCIDA_INVALID_ARG, "scene too large: node and triangle indices are packed in 29 bits");
     if (stats) std::memset(stats, 0, sizeof *stats);
     if (samples == 0 || uniforms->max_depth == 0) return LUCIDA_OK;
     lucida_status st = ensure_ggx_table(ctx);
@@ -749,8 +791,27 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     hipStream_t s = ctx->stream;
     const uint32_t depth = uniforms->max_depth;
     const uint32_t stats_on = (stats && cfg.count_traversal) ? 1u : 0u;
-    uint64_t totals[5] = {0, 0, 0, 0, 0};
+    const bool profile = stats && cfg.profile_kernels;
+    const uint32_t batches = (samples + spb - 1) / spb;
+
+    // Optional per-kernel events: one start/stop pair per launch.
+    std::vector<hipEvent_t> evs;
+    std::vector<int> ev_kind;
+    if (profile) {
+        const size_t launches = (size_t)batches * (2 + 3 * depth);
+        evs.resize(2 * launches, nullptr);
+        for (auto& e : evs) LUCIDA_HIP(hipEventCreate(&e), LUCIDA_VALIDATION);
+        ev_kind.reserve(launches);
+    }
+    size_t ev_next = 0;
+    auto mark = [&](int kind, bool start) {
+        if (!profile) return;
+        if (start) ev_kind.push_back(kind);
+        (void)hipEventRecord(evs[ev_next++], s);
+    };
+
     LUCIDA_HIP(hipEventRecord(ctx->ev0, s), LUCIDA_VALIDATION);
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT + 16u);
     for (uint32_t done = 0; done < samples; done += spb) {
         const uint32_t this_spb = std::min(spb, samples - done);
         GpuUniforms u = *uniforms;
@@ -758,47 +819,56 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         u.samples_per_launch = this_spb;
         const size_t Pb = pixels * this_spb;
         LUCIDA_HIP(hipMemcpyAsync(dU, &u, sizeof u, hipMemcpyHostToDevice, s), LUCIDA_VALIDATION);
-        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, 256u);
+        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT);
+        mark(0, true);
         hipLaunchKernelGGL(k_generate, dim3(grid_for(Pb, 256)), dim3(256), 0, s, dU, p);
+        mark(0, false);
         for (uint32_t b = 0; b < depth; ++b) {
             uint32_t* qin = (b & 1) ? p.q1 : p.q0;
             uint32_t* qout = (b & 1) ? p.q0 : p.q1;
+            mark(1, true);
             hipLaunchKernelGGL(k_extend, dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc, p, qin,
                                p.counters + b, stats_on);
+            mark(1, false);
+            mark(2, true);
             hipLaunchKernelGGL(k_shade, dim3(grid_for(Pb, WG)), dim3(WG), 0, s, sc, dU, p, qin, p.counters + b, qout,
-                               p.counters + b + 1, p.counters + 64 + b);
+                               p.counters + b + 1, p.counters + SHADOW_AT + b);
+            mark(2, false);
+            mark(3, true);
             hipLaunchKernelGGL(k_connect, dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc, p,
-                               p.counters + 64 + b, stats_on);
+                               p.counters + SHADOW_AT + b, stats_on);
+            mark(3, false);
         }
+        mark(4, true);
         hipLaunchKernelGGL(k_resolve, dim3(grid_for(pixels, 256)), dim3(256), 0, s, dU, p, film->accum);
+        mark(4, false);
+        hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, depth);
         LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
-        {
-            uint32_t host[256];
-            LUCIDA_HIP(hipMemcpyAsync(host, p.counters, sizeof host, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
-            LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
-            for (uint32_t b = 0; b < depth; ++b) {
-                totals[0] += host[b];
-                totals[1] += host[64 + b];
-            }
-            const uint64_t* s64 = reinterpret_cast<const uint64_t*>(host + 128);
-            totals[2] += s64[2];
-            totals[3] += s64[3];
-            totals[4] += s64[4];
-        }
     }
     LUCIDA_HIP(hipEventRecord(ctx->ev1, s), LUCIDA_VALIDATION);
     LUCIDA_HIP(hipEventSynchronize(ctx->ev1), LUCIDA_VALIDATION);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
     if (stats) {
+        uint64_t totals[8];
+        LUCIDA_HIP(hipMemcpy(totals, p.counters + TOTALS_AT, sizeof totals, hipMemcpyDeviceToHost), LUCIDA_READBACK);
         float ms = 0.0f;
         LUCIDA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), LUCIDA_VALIDATION);
         stats->extension_rays = totals[0];
         stats->shadow_rays = totals[1];
-        stats->node_visits = totals[2];
-        stats->triangle_tests = totals[3];
-        stats->stack_overflows = totals[4];
+        stats->extend_node_visits = totals[2];
+        stats->extend_triangle_tests = totals[3];
+        stats->node_visits = totals[2] + totals[4];
+        stats->triangle_tests = totals[3] + totals[5];
+        stats->stack_overflows = totals[6];
         stats->elapsed_ms = ms;
+        for (size_t k = 0; k < ev_kind.size(); ++k) {
+            float t = 0.0f;
+            LUCIDA_HIP(hipEventElapsedTime(&t, evs[2 * k], evs[2 * k + 1]), LUCIDA_VALIDATION);
+            stats->kernel_ms[ev_kind[k]] += t;
+            stats->kernel_launches[ev_kind[k]] += 1;
+        }
     }
+    for (auto& e : evs) (void)hipEventDestroy(e);
     return LUCIDA_OK;
 }
 

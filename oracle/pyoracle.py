@@ -46,6 +46,20 @@ def lib() -> C.CDLL:
             "oracle_sah_cost": ([P, S], F),
             "oracle_validate": ([P, S, P, S, S], S),
             "oracle_sort_pairs": ([P, P, S, P, P], None),
+            "oracle_sobol": ([U32, U32], U32),
+            "oracle_owen_scramble": ([U32, U32], U32),
+            "oracle_hash_u32": ([U32], U32),
+            "oracle_sobol_sample": ([U32, U32, U32], F),
+            "oracle_sampler_draws": ([U32, U32, U32, U32, P, S, P], None),
+            "oracle_ggx_table": ([P], None),
+            "oracle_directional_albedo_grid": ([F, F, U32], F),
+            "oracle_offset_ray_origin": ([P, P, P], None),
+            "oracle_render": ([P, P, P, U32, U32, U32, P, P, P], None),
+            "oracle_denoise": ([P, P, P, P, U32, U32, P, P], None),
+            "oracle_compare": ([P, P, S, P], None),
+            "oracle_bsdf_sample": ([P, U32, P, F, F, F, P], F),
+            "oracle_bsdf_eval_pdf": ([P, U32, P, P, P], F),
+            "oracle_bsdf_albedo": ([P, F, U32, U32], C.c_double),
         }
         for name, (args, res) in sigs.items():
             try:
@@ -133,3 +147,109 @@ def sort_pairs(keys, vals):
     ko, vo = np.empty_like(keys), np.empty_like(vals)
     lib().oracle_sort_pairs(_p(keys), _p(vals), len(keys), _p(ko), _p(vo))
     return ko, vo
+
+
+# --- tracer ------------------------------------------------------------------
+
+
+class RenderCounts(C.Structure):
+    _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64),
+                ("triangle_tests", C.c_uint64)]
+
+
+def sobol(index: int, dim: int) -> int:
+    return lib().oracle_sobol(index, dim)
+
+
+def owen_scramble(x: int, seed: int) -> int:
+    return lib().oracle_owen_scramble(x, seed)
+
+
+def hash_u32(x: int) -> int:
+    return lib().oracle_hash_u32(x)
+
+
+def sobol_sample(index: int, dim: int, seed: int) -> float:
+    return lib().oracle_sobol_sample(index, dim, seed)
+
+
+def sampler_draws(kind: int, frame: int, pixel: int, sample: int, pattern) -> np.ndarray:
+    """Draws from a Sampler: pattern entries 1 (next_f32) or 2 (next_vec2)."""
+    pat = np.ascontiguousarray(pattern, np.uint32)
+    out = np.empty(int(pat.sum()), np.float32)
+    lib().oracle_sampler_draws(kind, frame, pixel, sample, _p(pat), len(pat), _p(out))
+    return out
+
+
+def ggx_table() -> np.ndarray:
+    out = np.empty(1024, np.float32)
+    lib().oracle_ggx_table(_p(out))
+    return out
+
+
+def directional_albedo_grid(roughness: float, cos_o: float, grid: int) -> float:
+    return lib().oracle_directional_albedo_grid(roughness, cos_o, grid)
+
+
+def offset_ray_origin(p, n) -> np.ndarray:
+    p = np.asarray(p, np.float32)
+    n = np.asarray(n, np.float32)
+    out = np.empty(3, np.float32)
+    lib().oracle_offset_ray_origin(_p(p), _p(n), _p(out))
+    return out
+
+
+def render(desc, uniforms, background=(0.0, 0.0, 0.0), threads: int = 0, rows=None, guides: bool = True):
+    """integrator::render_with_guides over rows [r0, r1): (rgb (h,w,3), guides (h,w,8), counts)."""
+    r0, r1 = rows if rows is not None else (0, uniforms.height)
+    h = r1 - r0
+    rgb = np.zeros((h, uniforms.width, 3), np.float32)
+    g = np.zeros((h, uniforms.width, 8), np.float32) if guides else None
+    bg = np.asarray(background, np.float32)
+    counts = RenderCounts()
+    lib().oracle_render(C.byref(desc), C.byref(uniforms), _p(bg), threads, r0, r1, _p(rgb), _p(g), C.byref(counts))
+    return rgb, g, counts
+
+
+def denoise(film, albedo, normal, depth, params) -> np.ndarray:
+    """denoise::denoise (crates/core/src/denoise.rs:134)."""
+    h, w, _ = film.shape
+    f, a, n = (np.ascontiguousarray(x, np.float32) for x in (film, albedo, normal))
+    d = np.ascontiguousarray(depth, np.float32)
+    out = np.empty_like(f)
+    lib().oracle_denoise(_p(f), _p(a), _p(n), _p(d), w, h, C.byref(params), _p(out))
+    return out
+
+
+def compare(a, b) -> dict:
+    """image::compare (crates/core/src/image.rs:178) plus the mean |a-b| per channel."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    assert a.shape == b.shape
+    out = np.empty(6, np.float64)
+    lib().oracle_compare(_p(a), _p(b), a.size // 3, _p(out))
+    return dict(zip(("rmse", "max_abs", "mean_rel", "mean_a", "mean_b", "mean_abs"), out.tolist()))
+
+
+def bsdf_sample(material: np.ndarray, front: bool, wo, u_lobe: float, ux: float, uy: float):
+    """surface::sample (bsdf.rs:1571): (pdf, wi, weight)."""
+    m = np.ascontiguousarray(material)
+    wo = np.asarray(wo, np.float32)
+    out = np.empty(6, np.float32)
+    pdf = lib().oracle_bsdf_sample(_p(m), int(front), _p(wo), u_lobe, ux, uy, _p(out))
+    return pdf, out[:3], out[3:]
+
+
+def bsdf_eval_pdf(material: np.ndarray, front: bool, wo, wi):
+    """surface::eval and surface::pdf (bsdf.rs:1493/1543): (pdf, f)."""
+    m = np.ascontiguousarray(material)
+    wo = np.asarray(wo, np.float32)
+    wi = np.asarray(wi, np.float32)
+    f = np.empty(3, np.float32)
+    pdf = lib().oracle_bsdf_eval_pdf(_p(m), int(front), _p(wo), _p(wi), _p(f))
+    return pdf, f
+
+
+def bsdf_albedo(material: np.ndarray, cos_o: float, n: int = 200_000, seed: int = 1) -> float:
+    m = np.ascontiguousarray(material)
+    return lib().oracle_bsdf_albedo(_p(m), cos_o, n, seed)

@@ -1163,3 +1163,45 @@ void oracle_compare(const float* a, const float* b, size_t pixels, double* out6)
 }
 
 }  // extern "C"
+
+extern "C" {
+
+float oracle_bsdf_sample(const GpuMaterial* m, uint32_t front, const float wo[3], float u_lobe, float ux, float uy,
+                         float out[6]) {
+    Surface s = Surface::from_material(*m, front != 0);
+    BsdfSample b = surface_sample(s, Vec3::from(wo), u_lobe, Vec2{ux, uy});
+    b.wi.store(out);
+    b.weight.store(out + 3);
+    return b.pdf;
+}
+
+float oracle_bsdf_eval_pdf(const GpuMaterial* m, uint32_t front, const float wo[3], const float wi[3], float f_out[3]) {
+    Surface s = Surface::from_material(*m, front != 0);
+    surface_eval(s, Vec3::from(wo), Vec3::from(wi)).store(f_out);
+    return surface_pdf(s, Vec3::from(wo), Vec3::from(wi));
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Directional albedo of a material's full surface BSDF at cos_o, estimated
+// from n one-sample draws of surface::sample (bsdf.rs:1571) with a PCG stream.
+// A white, energy-compensated lobe integrates to 1 (the white furnace).
+double oracle_bsdf_albedo(const GpuMaterial* m, float cos_o, uint32_t n, uint32_t seed) {
+    Surface s = Surface::from_material(*m, true);
+    float sin_o = std::sqrt(std::fmax(1.0f - cos_o * cos_o, 0.0f));
+    Vec3 wo(sin_o, 0.0f, cos_o);
+    Rng rng(seed, 0, 0);
+    double sum = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        float ul = rng.next_f32();
+        float ux = rng.next_f32();
+        float uy = rng.next_f32();
+        BsdfSample b = surface_sample(s, wo, ul, Vec2{ux, uy});
+        if (b.pdf > 0.0f) sum += (double)b.weight.x;
+    }
+    return sum / (double)n;
+}
+
+}  // extern "C"

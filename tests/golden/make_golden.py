@@ -94,6 +94,7 @@ def main(ref: Path) -> None:
     # --- GGX energy table --------------------------------------------------------
     wgsl = (ref / "shaders/common/ggx_energy.wgsl").read_text()
     body = wgsl[wgsl.index("array<f32, 1024>("):]
+    body = re.sub(r"//[^\n]*", "", body)  # row comments carry numbers too
     vals = [float(v) for v in re.findall(r"\d+\.\d+", body)][:1024]
     assert len(vals) == 1024
     (OUT / "ggx_energy_table.json").write_text(json.dumps(vals))
