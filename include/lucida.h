@@ -283,6 +283,12 @@ lucida_status lucida_copy_to_host(lucida_ctx* ctx, void* dst, const void* src, s
 /* LBVH (crates/gpu/src/lbvh.rs)                                            */
 /* ------------------------------------------------------------------------ */
 
+/* Chooses how the LBVH fit orders its bottom-up unions: 0 (default) one launch
+ * per tree level, deepest first; 1 the atomic bottom-up walk (the second
+ * arrival at each node continues), which needs agent-scope acquire/release
+ * between XCDs. Both produce identical boxes (crates/core/src/lbvh.rs:490-525). */
+lucida_status lucida_ctx_set_lbvh_fit(lucida_ctx* ctx, int mode);
+
 /* LbvhBuilder::sort (crates/gpu/src/lbvh.rs:669): stable ascending sort of u32
  * keys carrying u32 values. Host arrays; n==0 is not an error. */
 lucida_status lucida_sort_u32(lucida_ctx* ctx, const uint32_t* keys, const uint32_t* vals, size_t n,
@@ -431,6 +437,43 @@ typedef struct lucida_denoise_params {
  * guide channels are copied). in and out are distinct films of one size. */
 lucida_status lucida_denoise_atrous(lucida_ctx* ctx, const lucida_film* in, const lucida_denoise_params* params,
                                     lucida_film* out);
+
+/* ---- Learned denoiser --------------------------------------------------------
+ * No counterpart in the reference (its only denoiser is the a-trous filter,
+ * crates/core/src/denoise.rs); it consumes the reference's Accum channels
+ * (gpu_layout.rs:345). A 4-level encoder-decoder of 3x3 convolutions
+ * (32/48/64/96 channels, skip concatenations, ReLU), BF16 weights and
+ * activations with FP32 accumulation, trained on the device with Adam.
+ * Parameters are fp32, per layer W[9][cin][cout] (tap = 3*dy + dx, dy/dx in
+ * 0..2 for offsets -1..1) followed by b[cout], layers in the order
+ *   16->32 32->32 | 32->48 48->48 | 48->64 64->64 | 64->96 96->96 |
+ *   160->64 64->64 | 112->48 48->48 | 80->32 32->32 | 32->16 (no ReLU).
+ * Crops and films must have a width that is a multiple of 128 and a height
+ * that is a multiple of 8. */
+typedef struct lucida_net lucida_net;
+
+lucida_status lucida_net_create(lucida_ctx* ctx, uint32_t seed, lucida_net** out);
+void lucida_net_destroy(lucida_net* net);
+size_t lucida_net_param_count(void);
+/* which = 0: parameters, 1: gradients of the last forward_backward. */
+lucida_status lucida_net_get_params(lucida_net* net, float* out, int which);
+lucida_status lucida_net_set_params(lucida_net* net, const float* in);
+/* Loss = mean squared error in log1p(radiance / albedo) space against refs,
+ * over `count` crops (x0[k], y0[k], w, h) of inputs[k] / refs[k] (x0, y0 may
+ * be NULL for crops at the origin). Leaves the gradients in the net. */
+lucida_status lucida_net_forward_backward(lucida_net* net, const lucida_film* const* inputs,
+                                          const lucida_film* const* refs, uint32_t count, const uint32_t* x0,
+                                          const uint32_t* y0, uint32_t w, uint32_t h, float* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the held gradients. */
+lucida_status lucida_net_adam(lucida_net* net, float lr);
+lucida_status lucida_net_train_step(lucida_net* net, const lucida_film* const* inputs,
+                                    const lucida_film* const* refs, uint32_t count, const uint32_t* x0,
+                                    const uint32_t* y0, uint32_t w, uint32_t h, float lr, float* loss);
+/* Denoises `in` into `out` (Accum-shaped, samples = 1, guides copied). */
+lucida_status lucida_net_denoise(lucida_net* net, const lucida_film* in, lucida_film* out);
+/* Test hook: tensor `index` (0 = input features, 1..15 = layer outputs) of
+ * the last forward pass, or its gradient, as floats (count must match). */
+lucida_status lucida_net_read_tensor(lucida_net* net, int index, int gradient, float* out, size_t count);
 
 #ifdef __cplusplus
 } /* extern "C" */

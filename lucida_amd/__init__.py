@@ -142,6 +142,7 @@ def lib() -> C.CDLL:
             "lucida_device_free": ([P, P], I),
             "lucida_copy_to_device": ([P, P, P, S], I),
             "lucida_copy_to_host": ([P, P, P, S], I),
+            "lucida_ctx_set_lbvh_fit": ([P, I], I),
             "lucida_sort_u32": ([P, P, P, S, P, P], I),
             "lucida_sort_u32_device": ([P, P, P, P, P, S], I),
             "lucida_lbvh_build_parts": ([P, P, S, P, S, C.POINTER(LbvhParts)], I),
@@ -161,6 +162,16 @@ def lib() -> C.CDLL:
                                          C.POINTER(RenderStats)], I),
             "lucida_denoise_atrous": ([P, P, C.POINTER(DenoiseParams), P], I),
             "lucida_energy_table": ([P, P], I),
+            "lucida_net_create": ([P, U32, C.POINTER(P)], I),
+            "lucida_net_destroy": ([P], None),
+            "lucida_net_param_count": ([], S),
+            "lucida_net_get_params": ([P, P, I], I),
+            "lucida_net_set_params": ([P, P], I),
+            "lucida_net_forward_backward": ([P, P, P, U32, P, P, U32, U32, C.POINTER(C.c_float)], I),
+            "lucida_net_adam": ([P, C.c_float], I),
+            "lucida_net_train_step": ([P, P, P, U32, P, P, U32, U32, C.c_float, C.POINTER(C.c_float)], I),
+            "lucida_net_denoise": ([P, P, P], I),
+            "lucida_net_read_tensor": ([P, I, I, P, S], I),
             "lucida_blob_create": ([C.c_char_p, C.POINTER(BlobParams), C.POINTER(P)], I),
             "lucida_blob_destroy": ([P], None),
             "lucida_blob_desc": ([P, C.POINTER(SceneDesc)], I),
@@ -312,10 +323,14 @@ class Gpu:
 
 
 class LbvhBuilder:
-    """LbvhBuilder (crates/gpu/src/lbvh.rs:153-758)."""
+    """LbvhBuilder (crates/gpu/src/lbvh.rs:153-758). `fit` picks the fit's
+    ordering: "levels" (one launch per level) or "atomic" (bottom-up walk)."""
 
-    def __init__(self, gpu: Gpu):
+    FITS = {"levels": 0, "atomic": 1}
+
+    def __init__(self, gpu: Gpu, fit: str = "levels"):
         self.gpu = gpu
+        _check(lib().lucida_ctx_set_lbvh_fit(gpu._h, self.FITS[fit]))
 
     def sort(self, keys: np.ndarray, values: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
         """LbvhBuilder::sort (lbvh.rs:669): stable ascending sort carrying values."""
@@ -458,3 +473,88 @@ class Denoiser:
     def run(self, film_in: Film, film_out: Film, params: DenoiseParams | None = None) -> None:
         p = params or DenoiseParams.default()
         _check(lib().lucida_denoise_atrous(self.gpu._h, film_in._h, C.byref(p), film_out._h))
+
+
+# Layer table of the learned denoiser: (cin, cout, relu) and resolution level.
+NET_LAYERS = [(16, 32, 1), (32, 32, 1), (32, 48, 1), (48, 48, 1), (48, 64, 1), (64, 64, 1), (64, 96, 1),
+              (96, 96, 1), (160, 64, 1), (64, 64, 1), (112, 48, 1), (48, 48, 1), (80, 32, 1), (32, 32, 1),
+              (32, 16, 0)]
+NET_LEVEL = [0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0]
+
+
+class LearnedDenoiser:
+    """The BF16 encoder-decoder denoiser trained and run on the device
+    (no reference counterpart; inputs are Accum films)."""
+
+    def __init__(self, gpu: Gpu, seed: int = 1):
+        self.gpu = gpu
+        h = C.c_void_p()
+        _check(lib().lucida_net_create(gpu._h, seed, C.byref(h)))
+        self._h = h
+        self._crop = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lucida_net_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def param_count() -> int:
+        return int(lib().lucida_net_param_count())
+
+    def params(self, gradients: bool = False) -> np.ndarray:
+        out = np.empty(self.param_count(), np.float32)
+        _check(lib().lucida_net_get_params(self._h, _ptr(out), 1 if gradients else 0))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        a = np.ascontiguousarray(p, np.float32)
+        assert a.size == self.param_count()
+        _check(lib().lucida_net_set_params(self._h, _ptr(a)))
+
+    @staticmethod
+    def _batch(inputs, refs, crops):
+        k = len(inputs)
+        ins = (C.c_void_p * k)(*[f._h.value for f in inputs])
+        rfs = (C.c_void_p * k)(*[f._h.value for f in refs]) if refs is not None else None
+        if crops is None:
+            return k, ins, rfs, None, None
+        x0 = np.ascontiguousarray([c[0] for c in crops], np.uint32)
+        y0 = np.ascontiguousarray([c[1] for c in crops], np.uint32)
+        return k, ins, rfs, x0, y0
+
+    def forward_backward(self, inputs, refs, width: int, height: int, crops=None) -> float:
+        k, ins, rfs, x0, y0 = self._batch(inputs, refs, crops)
+        loss = C.c_float()
+        _check(lib().lucida_net_forward_backward(self._h, ins, rfs, k, None if x0 is None else _ptr(x0),
+                                                 None if y0 is None else _ptr(y0), width, height,
+                                                 C.byref(loss)))
+        self._crop = (k, height, width)
+        return float(loss.value)
+
+    def adam(self, lr: float = 1e-3) -> None:
+        _check(lib().lucida_net_adam(self._h, lr))
+
+    def train_step(self, inputs, refs, width: int, height: int, crops=None, lr: float = 1e-3) -> float:
+        k, ins, rfs, x0, y0 = self._batch(inputs, refs, crops)
+        loss = C.c_float()
+        _check(lib().lucida_net_train_step(self._h, ins, rfs, k, None if x0 is None else _ptr(x0),
+                                           None if y0 is None else _ptr(y0), width, height, lr,
+                                           C.byref(loss)))
+        self._crop = (k, height, width)
+        return float(loss.value)
+
+    def denoise(self, film_in: Film, film_out: Film) -> None:
+        _check(lib().lucida_net_denoise(self._h, film_in._h, film_out._h))
+        self._crop = (1, film_in.height, film_in.width)
+
+    def tensor(self, index: int, gradient: bool = False) -> np.ndarray:
+        """Activation `index` (0 = features, 1..15 = layer outputs) of the last
+        forward pass, or its gradient, shape (n, h, w, c)."""
+        n, h, w = self._crop
+        lvl = 0 if index == 0 else NET_LEVEL[index - 1]
+        c = 16 if index == 0 else NET_LAYERS[index - 1][1]
+        shape = (n, h >> lvl, w >> lvl, c)
+        out = np.empty(shape, np.float32)
+        _check(lib().lucida_net_read_tensor(self._h, index, 1 if gradient else 0, _ptr(out), out.size))
+        return out

@@ -80,7 +80,7 @@ void lucida_ctx_destroy(lucida_ctx* ctx) {
     DevBuf* bufs[] = {&ctx->lb_bounds, &ctx->lb_partial, &ctx->lb_codes, &ctx->lb_vals,   &ctx->lb_tmpk,
                       &ctx->lb_tmpv,   &ctx->lb_hist,    &ctx->lb_karras, &ctx->lb_parent, &ctx->lb_nodebox,
                       &ctx->lb_subtree, &ctx->lb_flags,  &ctx->lb_misc,  &ctx->lb_rank,   &ctx->lb_nodes,
-                      &ctx->lb_tris,   &ctx->lb_pos,     &ctx->lb_prims, &ctx->ggx_table, &ctx->wf_planes,
+                      &ctx->lb_tris,   &ctx->lb_pos,     &ctx->lb_prims, &ctx->lb_level, &ctx->ggx_table, &ctx->wf_planes,
                       &ctx->wf_hits,   &ctx->wf_shadow,  &ctx->wf_queues, &ctx->wf_counters, &ctx->wf_uniforms,
                       &ctx->dn_buf};
     for (DevBuf* b : bufs) b->release();

@@ -56,7 +56,8 @@ struct lucida_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // LBVH scratch, reused between builds.
     lucida::DevBuf lb_bounds, lb_partial, lb_codes, lb_vals, lb_tmpk, lb_tmpv, lb_hist, lb_karras, lb_parent,
-        lb_nodebox, lb_subtree, lb_flags, lb_misc, lb_rank, lb_nodes, lb_tris, lb_pos, lb_prims;
+        lb_nodebox, lb_subtree, lb_flags, lb_misc, lb_rank, lb_nodes, lb_tris, lb_pos, lb_prims, lb_level;
+    int lbvh_fit = 0;  // 0: level-synchronous fit, 1: atomic bottom-up fit
     // GGX directional-albedo table (energy compensation), 32 x 32 floats.
     lucida::DevBuf ggx_table;
     bool ggx_ready = false;

@@ -366,9 +366,12 @@ __global__ __launch_bounds__(256) void k_fit(const LucidaAabb* __restrict__ tri_
     box_store(&node_box[leaf_base + k], lo, hi);
     uint32_t node = parent[leaf_base + k];
     while (node != NONE) {
-        __threadfence();
-        uint32_t old = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        // Release this child's box, count the arrival, and acquire the sibling's
+        // box only when this thread is the second arrival.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        uint32_t old = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old == 0) return;  // first arrival: the sibling's thread finishes this node
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const uint2 c = karras[node];
         float llo[3], lhi[3], rlo[3], rhi[3];
         box_load(&node_box[c.x], llo, lhi);
@@ -388,6 +391,94 @@ __global__ __launch_bounds__(256) void k_fit(const LucidaAabb* __restrict__ tri_
                            __HIP_MEMORY_SCOPE_AGENT);
         node = parent[node];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Level-synchronous bottom-up fit: the same unions as the atomic fit, ordered
+// by depth instead of by arrival. Each internal node's depth comes from walking
+// its parents; nodes are bucketed by depth and one launch per level, deepest
+// first, unions the children. Launch boundaries order the levels, so no
+// agent-scope fences (and their L2 write-back / invalidate on every XCD) are
+// needed.
+// ---------------------------------------------------------------------------
+
+constexpr int MAX_LEVELS = 128;
+
+__global__ __launch_bounds__(256) void k_leaf_boxes(const LucidaAabb* __restrict__ tri_bounds,
+                                                    const uint32_t* __restrict__ order, uint32_t n,
+                                                    LucidaAabb* __restrict__ node_box) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) node_box[n - 1 + k] = tri_bounds[order[k]];
+}
+
+// Depth of every internal node and a per-level histogram (LDS per block, one
+// atomic per level per block). depth >= MAX_LEVELS raises the overflow flag.
+__global__ __launch_bounds__(256) void k_depth(uint32_t n, const uint32_t* __restrict__ parent,
+                                               uint32_t* __restrict__ depth, uint32_t* __restrict__ level_count,
+                                               uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t h[MAX_LEVELS];
+    for (int i = threadIdx.x; i < MAX_LEVELS; i += 256) h[i] = 0;
+    __syncthreads();
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x < n - 1) {
+        uint32_t d = 0, cur = x;
+        while (cur != 0 && d < MAX_LEVELS) {
+            cur = parent[cur];
+            ++d;
+        }
+        if (d >= MAX_LEVELS) {
+            atomicOr(overflow, 1u);
+            d = MAX_LEVELS - 1;
+        }
+        depth[x] = d;
+        atomicAdd(&h[d], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < MAX_LEVELS; i += 256)
+        if (h[i]) atomicAdd(&level_count[i], h[i]);
+}
+
+// Scatters node ids into per-level lists: cursor[d] starts at level d's offset.
+__global__ __launch_bounds__(256) void k_bucket(uint32_t n, const uint32_t* __restrict__ depth,
+                                                uint32_t* __restrict__ cursor, uint32_t* __restrict__ list) {
+    __shared__ uint32_t h[MAX_LEVELS], base[MAX_LEVELS];
+    for (int i = threadIdx.x; i < MAX_LEVELS; i += 256) h[i] = 0;
+    __syncthreads();
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    uint32_t d = 0, local = 0;
+    if (x < n - 1) {
+        d = depth[x];
+        local = atomicAdd(&h[d], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < MAX_LEVELS; i += 256) base[i] = h[i] ? atomicAdd(&cursor[i], h[i]) : 0u;
+    __syncthreads();
+    if (x < n - 1) list[base[d] + local] = x;
+}
+
+__global__ __launch_bounds__(256) void k_fit_level(const uint32_t* __restrict__ list, uint32_t count, uint32_t n,
+                                                   const uint2* __restrict__ karras, LucidaAabb* node_box,
+                                                   uint32_t* subtree, uint32_t* inner) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t node = list[t];
+    const uint32_t leaf_base = n - 1;
+    const uint2 c = karras[node];
+    float llo[3], lhi[3], rlo[3], rhi[3], lo[3], hi[3];
+    box_load(&node_box[c.x], llo, lhi);
+    box_load(&node_box[c.y], rlo, rhi);
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = fminf(llo[a], rlo[a]);
+        hi[a] = fmaxf(lhi[a], rhi[a]);
+    }
+    const uint32_t sl = c.x >= leaf_base ? 1u : subtree[c.x];
+    const uint32_t sr = c.y >= leaf_base ? 1u : subtree[c.y];
+    const uint32_t il = c.x >= leaf_base ? 0u : inner[c.x];
+    const uint32_t ir = c.y >= leaf_base ? 0u : inner[c.y];
+    const uint32_t size = sl + sr;
+    box_store(&node_box[node], lo, hi);
+    subtree[node] = size;
+    inner[node] = size > MAX_LEAF ? 1u + il + ir : 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -505,6 +596,48 @@ static lucida_status sort_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals
     return LUCIDA_OK;
 }
 
+// Level-synchronous fit (see k_depth / k_bucket / k_fit_level): one readback
+// of the per-level histogram, then one launch per level, deepest first.
+static lucida_status fit_levels(lucida_ctx* ctx, size_t n) {
+    hipStream_t s = ctx->stream;
+    const size_t internal = n - 1;
+    LUCIDA_HIP(ctx->lb_level.ensure(internal * 4 + (3 * MAX_LEVELS + 8) * 4), LUCIDA_OOM);
+    uint32_t* list = ctx->lb_level.as<uint32_t>();
+    uint32_t* level_count = list + internal;
+    uint32_t* cursor = level_count + MAX_LEVELS;
+    uint32_t* overflow = cursor + MAX_LEVELS;
+    uint32_t* depth = ctx->lb_flags.as<uint32_t>();
+    uint32_t* subtree = ctx->lb_subtree.as<uint32_t>();
+    LUCIDA_HIP(hipMemsetAsync(level_count, 0, (2 * MAX_LEVELS + 8) * 4, s), LUCIDA_VALIDATION);
+    hipLaunchKernelGGL(k_leaf_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, s, ctx->lb_bounds.as<LucidaAabb>(),
+                       ctx->lb_vals.as<uint32_t>(), (uint32_t)n, ctx->lb_nodebox.as<LucidaAabb>());
+    hipLaunchKernelGGL(k_depth, dim3(blocks_for(internal, 256)), dim3(256), 0, s, (uint32_t)n,
+                       ctx->lb_parent.as<uint32_t>(), depth, level_count, overflow);
+    uint32_t host[MAX_LEVELS + 1];
+    LUCIDA_HIP(hipMemcpyAsync(host, level_count, MAX_LEVELS * 4, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
+    LUCIDA_HIP(hipMemcpyAsync(host + MAX_LEVELS, overflow, 4, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
+    LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
+    if (host[MAX_LEVELS]) return set_error(LUCIDA_VALIDATION, "LBVH deeper than 128 levels");
+    uint32_t off[MAX_LEVELS], acc = 0;
+    int deepest = 0;
+    for (int l = 0; l < MAX_LEVELS; ++l) {
+        off[l] = acc;
+        acc += host[l];
+        if (host[l]) deepest = l;
+    }
+    // Blocking copy: the stream is idle here (synchronised above) and `off` is a stack array.
+    LUCIDA_HIP(hipMemcpy(cursor, off, sizeof off, hipMemcpyHostToDevice), LUCIDA_VALIDATION);
+    hipLaunchKernelGGL(k_bucket, dim3(blocks_for(internal, 256)), dim3(256), 0, s, (uint32_t)n, depth, cursor, list);
+    for (int l = deepest; l >= 0; --l) {
+        if (!host[l]) continue;
+        hipLaunchKernelGGL(k_fit_level, dim3(blocks_for(host[l], 256)), dim3(256), 0, s, list + off[l], host[l],
+                           (uint32_t)n, ctx->lb_karras.as<uint2>(), ctx->lb_nodebox.as<LucidaAabb>(), subtree,
+                           subtree + internal);
+    }
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    return LUCIDA_OK;
+}
+
 // The device stages of the build, up to and including the fit. Results stay in
 // the context's scratch (lb_codes = sorted codes, lb_vals = order, ...).
 static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
@@ -557,12 +690,17 @@ static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size
     }
     if (timed) LUCIDA_HIP(hipEventRecord(ev[3], s), LUCIDA_VALIDATION);
     if (n >= 2) {
-        LUCIDA_HIP(hipMemsetAsync(ctx->lb_flags.ptr, 0, (n - 1) * 4, s), LUCIDA_VALIDATION);
         uint32_t* subtree = ctx->lb_subtree.as<uint32_t>();
-        hipLaunchKernelGGL(k_fit, dim3(blocks_for(n, 256)), dim3(256), 0, s, ctx->lb_bounds.as<LucidaAabb>(),
-                           ctx->lb_vals.as<uint32_t>(), (uint32_t)n, ctx->lb_karras.as<uint2>(),
-                           ctx->lb_parent.as<uint32_t>(), ctx->lb_nodebox.as<LucidaAabb>(), subtree,
-                           subtree + internal, ctx->lb_flags.as<uint32_t>());
+        if (ctx->lbvh_fit == 1) {
+            LUCIDA_HIP(hipMemsetAsync(ctx->lb_flags.ptr, 0, (n - 1) * 4, s), LUCIDA_VALIDATION);
+            hipLaunchKernelGGL(k_fit, dim3(blocks_for(n, 256)), dim3(256), 0, s, ctx->lb_bounds.as<LucidaAabb>(),
+                               ctx->lb_vals.as<uint32_t>(), (uint32_t)n, ctx->lb_karras.as<uint2>(),
+                               ctx->lb_parent.as<uint32_t>(), ctx->lb_nodebox.as<LucidaAabb>(), subtree,
+                               subtree + internal, ctx->lb_flags.as<uint32_t>());
+        } else {
+            lucida_status fs = fit_levels(ctx, n);
+            if (fs != LUCIDA_OK) return fs;
+        }
     }
     if (timed) LUCIDA_HIP(hipEventRecord(ev[4], s), LUCIDA_VALIDATION);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
@@ -646,6 +784,12 @@ lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t
 using namespace lucida;
 
 extern "C" {
+
+lucida_status lucida_ctx_set_lbvh_fit(lucida_ctx* ctx, int mode) {
+    if (!ctx || mode < 0 || mode > 1) return set_error(LUCIDA_INVALID_ARG, "lucida_ctx_set_lbvh_fit: mode is 0 or 1");
+    ctx->lbvh_fit = mode;
+    return LUCIDA_OK;
+}
 
 lucida_status lucida_sort_u32_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
                                      size_t n) {

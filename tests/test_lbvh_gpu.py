@@ -156,6 +156,23 @@ def test_full_size_c2_build_is_bit_exact(gpu, oracle):
     np.testing.assert_array_equal(bits(nodes), bits(o["nodes"]))
 
 
+@pytest.mark.parametrize("case", ["icosphere-3", "cornell-mesh", "duplicates", "n=2", "n=5", "sphere-c2"])
+def test_atomic_and_level_fits_agree(gpu, oracle, case):
+    """The atomic bottom-up fit and the level-synchronous fit produce the same
+    tree (and both the oracle's)."""
+    tris, pos = geometry(case) if case == "sphere-c2" else case_geometry(case)
+    say(f"lbvh fit modes case: {case} n={len(tris)}")
+    try:
+        n_atomic, p_atomic = L.LbvhBuilder(gpu, fit="atomic").build(tris, pos)
+    finally:
+        L.LbvhBuilder(gpu, fit="levels")
+    n_levels, p_levels = L.LbvhBuilder(gpu, fit="levels").build(tris, pos)
+    np.testing.assert_array_equal(bits(n_atomic), bits(n_levels))
+    np.testing.assert_array_equal(p_atomic, p_levels)
+    o = oracle.lbvh_build(tris, pos)
+    np.testing.assert_array_equal(bits(n_levels), bits(o["nodes"]))
+
+
 def test_scene_lbvh_is_device_resident_and_compacted(gpu, oracle):
     b = L.Blob("cornell-mesh")
     a = b.arrays()
