@@ -7,9 +7,11 @@ cleared film. The scene, its GPU-built LBVH and the film are resident in HBM
 before timing starts. value = paths per second over the timed steps.
 
 Prints one JSON line. Extra figures (Mrays/s, LBVH Mtris/s, a-trous time,
-per-kernel times) ride in "extra".
+per-kernel times) ride in "extra". "time_to_relmse" holds the metric's second
+half: device time until the plain and the denoised images of the same scene
+reach relMSE 0.01 against an independent 4096-spp render.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--cpu-seconds 15] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--cpu-seconds 15] [--no-cpu] [--no-relmse]
 """
 from __future__ import annotations
 
@@ -94,6 +96,85 @@ def cpu_baseline(blob, u, target_s: float) -> dict:
                       f"{u.max_depth} bounces ({paths} paths, {dt:.1f} s)"}
 
 
+def relmse(x: np.ndarray, ref: np.ndarray) -> float:
+    """Relative MSE, mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2).
+    The reference defines no relMSE; this is the definition used throughout."""
+    x = x.astype(np.float64)
+    r = ref.astype(np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+
+def time_to_relmse(L, gpu, scene, blob, width: int, height: int, depth: int, threshold: float, ref_spp: int,
+                   max_spp: int, net=None) -> dict:
+    """Progressive render of the same scene, sample counts 1, 2, 4, ..., max_spp,
+    scored after each doubling against an independent ref_spp render (another
+    frame seed, so no sample is shared). Time = device render time of all
+    samples so far (+ the denoiser's time for the denoised variants); the
+    crossing of `threshold` is interpolated log-log between the bracketing
+    sample counts, with time linear in samples."""
+    mk = lambda spp, off, seed: blob.uniforms(width, height, samples=spp, max_depth=depth, sampling=L.SAMPLING_MIS,
+                                              sampler=L.SAMPLER_SOBOL, sample_offset=off, frame_seed=seed)
+    ref_film = L.Film(gpu, width, height)
+    t0 = time.perf_counter()
+    st = L.render_wavefront(gpu, scene, mk(ref_spp, 0, 0x0BADF00D), ref_spp, ref_film)
+    gpu.check_errors()
+    ref_s = time.perf_counter() - t0
+    ref = L.film_mean(ref_film.download(), width, height)
+    del ref_film
+    film = L.Film(gpu, width, height)
+    den = L.Film(gpu, width, height)
+    dn = L.Denoiser(gpu)
+    variants = {"plain": [], "atrous": []}
+    if net is not None:
+        variants["learned"] = []
+    render_ms = 0.0
+    done = 0
+    spp = 1
+    while spp <= max_spp:
+        add = spp - done
+        st = L.render_wavefront(gpu, scene, mk(add, done, 0x5EED1234), add, film)
+        render_ms += st.elapsed_ms
+        done = spp
+        acc = film.download()
+        variants["plain"].append((spp, render_ms, relmse(L.film_mean(acc, width, height), ref)))
+        for name in [k for k in variants if k != "plain"]:
+            gpu.check_errors()
+            ta = time.perf_counter()
+            if name == "atrous":
+                dn.run(film, den)
+            else:
+                net.denoise(film, den)
+            gpu.check_errors()
+            dms = (time.perf_counter() - ta) * 1e3
+            variants[name].append((spp, render_ms + dms, relmse(L.film_mean(den.download(), width, height), ref)))
+        if all(v[-1][2] <= threshold for v in variants.values()):
+            break
+        spp *= 2
+
+    def crossing(pts):
+        for i, (s, t, e) in enumerate(pts):
+            if e <= threshold:
+                if i == 0:
+                    return {"spp": float(s), "ms": round(t, 3), "relmse": e}
+                s0, t0_, e0 = pts[i - 1]
+                f = (np.log(e0) - np.log(threshold)) / (np.log(e0) - np.log(e))
+                s_x = float(np.exp(np.log(s0) + f * (np.log(s) - np.log(s0))))
+                return {"spp": round(s_x, 2), "ms": round(t0_ + (t - t0_) * (s_x - s0) / (s - s0), 3),
+                        "relmse_at_bracket": [e0, e]}
+        return None
+
+    out = {"threshold": threshold, "definition": "mean (x-r)^2/(r^2+1e-2) over pixels and channels",
+           "resolution": [width, height], "reference_spp": ref_spp, "reference_render_s": round(ref_s, 2),
+           "max_spp": max_spp}
+    for name, pts in variants.items():
+        out[name] = {"reached": crossing(pts), "curve": [[s, round(t, 3), round(e, 6)] for s, t, e in pts]}
+    p = out["plain"]["reached"]
+    for name in [k for k in variants if k != "plain"]:
+        r = out[name]["reached"]
+        out[name]["speedup_vs_plain"] = round(p["ms"] / r["ms"], 2) if p and r else None
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +188,12 @@ def main() -> int:
     ap.add_argument("--samples-per-batch", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--relmse-ref-spp", type=int, default=4096)
+    ap.add_argument("--relmse-max-spp", type=int, default=1024)
+    ap.add_argument("--no-relmse", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the library's streaming copy of 256 MiB three times (the known-size kernel "
+                         "the FETCH_SIZE / WRITE_SIZE counters are scaled by) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
@@ -117,6 +204,21 @@ def main() -> int:
 
     gpu = L.Gpu(0)
     dev_name, cus = gpu.info()
+    if a.copy_calibration:
+        lib = L.lib()
+        nbytes = 256 << 20
+        bufs = []
+        for _ in range(2):
+            p = C.c_void_p()
+            L._check(lib.lucida_device_alloc(gpu._h, nbytes, C.byref(p)))
+            bufs.append(p)
+        for _ in range(3):
+            L._check(lib.lucida_copy_device(gpu._h, bufs[1], bufs[0], nbytes))
+        for p in bufs:
+            lib.lucida_device_free(gpu._h, p)
+        print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 3}), flush=True)
+        gpu.close()
+        return 0
     blob = L.Blob("sphere-c2", frequency=a.frequency)
     ntris = blob.desc().num_triangles
     lbvh = lbvh_build_rate(L, gpu, blob)
@@ -205,6 +307,9 @@ def main() -> int:
             "atrous_ms": round(atrous_ms, 3),
         },
     }
+    if not a.no_relmse:
+        out["time_to_relmse"] = time_to_relmse(L, gpu, scene, blob, a.width, a.height, a.depth, 0.01,
+                                               a.relmse_ref_spp, a.relmse_max_spp)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(blob, u, a.cpu_seconds)
     else:

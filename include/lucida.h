@@ -278,6 +278,10 @@ lucida_status lucida_device_alloc(lucida_ctx* ctx, size_t bytes, void** dptr);
 lucida_status lucida_device_free(lucida_ctx* ctx, void* dptr);
 lucida_status lucida_copy_to_device(lucida_ctx* ctx, void* dst, const void* src, size_t bytes);
 lucida_status lucida_copy_to_host(lucida_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a streaming kernel (16-byte loads and stores).
+ * dst, src and bytes must be multiples of 16. No reference counterpart: it is
+ * the known-size kernel the HBM byte counters are calibrated against. */
+lucida_status lucida_copy_device(lucida_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ------------------------------------------------------------------------ */
 /* LBVH (crates/gpu/src/lbvh.rs)                                            */

@@ -142,6 +142,7 @@ def lib() -> C.CDLL:
             "lucida_device_free": ([P, P], I),
             "lucida_copy_to_device": ([P, P, P, S], I),
             "lucida_copy_to_host": ([P, P, P, S], I),
+            "lucida_copy_device": ([P, P, P, S], I),
             "lucida_ctx_set_lbvh_fit": ([P, I], I),
             "lucida_sort_u32": ([P, P, P, S, P, P], I),
             "lucida_sort_u32_device": ([P, P, P, P, P, S], I),

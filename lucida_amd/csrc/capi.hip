@@ -6,6 +6,7 @@
 // Unlike the reference, a scene is uploaded once and stays resident.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -14,6 +15,12 @@
 namespace lucida {
 
 static thread_local std::string g_last_error;
+
+// Streaming copy, 16 bytes per lane per iteration.
+__global__ void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 
 lucida_status set_error(lucida_status code, const char* msg) {
     g_last_error = msg ? msg : "";
@@ -36,8 +43,8 @@ static lucida_status upload(lucida_ctx* ctx, T** dst, const T* src, size_t count
 }
 
 static void scene_free(lucida_scene* s) {
-    void* ptrs[] = {s->materials, s->primitives, s->positions, s->vertex_attrs, s->triangles, s->bvh_nodes,
-                    s->lights,    s->env_pixels, s->env_conditional, s->env_marginal};
+    void* ptrs[] = {s->materials, s->primitives, s->positions,       s->vertex_attrs, s->triangles, s->tri_edges,
+                    s->bvh_nodes, s->lights,     s->env_pixels,      s->env_conditional, s->env_marginal};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -138,6 +145,20 @@ lucida_status lucida_copy_to_host(lucida_ctx* ctx, void* dst, const void* src, s
     return LUCIDA_OK;
 }
 
+lucida_status lucida_copy_device(lucida_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return set_error(LUCIDA_INVALID_ARG, "lucida_copy_device: null argument");
+    if (((uintptr_t)dst | (uintptr_t)src | bytes) & 15u)
+        return set_error(LUCIDA_INVALID_ARG, "lucida_copy_device: pointers and size must be multiples of 16");
+    if (!bytes) return LUCIDA_OK;
+    const size_t n = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, ctx->stream, reinterpret_cast<const float4*>(src),
+                       reinterpret_cast<float4*>(dst), n);
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
+    return LUCIDA_OK;
+}
+
 lucida_status lucida_scene_create(lucida_ctx* ctx, const lucida_scene_desc* d, lucida_scene** out) {
     if (!ctx || !d || !out) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_create: null argument");
     clear_error();
@@ -215,6 +236,7 @@ lucida_status lucida_scene_create(lucida_ctx* ctx, const lucida_scene_desc* d, l
             if (sorted) (void)hipFree(sorted);
         }
     }
+    if (st == LUCIDA_OK && s->num_triangles) st = build_tri_edges(ctx, s);
     if (st == LUCIDA_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
         st = set_error(LUCIDA_VALIDATION, "scene upload failed");
     if (st != LUCIDA_OK) {

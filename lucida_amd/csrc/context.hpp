@@ -75,6 +75,7 @@ struct lucida_scene {
     float* positions = nullptr;
     GpuVertexAttr* vertex_attrs = nullptr;
     GpuTriangle* triangles = nullptr;
+    float4* tri_edges = nullptr;  // per triangle: p0, p1 - p0, p2 - p0 (the intersection's own operands)
     GpuBvhNode* bvh_nodes = nullptr;
     GpuLight* lights = nullptr;
     float* env_pixels = nullptr;
@@ -95,4 +96,6 @@ struct lucida_film {
 namespace lucida {
 // Ensures the context's GGX energy table is resident (computed on the device).
 lucida_status ensure_ggx_table(lucida_ctx* ctx);
+// Fills scene->tri_edges from its (final, leaf-ordered) triangles.
+lucida_status build_tri_edges(lucida_ctx* ctx, lucida_scene* scene);
 }  // namespace lucida

@@ -205,30 +205,97 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_hist(const uint32_t* __res
     hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-constexpr int SCAN_BLOCK = 1024;
+// Exclusive scan of the digit-major histogram, in three launches: per-segment
+// sums (k_scan_reduce), a scan of those sums (k_scan_top), and a scan of each
+// segment from its carry (k_scan_down). Every global access is coalesced; a
+// segment is SCAN_ITEMS words per thread of a SCAN_BLOCK block.
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_SEG = SCAN_BLOCK * SCAN_ITEMS;
 
-// Exclusive scan of `count` words in place, one block: every thread scans a
-// contiguous chunk, the chunk totals are scanned in LDS.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(uint32_t* __restrict__ data, uint32_t count) {
-    __shared__ uint32_t sums[SCAN_BLOCK];
-    const uint32_t per = (count + SCAN_BLOCK - 1) / SCAN_BLOCK;
-    const uint32_t begin = threadIdx.x * per;
-    const uint32_t end = begin + per < count ? begin + per : count;
-    uint32_t s = 0;
-    for (uint32_t k = begin; k < end; ++k) s += data[k];
-    sums[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
-        uint32_t v = threadIdx.x >= (uint32_t)off ? sums[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sums[threadIdx.x] += v;
-        __syncthreads();
+// Inclusive scan across the block of one value per thread (wave64 shuffles,
+// then the wave totals through LDS). `wsum` holds SCAN_BLOCK / 64 words.
+__device__ inline uint32_t block_inclusive_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
     }
-    uint32_t run = sums[threadIdx.x] - s;  // exclusive prefix of this chunk
-    for (uint32_t k = begin; k < end; ++k) {
-        uint32_t v = data[k];
-        data[k] = run;
-        run += v;
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+        const uint32_t s = wsum[w];
+        if (w < wave) before += s;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return v + before;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* __restrict__ data, uint32_t count,
+                                                            uint32_t* __restrict__ partials) {
+    __shared__ uint32_t wsum[SCAN_BLOCK / 64];
+    const uint32_t base = blockIdx.x * SCAN_SEG;
+    uint32_t s = 0;
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        const uint32_t idx = base + r * SCAN_BLOCK + threadIdx.x;
+        if (idx < count) s += data[idx];
+    }
+    uint32_t total;
+    (void)block_inclusive_scan(s, wsum, &total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of `count` partial sums in place, SCAN_BLOCK at a time.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_top(uint32_t* __restrict__ partials, uint32_t count) {
+    __shared__ uint32_t wsum[SCAN_BLOCK / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < count; base += SCAN_BLOCK) {
+        const uint32_t idx = base + threadIdx.x;
+        const uint32_t v = idx < count ? partials[idx] : 0u;
+        uint32_t total;
+        const uint32_t inc = block_inclusive_scan(v, wsum, &total);
+        if (idx < count) partials[idx] = carry + inc - v;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(uint32_t* __restrict__ data, uint32_t count,
+                                                          const uint32_t* __restrict__ partials) {
+    __shared__ uint32_t seg[SCAN_SEG];
+    __shared__ uint32_t wsum[SCAN_BLOCK / 64];
+    const uint32_t base = blockIdx.x * SCAN_SEG;
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        const uint32_t idx = base + r * SCAN_BLOCK + threadIdx.x;
+        seg[r * SCAN_BLOCK + threadIdx.x] = idx < count ? data[idx] : 0u;
+    }
+    __syncthreads();
+    // thread t owns segment words [t * SCAN_ITEMS, (t + 1) * SCAN_ITEMS)
+    uint32_t local[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        local[i] = seg[threadIdx.x * SCAN_ITEMS + i];
+        s += local[i];
+    }
+    uint32_t total;
+    uint32_t run = partials[blockIdx.x] + block_inclusive_scan(s, wsum, &total) - s;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        seg[threadIdx.x * SCAN_ITEMS + i] = run;
+        run += local[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        const uint32_t idx = base + r * SCAN_BLOCK + threadIdx.x;
+        if (idx < count) data[idx] = seg[r * SCAN_BLOCK + threadIdx.x];
     }
 }
 
@@ -574,15 +641,20 @@ static lucida_status sort_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals
     if (n == 0) return LUCIDA_OK;
     if (n > 0xFFFFFFFFull / 2) return set_error(LUCIDA_INVALID_ARG, "sort: more than 2^31 keys");
     const uint32_t tiles = blocks_for(n, SORT_TILE);
-    LUCIDA_HIP(ctx->lb_hist.ensure((size_t)tiles * RADIX * 4), LUCIDA_OOM);
+    const uint32_t hist_words = tiles * RADIX;
+    const uint32_t segs = blocks_for(hist_words, SCAN_SEG);
+    LUCIDA_HIP(ctx->lb_hist.ensure(((size_t)hist_words + segs) * 4), LUCIDA_OOM);
     uint32_t* hist = ctx->lb_hist.as<uint32_t>();
+    uint32_t* partials = hist + hist_words;
     uint32_t *src_k = keys, *src_v = vals, *dst_k = tk, *dst_v = tv;
     const uint32_t passes = (bits + 7) / 8;
     for (uint32_t pass = 0; pass < passes; ++pass) {
         const uint32_t shift = pass * 8;
         hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_BLOCK), 0, ctx->stream, src_k, (uint32_t)n, shift,
                            hist, tiles);
-        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, hist, tiles * RADIX);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, hist, hist_words, partials);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, partials, segs);
+        hipLaunchKernelGGL(k_scan_down, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, hist, hist_words, partials);
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_BLOCK), 0, ctx->stream, src_k, src_v,
                            (uint32_t)n, shift, hist, tiles, dst_k, dst_v);
         std::swap(src_k, dst_k);

@@ -73,6 +73,7 @@ struct SceneView {
     const float4* __restrict__ pos;
     const GpuVertexAttr* __restrict__ attrs;
     const GpuTriangle* __restrict__ tris;
+    const float4* __restrict__ edges;  // per triangle: p0, p1 - p0, p2 - p0
     const GpuBvhNode* __restrict__ nodes;
     const GpuLight* __restrict__ lights;
     const float* __restrict__ ggx;
@@ -192,12 +193,16 @@ __device__ inline V3 safe_inv_dir(V3 d) {
               fabsf(d.z) > SMALL ? 1.0f / d.z : copysignf(BIG, d.z)};
 }
 
-// bvh.rs:930 intersect_triangle, in the twin's operation order.
-__device__ inline bool tri_test(const float4* __restrict__ pos, const GpuTriangle& tri, V3 o, V3 d, float t_min,
+// bvh.rs:930 intersect_triangle, in the twin's operation order. The edges
+// e1 = p1 - p0 and e2 = p2 - p0 come precomputed (the same f32 subtractions,
+// done once per triangle by k_tri_edges), so a test is three independent
+// 16-byte loads instead of an index load followed by three gathers.
+__device__ inline bool tri_test(const float4* __restrict__ edges, uint32_t ti, V3 o, V3 d, float t_min,
                                 float t_max, float* t_out, float* u_out, float* v_out) {
-    V3 p0 = f4xyz(pos[tri.i0]);
-    V3 e1 = f4xyz(pos[tri.i1]) - p0;
-    V3 e2 = f4xyz(pos[tri.i2]) - p0;
+    const float4* e = edges + 3 * (size_t)ti;
+    const V3 p0 = f4xyz(e[0]);
+    const V3 e1 = f4xyz(e[1]);
+    const V3 e2 = f4xyz(e[2]);
     V3 pv = cross(d, e2);
     float det = dot(e1, pv);
     if (fabsf(det) < 1.0e-12f) return false;
@@ -229,7 +234,7 @@ __device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk,
     if (sc.num_nodes == 0) {
         for (uint32_t i = 0; i < sc.num_tris; ++i) {
             float t, u, v;
-            if (tri_test(sc.pos, sc.tris[i], o, d, t_min, closest, &t, &u, &v)) {
+            if (tri_test(sc.edges, i, o, d, t_min, closest, &t, &u, &v)) {
                 if (ANY_HIT) return true;
                 closest = t;
                 *t_out = t; *u_out = u; *v_out = v; *tri_out = i;
@@ -254,7 +259,7 @@ __device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk,
                 const uint32_t ti = first + k;
                 *tests += 1;
                 float t, u, v;
-                if (tri_test(sc.pos, sc.tris[ti], o, d, t_min, closest, &t, &u, &v)) {
+                if (tri_test(sc.edges, ti, o, d, t_min, closest, &t, &u, &v)) {
                     if (ANY_HIT) return true;
                     closest = t;
                     *t_out = t; *u_out = u; *v_out = v; *tri_out = ti;
@@ -673,7 +678,31 @@ __global__ __launch_bounds__(256) void k_ggx_table(float* table) {
 
 inline uint32_t grid_for(size_t n, int block) { return (uint32_t)((n + block - 1) / block); }
 
+// Per triangle (in the scene's leaf order): p0, p1 - p0, p2 - p0.
+__global__ void k_tri_edges(const GpuTriangle* __restrict__ tris, const float4* __restrict__ pos, size_t n,
+                            float4* __restrict__ edges) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const GpuTriangle t = tris[i];
+        const V3 p0 = f4xyz(pos[t.i0]);
+        const V3 e1 = f4xyz(pos[t.i1]) - p0;
+        const V3 e2 = f4xyz(pos[t.i2]) - p0;
+        edges[3 * i] = make_float4(p0.x, p0.y, p0.z, 0.0f);
+        edges[3 * i + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+        edges[3 * i + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    }
+}
+
 }  // namespace
+
+lucida_status build_tri_edges(lucida_ctx* ctx, lucida_scene* scene) {
+    const size_t n = scene->num_triangles;
+    if (!scene->tri_edges) LUCIDA_HIP(hipMalloc(&scene->tri_edges, n * 3 * sizeof(float4)), LUCIDA_OOM);
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_tri_edges, dim3(blocks), dim3(256), 0, ctx->stream, scene->triangles,
+                       reinterpret_cast<const float4*>(scene->positions), n, scene->tri_edges);
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    return LUCIDA_OK;
+}
 
 lucida_status ensure_ggx_table(lucida_ctx* ctx) {
     if (ctx->ggx_ready) return LUCIDA_OK;
@@ -774,6 +803,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     sc.pos = reinterpret_cast<const float4*>(scene->positions);
     sc.attrs = scene->vertex_attrs;
     sc.tris = scene->triangles;
+    sc.edges = scene->tri_edges;
     sc.nodes = scene->bvh_nodes;
     sc.lights = scene->lights;
     sc.ggx = ctx->ggx_table.as<float>();
