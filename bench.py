@@ -30,6 +30,19 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mpaths/s @1M tris 8 bounces; time-to-relMSE 0.01: denoised vs plain"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = ROOT / "profiles" / "wavefront_c2_1024x16spp_hbm_bytes.md"
+
+
+def measured_extend_traffic():
+    """HBM bytes per EXTEND launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (counters scaled by the copy-kernel calibration), or None."""
+    try:
+        for line in TRAFFIC_SUMMARY.read_text().splitlines():
+            if line.startswith("extend_hbm_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def lbvh_build_rate(L, gpu, blob, runs: int = 5) -> dict:
@@ -191,6 +204,9 @@ def main() -> int:
     ap.add_argument("--relmse-ref-spp", type=int, default=4096)
     ap.add_argument("--relmse-max-spp", type=int, default=1024)
     ap.add_argument("--no-relmse", action="store_true")
+    ap.add_argument("--net-scenes", type=int, default=24, help="training scenes for the learned denoiser")
+    ap.add_argument("--net-ref-spp", type=int, default=512, help="target samples per pixel of the training pairs")
+    ap.add_argument("--net-steps", type=int, default=1500, help="Adam steps (0: no learned denoiser)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the library's streaming copy of 256 MiB three times (the known-size kernel "
                          "the FETCH_SIZE / WRITE_SIZE counters are scaled by) and exit")
@@ -291,7 +307,9 @@ def main() -> int:
                    "spp": a.spp, "max_depth": a.depth, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "extend (closest-hit traversal)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": measured_extend_traffic(),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the copy calibration "
+                                       "(profiles/wavefront_c2_1024x16spp_hbm_bytes.md)",
                      "avg_launch_ms": round(ext_ms / max(launches["extend"], 1), 4),
                      "bytes_per_launch": int(ext_bytes / max(launches["extend"], 1))},
         "extra": {
@@ -308,8 +326,25 @@ def main() -> int:
         },
     }
     if not a.no_relmse:
+        net = None
+        if a.net_steps > 0:
+            from lucida_amd import denoiser_training as T
+            t0 = time.perf_counter()
+            pairs, data_paths = T.make_training_set(gpu, a.net_scenes, ref_spp=a.net_ref_spp)
+            t1 = time.perf_counter()
+            net = L.LearnedDenoiser(gpu, seed=1)
+            losses = T.train(gpu, net, pairs, a.net_steps)
+            t2 = time.perf_counter()
+            del pairs
+            out["learned_denoiser"] = {
+                "params": L.LearnedDenoiser.param_count(),
+                "training_scenes": a.net_scenes, "crops": "8 x 128x128 per step", "steps": a.net_steps,
+                "target_spp": a.net_ref_spp, "data_paths": data_paths, "data_s": round(t1 - t0, 2),
+                "train_s": round(t2 - t1, 2), "ms_per_step": round((t2 - t1) / a.net_steps * 1e3, 3),
+                "loss_first_100": float(np.mean(losses[:100])), "loss_last_100": float(np.mean(losses[-100:])),
+                "held_out": "the benchmark scene (seed 0, frequency 229) is not in the training set"}
         out["time_to_relmse"] = time_to_relmse(L, gpu, scene, blob, a.width, a.height, a.depth, 0.01,
-                                               a.relmse_ref_spp, a.relmse_max_spp)
+                                               a.relmse_ref_spp, a.relmse_max_spp, net)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(blob, u, a.cpu_seconds)
     else:

@@ -264,6 +264,7 @@ const char* lucida_last_error(void);
 /* Gpu::new (crates/gpu/src/lib.rs:144). Binds GPU `device`, creates the
  * context's stream and loads the kernels. NO_ADAPTER when no GPU is visible. */
 lucida_status lucida_ctx_create(int device, lucida_ctx** out);
+/* Scenes, films and networks created on a context must be destroyed before it. */
 void lucida_ctx_destroy(lucida_ctx* ctx);
 /* Gpu::check_errors (crates/gpu/src/lib.rs:204): waits for the context's stream
  * and surfaces any asynchronous device error as VALIDATION. */

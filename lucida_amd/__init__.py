@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -303,9 +304,17 @@ class Gpu:
         h = C.c_void_p()
         _check(lib().lucida_ctx_create(device, C.byref(h)))
         self._h = h
+        # Scenes, films and networks of this context: the C ABI requires them
+        # to be destroyed before their context, so close() releases them first.
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj) -> None:
+        self._children.add(obj)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
+            for child in list(self._children):
+                child._release()
             lib().lucida_ctx_destroy(self._h)
             self._h = None
 
@@ -385,12 +394,16 @@ class Scene:
         h = C.c_void_p()
         _check(lib().lucida_scene_create(gpu._h, C.byref(d), C.byref(h)))
         self._h = h
+        gpu._adopt(self)
         self.num_triangles = d.num_triangles
 
-    def __del__(self):
+    def _release(self) -> None:
         if getattr(self, "_h", None):
             lib().lucida_scene_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self._release()
 
     def num_bvh_nodes(self) -> int:
         n = C.c_size_t()
@@ -413,11 +426,15 @@ class Film:
         h = C.c_void_p()
         _check(lib().lucida_film_create(gpu._h, width, height, C.byref(h)))
         self._h = h
+        gpu._adopt(self)
 
-    def __del__(self):
+    def _release(self) -> None:
         if getattr(self, "_h", None):
             lib().lucida_film_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self._release()
 
     def clear(self) -> None:
         _check(lib().lucida_film_clear(self._h))
@@ -492,12 +509,16 @@ class LearnedDenoiser:
         h = C.c_void_p()
         _check(lib().lucida_net_create(gpu._h, seed, C.byref(h)))
         self._h = h
+        gpu._adopt(self)
         self._crop = None
 
-    def __del__(self):
+    def _release(self) -> None:
         if getattr(self, "_h", None):
             lib().lucida_net_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self._release()
 
     @staticmethod
     def param_count() -> int:

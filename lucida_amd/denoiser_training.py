@@ -1,0 +1,120 @@
+"""Training data and training loop for the learned denoiser.
+
+The tracer renders procedurally varied scenes twice: a low-sample input film
+(1 to 16 samples per pixel, with its albedo / normal / depth guides) and an
+independent high-sample target film of the same view (another frame seed, so
+no sample is shared). The network is trained on random crops of those pairs
+by the library's own forward, backward and Adam kernels; everything stays
+resident on the device.
+
+    python -m lucida_amd.denoiser_training --scenes 24 --steps 1500 --out weights.f32
+"""
+from __future__ import annotations
+
+import argparse
+import math
+import time
+
+import numpy as np
+
+import lucida_amd as L
+
+
+def _rotate_y(v, angle: float):
+    c, s = math.cos(angle), math.sin(angle)
+    return (c * v[0] + s * v[2], v[1], -s * v[0] + c * v[2])
+
+
+def orbit_camera(u: L.Uniforms, angle: float, lift: float = 0.0) -> L.Uniforms:
+    """The scene's camera turned by `angle` about the vertical axis through the
+    origin and raised by `lift` (camera frame vectors rotate with it)."""
+    for name in ("cam_origin", "cam_upper_left"):
+        v = _rotate_y(tuple(getattr(u, name)), angle)
+        getattr(u, name)[:] = (v[0], v[1] + lift, v[2])
+    for name in ("cam_horizontal", "cam_vertical"):
+        getattr(u, name)[:] = _rotate_y(tuple(getattr(u, name)), angle)
+    return u
+
+
+def scene_variant(k: int, rng: np.random.Generator):
+    """Scene k of the training set: mostly displaced spheres in the room with
+    random seeds, noise amplitudes and tessellations, some Cornell boxes."""
+    kind = rng.random()
+    if kind < 0.8:
+        blob = L.Blob("sphere-c2", seed=1000 + k, frequency=int(rng.integers(24, 96)),
+                      displacement=float(rng.uniform(0.01, 0.12)))
+        return blob, float(rng.uniform(-0.9, 0.9)), float(rng.uniform(-0.2, 0.4))
+    return L.Blob("box-c1" if kind < 0.9 else "cornell-mesh"), 0.0, 0.0
+
+
+def make_training_set(gpu: L.Gpu, scenes: int = 24, width: int = 256, height: int = 256,
+                      input_spp=(1, 2, 4, 8, 16), ref_spp: int = 512, depth: int = 8, seed: int = 7,
+                      log=None):
+    """Renders the (input, target) film pairs; returns them and the paths traced."""
+    rng = np.random.default_rng(seed)
+    pairs = []
+    paths = 0
+    for k in range(scenes):
+        blob, angle, lift = scene_variant(k, rng)
+        scene = L.Scene(gpu, blob, build_lbvh=True)
+        spp = int(rng.choice(input_spp))
+        mk = lambda s, fs: orbit_camera(blob.uniforms(width, height, samples=s, max_depth=depth,
+                                                       sampling=L.SAMPLING_MIS, sampler=L.SAMPLER_SOBOL,
+                                                       frame_seed=fs), angle, lift)
+        film_in = L.Film(gpu, width, height)
+        film_ref = L.Film(gpu, width, height)
+        L.render_wavefront(gpu, scene, mk(spp, 0x5EED0000 + k), spp, film_in)
+        L.render_wavefront(gpu, scene, mk(ref_spp, 0x7A7A0000 + k), ref_spp, film_ref)
+        paths += width * height * (spp + ref_spp)
+        pairs.append((film_in, film_ref))
+        if log:
+            log(f"scene {k}: {blob.desc().num_triangles} tris, input {spp} spp")
+    gpu.check_errors()
+    return pairs, paths
+
+
+def train(gpu: L.Gpu, net: L.LearnedDenoiser, pairs, steps: int = 1500, batch: int = 8, crop_w: int = 128,
+          crop_h: int = 128, lr: float = 1e-3, seed: int = 11, log_every: int = 0, log=None):
+    """Adam on random crops of random pairs; returns the per-step losses."""
+    rng = np.random.default_rng(seed)
+    losses = []
+    for step in range(steps):
+        idx = rng.integers(0, len(pairs), size=batch)
+        ins = [pairs[i][0] for i in idx]
+        refs = [pairs[i][1] for i in idx]
+        crops = [(int(rng.integers(0, ins[j].width - crop_w + 1)), int(rng.integers(0, ins[j].height - crop_h + 1)))
+                 for j in range(batch)]
+        # cosine decay to a tenth of the initial rate
+        rate = lr * (0.1 + 0.9 * 0.5 * (1.0 + math.cos(math.pi * step / max(steps, 1))))
+        losses.append(net.train_step(ins, refs, crop_w, crop_h, crops=crops, lr=rate))
+        if log and log_every and (step + 1) % log_every == 0:
+            log(f"step {step + 1}: loss {np.mean(losses[-log_every:]):.5f}")
+    gpu.check_errors()
+    return losses
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scenes", type=int, default=24)
+    ap.add_argument("--ref-spp", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", type=str, default="")
+    a = ap.parse_args()
+    gpu = L.Gpu(0)
+    t0 = time.perf_counter()
+    pairs, paths = make_training_set(gpu, a.scenes, ref_spp=a.ref_spp, log=print)
+    t1 = time.perf_counter()
+    print(f"data: {len(pairs)} pairs, {paths / 1e9:.2f} Gpaths in {t1 - t0:.1f} s", flush=True)
+    net = L.LearnedDenoiser(gpu, seed=1)
+    losses = train(gpu, net, pairs, a.steps, a.batch, lr=a.lr, log_every=100, log=lambda s: print(s, flush=True))
+    print(f"trained {a.steps} steps in {time.perf_counter() - t1:.1f} s, final loss {np.mean(losses[-100:]):.5f}")
+    if a.out:
+        net.params().tofile(a.out)
+    gpu.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -191,12 +191,28 @@ def forward(params, x, bf16=False):
     return A
 
 
-def loss_and_grad(params, x, tgt, bf16=False):
+def activations_from_layers(x, layer_outputs, bf16=False):
+    """The full activation set from given layer outputs l0..l14 (e.g. the
+    device's), with the pools and concatenations recomputed from them."""
+    R = to_bf16 if bf16 else (lambda v: v)
+    A = {"x": np.asarray(x, np.float64)}
+    for i, v in enumerate(layer_outputs):
+        A[f"l{i}"] = np.asarray(v, np.float64)
+    A["p0"], A["p1"], A["p2"] = R(pool(A["l1"])), R(pool(A["l3"])), R(pool(A["l5"]))
+    A["c2"] = upcat(A["l7"], A["l5"])
+    A["c1"] = upcat(A["l9"], A["l3"])
+    A["c0"] = upcat(A["l11"], A["l1"])
+    return A
+
+
+def loss_and_grad(params, x, tgt, bf16=False, acts=None):
     """Mean squared error over the 3 used output channels and its gradient
     with respect to the flat parameters. Also returns the activation
-    gradients ('d*' keys), computed as the device stores them."""
+    gradients ('d*' keys), computed as the device stores them. `acts`
+    (from activations_from_layers) replaces the forward pass, so the backward
+    pass can be checked on its own."""
     R = to_bf16 if bf16 else (lambda v: v)
-    A = forward(params, x, bf16)
+    A = forward(params, x, bf16) if acts is None else acts
     layers = [(R(W), b) for W, b in unpack(np.asarray(params, np.float64))]
     out = A["l14"]
     e = out[..., :3] - tgt
