@@ -204,9 +204,9 @@ def main() -> int:
     ap.add_argument("--relmse-ref-spp", type=int, default=4096)
     ap.add_argument("--relmse-max-spp", type=int, default=1024)
     ap.add_argument("--no-relmse", action="store_true")
-    ap.add_argument("--net-scenes", type=int, default=24, help="training scenes for the learned denoiser")
-    ap.add_argument("--net-ref-spp", type=int, default=512, help="target samples per pixel of the training pairs")
-    ap.add_argument("--net-steps", type=int, default=1500, help="Adam steps (0: no learned denoiser)")
+    ap.add_argument("--net-scenes", type=int, default=32, help="training scenes for the learned denoiser")
+    ap.add_argument("--net-ref-spp", type=int, default=1024, help="target samples per pixel of the training pairs")
+    ap.add_argument("--net-steps", type=int, default=3000, help="Adam steps (0: no learned denoiser)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the library's streaming copy of 256 MiB three times (the known-size kernel "
                          "the FETCH_SIZE / WRITE_SIZE counters are scaled by) and exit")

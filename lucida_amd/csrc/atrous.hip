@@ -134,6 +134,43 @@ __global__ __launch_bounds__(TX* TY) void k_dn_modulate(const float4* __restrict
 }
 
 }  // namespace
+
+// Grows the context's a-trous planes to `pixels` (waits for queued work first).
+lucida_status atrous_reserve(lucida_ctx* ctx, size_t pixels) {
+    const size_t want = pixels * (16 * 4 + 4);
+    if (want > ctx->dn_buf.bytes) {
+        LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
+        LUCIDA_HIP(ctx->dn_buf.ensure(want), LUCIDA_OOM);
+    }
+    return LUCIDA_OK;
+}
+
+// Queues the filter of a w x h Accum film into `out` on the context's stream;
+// the planes must have been reserved for w * h pixels.
+void atrous_enqueue(lucida_ctx* ctx, const GpuAccum* in, uint32_t w, uint32_t h, const lucida_denoise_params& params,
+                    GpuAccum* out) {
+    const size_t n = (size_t)w * h;
+    float4* a = ctx->dn_buf.as<float4>();
+    float4* b = a + n;
+    float4* gnd = a + 2 * n;
+    float4* galb = a + 3 * n;
+    float* dev = reinterpret_cast<float*>(a + 4 * n);
+    const dim3 blk(TX, TY), grd((w + TX - 1) / TX, (h + TY - 1) / TY);
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(k_dn_prepare, grd, blk, 0, s, in, w, h, a, gnd, galb);
+    if (params.iterations == 0) {
+        hipLaunchKernelGGL(k_dn_modulate, grd, blk, 0, s, a, gnd, galb, w, h, out);
+    } else {
+        hipLaunchKernelGGL(k_dn_deviation, grd, blk, 0, s, a, w, h, dev);
+        for (uint32_t level = 0; level < params.iterations; ++level) {
+            const bool last = level + 1 == params.iterations;
+            hipLaunchKernelGGL(k_dn_pass, grd, blk, 0, s, a, b, gnd, galb, dev, w, h, 1 << level, params,
+                               last ? out : nullptr);
+            std::swap(a, b);
+        }
+    }
+}
+
 }  // namespace lucida
 
 using namespace lucida;
@@ -148,30 +185,11 @@ lucida_status lucida_denoise_atrous(lucida_ctx* ctx, const lucida_film* in, cons
         return set_error(LUCIDA_INVALID_ARG, "lucida_denoise_atrous: film sizes differ");
     if (params->iterations > 16) return set_error(LUCIDA_INVALID_ARG, "lucida_denoise_atrous: at most 16 iterations");
     clear_error();
-    const uint32_t w = in->width, h = in->height;
-    const size_t n = (size_t)w * h;
-    LUCIDA_HIP(ctx->dn_buf.ensure(n * (16 * 4 + 4)), LUCIDA_OOM);
-    float4* a = ctx->dn_buf.as<float4>();
-    float4* b = a + n;
-    float4* gnd = a + 2 * n;
-    float4* galb = a + 3 * n;
-    float* dev = reinterpret_cast<float*>(a + 4 * n);
-    const dim3 blk(TX, TY), grd((w + TX - 1) / TX, (h + TY - 1) / TY);
-    hipStream_t s = ctx->stream;
-    hipLaunchKernelGGL(k_dn_prepare, grd, blk, 0, s, in->accum, w, h, a, gnd, galb);
-    if (params->iterations == 0) {
-        hipLaunchKernelGGL(k_dn_modulate, grd, blk, 0, s, a, gnd, galb, w, h, out->accum);
-    } else {
-        hipLaunchKernelGGL(k_dn_deviation, grd, blk, 0, s, a, w, h, dev);
-        for (uint32_t level = 0; level < params->iterations; ++level) {
-            const bool last = level + 1 == params->iterations;
-            hipLaunchKernelGGL(k_dn_pass, grd, blk, 0, s, a, b, gnd, galb, dev, w, h, 1 << level, *params,
-                               last ? out->accum : nullptr);
-            std::swap(a, b);
-        }
-    }
+    lucida_status st = atrous_reserve(ctx, (size_t)in->width * in->height);
+    if (st != LUCIDA_OK) return st;
+    atrous_enqueue(ctx, in->accum, in->width, in->height, *params, out->accum);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
-    LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
     return LUCIDA_OK;
 }
 

@@ -98,4 +98,9 @@ namespace lucida {
 lucida_status ensure_ggx_table(lucida_ctx* ctx);
 // Fills scene->tri_edges from its (final, leaf-ordered) triangles.
 lucida_status build_tri_edges(lucida_ctx* ctx, lucida_scene* scene);
+// A-trous filter (atrous.hip) without the final wait: reserve the planes for
+// the largest film first, then queue filters on the context's stream.
+lucida_status atrous_reserve(lucida_ctx* ctx, size_t pixels);
+void atrous_enqueue(lucida_ctx* ctx, const GpuAccum* in, uint32_t w, uint32_t h, const lucida_denoise_params& params,
+                    GpuAccum* out);
 }  // namespace lucida

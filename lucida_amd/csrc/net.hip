@@ -15,6 +15,7 @@
 // weights; the weight-gradient pass stages 32-pixel chunks through LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -301,12 +302,14 @@ __global__ void k_upcat_bwd(const bf16* __restrict__ dcat, int c1, int c2, int n
 
 // Input features from an Accum film (means): log1p(radiance / albedo) (3),
 // albedo (3), normal (3), depth / (1 + depth) (1), log1p of the demodulated
-// luminance's standard error (1), log2(samples) / 16 (1), four zero channels. With
+// luminance's standard error (1), log2(samples) / 16 (1), the a-trous
+// estimate log1p(filtered / albedo) (3), one zero channel. With
 // `target` non-null also writes the target log1p(ref radiance / albedo) (3
 // floats per pixel) from the reference film, demodulated by the INPUT film's
 // albedo so inference can remodulate with the same guide.
-__global__ void k_features(const GpuAccum* __restrict__ film, const GpuAccum* __restrict__ ref, int fw, int x0, int y0,
-                           int cw, int ch, int img, bf16* __restrict__ feat, float* __restrict__ target) {
+__global__ void k_features(const GpuAccum* __restrict__ film, const GpuAccum* __restrict__ base,
+                           const GpuAccum* __restrict__ ref, int fw, int x0, int y0, int cw, int ch, int img,
+                           bf16* __restrict__ feat, float* __restrict__ target) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= cw || y >= ch) return;
     const size_t src = (size_t)(y0 + y) * fw + (x0 + x);
@@ -332,7 +335,14 @@ __global__ void k_features(const GpuAccum* __restrict__ film, const GpuAccum* __
     const float al_l = 0.2126f * al[0] + 0.7152f * al[1] + 0.0722f * al[2];
     f[10] = log1pf(sqrtf(var_l * inv) / al_l);
     f[11] = log2f(ns) * (1.0f / 16.0f);
-    for (int k = 12; k < 16; ++k) f[k] = 0.0f;
+    // The a-trous estimate of the same film, demodulated like channels 0..2:
+    // the network predicts a correction to it.
+    const float4 b0 = reinterpret_cast<const float4*>(base + src)[0];
+    const float binv = 1.0f / fmaxf(b0.w, 1.0f);
+    f[12] = log1pf(fmaxf(b0.x * binv / al[0], 0.0f));
+    f[13] = log1pf(fmaxf(b0.y * binv / al[1], 0.0f));
+    f[14] = log1pf(fmaxf(b0.z * binv / al[2], 0.0f));
+    f[15] = 0.0f;
     const size_t dst = ((size_t)img * ch + y) * cw + x;
     bf16x8 lo, hi;
     for (int k = 0; k < 8; ++k) {
@@ -349,10 +359,11 @@ __global__ void k_features(const GpuAccum* __restrict__ film, const GpuAccum* __
     }
 }
 
-// Mean squared error over the 3 used output channels; writes dL/dout (bf16,
+// Mean squared error of (output + a-trous estimate) against the target over
+// the 3 used channels; writes dL/dout (bf16,
 // zero on the padding channels) and a per-block partial loss.
-__global__ void k_loss(const bf16* __restrict__ out, const float* __restrict__ target, size_t pixels,
-                       bf16* __restrict__ dout, float* __restrict__ loss_partial) {
+__global__ void k_loss(const bf16* __restrict__ out, const bf16* __restrict__ feat, const float* __restrict__ target,
+                       size_t pixels, bf16* __restrict__ dout, float* __restrict__ loss_partial) {
     __shared__ float red[256];
     float s = 0.0f;
     const float scale = 2.0f / (float)(pixels * 3);
@@ -363,7 +374,7 @@ __global__ void k_loss(const bf16* __restrict__ out, const float* __restrict__ t
             d1[k] = f2b(0.0f);
         }
         for (int k = 0; k < 3; ++k) {
-            const float e = b2f(out[p * 16 + k]) - target[p * 3 + k];
+            const float e = (b2f(out[p * 16 + k]) + b2f(feat[p * 16 + 12 + k])) - target[p * 3 + k];
             s += e * e;
             d0[k] = f2b(scale * e);
         }
@@ -380,9 +391,10 @@ __global__ void k_loss(const bf16* __restrict__ out, const float* __restrict__ t
 }
 
 // Remodulated prediction into an Accum-shaped film: radiance =
-// expm1(max(out, 0)) * albedo, samples = 1, guides copied as means.
-__global__ void k_compose(const bf16* __restrict__ out, const GpuAccum* __restrict__ film, size_t pixels,
-                          GpuAccum* __restrict__ dst) {
+// expm1(max(out + a-trous estimate, 0)) * albedo, samples = 1, guides copied
+// as means.
+__global__ void k_compose(const bf16* __restrict__ out, const bf16* __restrict__ feat,
+                          const GpuAccum* __restrict__ film, size_t pixels, GpuAccum* __restrict__ dst) {
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pixels) return;
     const float4* a = reinterpret_cast<const float4*>(film + p);
@@ -390,7 +402,8 @@ __global__ void k_compose(const bf16* __restrict__ out, const GpuAccum* __restri
     const float inv = 1.0f / fmaxf(a[0].w, 1.0f);
     const float al[3] = {fmaxf(a1.x * inv, 1e-3f), fmaxf(a1.y * inv, 1e-3f), fmaxf(a1.z * inv, 1e-3f)};
     float rad[3];
-    for (int k = 0; k < 3; ++k) rad[k] = expm1f(fmaxf(b2f(out[p * 16 + k]), 0.0f)) * al[k];
+    for (int k = 0; k < 3; ++k)
+        rad[k] = expm1f(fmaxf(b2f(out[p * 16 + k]) + b2f(feat[p * 16 + 12 + k]), 0.0f)) * al[k];
     float4* o = reinterpret_cast<float4*>(dst + p);
     o[0] = make_float4(rad[0], rad[1], rad[2], 1.0f);
     o[1] = make_float4(al[0], al[1], al[2], a1.w * inv);
@@ -411,6 +424,10 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ grad, fl
         p[e] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
     }
 }
+
+// The a-trous filter the network corrects: DenoiseParams' defaults
+// (crates/core/src/denoise.rs:62-94).
+const lucida_denoise_params kAtrousDefaults = {4u, 64.0f, 0.08f, 4.0f, 0.1f};
 
 inline unsigned grid1(size_t n, int block = 256) {
     size_t g = (n + block - 1) / block;
@@ -468,6 +485,7 @@ struct lucida_net {
     DevBuf frag_fwd, frag_bwd;             // bf16 fragments
     size_t frag_fwd_off[kNetNumLayers], frag_bwd_off[kNetNumLayers], param_off[kNetNumLayers];
     DevBuf acts, dacts, target, loss_partial;
+    DevBuf base;  // a-trous estimate of the film being loaded (Accum-shaped)
     size_t act_capacity = 0;
     uint32_t step = 0;
     bool frags_valid = false;
@@ -736,7 +754,7 @@ void lucida_net_destroy(lucida_net* net) {
     if (!net) return;
     (void)hipStreamSynchronize(net->ctx->stream);
     DevBuf* b[] = {&net->params, &net->grads, &net->adam_m, &net->adam_v, &net->frag_fwd, &net->frag_bwd,
-                   &net->acts, &net->dacts, &net->target, &net->loss_partial};
+                   &net->acts, &net->dacts, &net->target, &net->loss_partial, &net->base};
     for (DevBuf* x : b) x->release();
     delete net;
 }
@@ -776,14 +794,26 @@ static lucida_status load_batch(lucida_net* net, const lucida_film* const* input
     }
     lucida_status st = net_ensure_acts(net, (int)count, (int)h, (int)w);
     if (st != LUCIDA_OK) return st;
+    size_t max_pixels = 0;
+    for (uint32_t k = 0; k < count; ++k)
+        max_pixels = std::max<size_t>(max_pixels, (size_t)inputs[k]->width * inputs[k]->height);
+    if (max_pixels * sizeof(GpuAccum) > net->base.bytes) {
+        LUCIDA_HIP(hipStreamSynchronize(net->ctx->stream), LUCIDA_VALIDATION);
+        LUCIDA_HIP(net->base.ensure(max_pixels * sizeof(GpuAccum)), LUCIDA_OOM);
+    }
+    st = atrous_reserve(net->ctx, max_pixels);
+    if (st != LUCIDA_OK) return st;
     const ActPlan a = plan_acts(net->n, net->h, net->w);
     bf16* X = net->acts.as<bf16>() + a.x;
     for (uint32_t k = 0; k < count; ++k) {
         const lucida_film* f = inputs[k];
         const uint32_t cx = x0 ? x0[k] : 0, cy = y0 ? y0[k] : 0;
+        // The whole film is filtered (crops then see the filter's real
+        // neighbourhood); stream order keeps one scratch film enough.
+        atrous_enqueue(net->ctx, f->accum, f->width, f->height, kAtrousDefaults, net->base.as<GpuAccum>());
         hipLaunchKernelGGL(k_features, dim3((w + 127) / 128, h), dim3(128), 0, net->ctx->stream, f->accum,
-                           refs ? refs[k]->accum : nullptr, (int)f->width, (int)cx, (int)cy, (int)w, (int)h, (int)k, X,
-                           refs ? net->target.as<float>() : nullptr);
+                           net->base.as<GpuAccum>(), refs ? refs[k]->accum : nullptr, (int)f->width, (int)cx, (int)cy,
+                           (int)w, (int)h, (int)k, X, refs ? net->target.as<float>() : nullptr);
     }
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
     return LUCIDA_OK;
@@ -802,7 +832,8 @@ lucida_status lucida_net_forward_backward(lucida_net* net, const lucida_film* co
     const size_t pixels = (size_t)net->n * net->h * net->w;
     const unsigned blocks = 1024;
     hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, net->ctx->stream, net->acts.as<bf16>() + a.l[14],
-                       net->target.as<float>(), pixels, net->dacts.as<bf16>() + a.l[14], net->loss_partial.as<float>());
+                       net->acts.as<bf16>() + a.x, net->target.as<float>(), pixels, net->dacts.as<bf16>() + a.l[14],
+                       net->loss_partial.as<float>());
     st = net_backward(net);
     if (st != LUCIDA_OK) return st;
     if (loss) {
@@ -852,7 +883,7 @@ lucida_status lucida_net_denoise(lucida_net* net, const lucida_film* in, lucida_
     const ActPlan a = plan_acts(net->n, net->h, net->w);
     const size_t pixels = (size_t)in->width * in->height;
     hipLaunchKernelGGL(k_compose, dim3(grid1(pixels)), dim3(256), 0, net->ctx->stream, net->acts.as<bf16>() + a.l[14],
-                       in->accum, pixels, out->accum);
+                       net->acts.as<bf16>() + a.x, in->accum, pixels, out->accum);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
     LUCIDA_HIP(hipStreamSynchronize(net->ctx->stream), LUCIDA_VALIDATION);
     return LUCIDA_OK;

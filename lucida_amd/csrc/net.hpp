@@ -9,7 +9,7 @@ struct NetLayer {
 };
 
 constexpr int kNetNumLayers = 15;
-constexpr int kNetInputsUsed = 12;  // feature channels 12..15 are zero
+constexpr int kNetInputsUsed = 15;  // feature channel 15 is zero
 extern const NetLayer kNetLayers[kNetNumLayers];
 
 // Parameters: per layer W[9][cin][cout] (tap = 3 * dy + dx) then b[cout], fp32.

@@ -1,13 +1,13 @@
 """Training data and training loop for the learned denoiser.
 
 The tracer renders procedurally varied scenes twice: a low-sample input film
-(1 to 16 samples per pixel, with its albedo / normal / depth guides) and an
+(1 to 32 samples per pixel, with its albedo / normal / depth guides) and an
 independent high-sample target film of the same view (another frame seed, so
 no sample is shared). The network is trained on random crops of those pairs
 by the library's own forward, backward and Adam kernels; everything stays
 resident on the device.
 
-    python -m lucida_amd.denoiser_training --scenes 24 --steps 1500 --out weights.f32
+    python -m lucida_amd.denoiser_training --scenes 32 --steps 3000 --out weights.f32
 """
 from __future__ import annotations
 
@@ -48,7 +48,7 @@ def scene_variant(k: int, rng: np.random.Generator):
 
 
 def make_training_set(gpu: L.Gpu, scenes: int = 24, width: int = 256, height: int = 256,
-                      input_spp=(1, 2, 4, 8, 16), ref_spp: int = 512, depth: int = 8, seed: int = 7,
+                      input_spp=(1, 2, 4, 8, 16, 32), ref_spp: int = 1024, depth: int = 8, seed: int = 7,
                       log=None):
     """Renders the (input, target) film pairs; returns them and the paths traced."""
     rng = np.random.default_rng(seed)
@@ -95,9 +95,9 @@ def train(gpu: L.Gpu, net: L.LearnedDenoiser, pairs, steps: int = 1500, batch: i
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--scenes", type=int, default=24)
-    ap.add_argument("--ref-spp", type=int, default=512)
-    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--scenes", type=int, default=32)
+    ap.add_argument("--ref-spp", type=int, default=1024)
+    ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", type=str, default="")

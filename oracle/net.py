@@ -23,7 +23,7 @@ LAYERS = [
     (160, 64, 1), (64, 64, 1), (112, 48, 1), (48, 48, 1), (80, 32, 1), (32, 32, 1), (32, 16, 0),
 ]
 LEVEL = [0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0]
-INPUTS_USED = 12
+INPUTS_USED = 15
 
 
 def param_count() -> int:
@@ -67,8 +67,9 @@ def _means(accum: np.ndarray):
     return a, ns, inv, al, rd
 
 
-def features(accum: np.ndarray) -> np.ndarray:
-    """16 input channels per pixel from Accum sums (shape [..., 16])."""
+def features(accum: np.ndarray, base: np.ndarray | None = None) -> np.ndarray:
+    """16 input channels per pixel from Accum sums (shape [..., 16]); `base`
+    is the a-trous output of the same film (Accum-shaped), channels 12..14."""
     a, ns, inv, al, rd = _means(accum)
     f = np.zeros(a.shape[:-1] + (16,))
     f[..., 0:3] = np.log1p(np.maximum(rd / al, 0.0))
@@ -82,6 +83,10 @@ def features(accum: np.ndarray) -> np.ndarray:
     al_l = (al * lw).sum(-1)
     f[..., 10] = np.log1p(np.sqrt(var_l * inv) / al_l)
     f[..., 11] = np.log2(ns) / 16.0
+    if base is not None:
+        b = np.asarray(base, np.float64)
+        br = b[..., 0:3] / np.maximum(b[..., 3], 1.0)[..., None]
+        f[..., 12:15] = np.log1p(np.maximum(br / al, 0.0))
     return f
 
 
@@ -93,11 +98,12 @@ def target(accum_in: np.ndarray, accum_ref: np.ndarray) -> np.ndarray:
     return np.log1p(np.maximum(rr / al, 0.0))
 
 
-def compose(out: np.ndarray, accum_in: np.ndarray) -> np.ndarray:
-    """Network output -> Accum-shaped film (radiance, samples = 1, guides)."""
+def compose(out: np.ndarray, accum_in: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Network output + the a-trous channels of its features `x` -> Accum-shaped
+    film (radiance, samples = 1, guides)."""
     a, _, inv, al, _ = _means(accum_in)
     res = np.zeros(a.shape[:-1] + (16,))
-    res[..., 0:3] = np.expm1(np.maximum(out[..., 0:3], 0.0)) * al
+    res[..., 0:3] = np.expm1(np.maximum(out[..., 0:3] + x[..., 12:15], 0.0)) * al
     res[..., 3] = 1.0
     res[..., 4:7] = al
     res[..., 7] = a[..., 7] * inv
@@ -206,7 +212,8 @@ def activations_from_layers(x, layer_outputs, bf16=False):
 
 
 def loss_and_grad(params, x, tgt, bf16=False, acts=None):
-    """Mean squared error over the 3 used output channels and its gradient
+    """Mean squared error of (output + a-trous channels of x) against the
+    target over the 3 used output channels, and its gradient
     with respect to the flat parameters. Also returns the activation
     gradients ('d*' keys), computed as the device stores them. `acts`
     (from activations_from_layers) replaces the forward pass, so the backward
@@ -215,7 +222,7 @@ def loss_and_grad(params, x, tgt, bf16=False, acts=None):
     A = forward(params, x, bf16) if acts is None else acts
     layers = [(R(W), b) for W, b in unpack(np.asarray(params, np.float64))]
     out = A["l14"]
-    e = out[..., :3] - tgt
+    e = (out[..., :3] + A["x"][..., 12:15]) - tgt
     loss = float((e * e).mean())
     d = np.zeros_like(out)
     d[..., :3] = R(2.0 / e.size * e)

@@ -113,8 +113,8 @@ def test_network_gradient_matches_finite_differences():
 
 
 def test_padding_channels_receive_no_gradient():
-    """Feature channels 12..15 are zero, so their weights get zero gradient;
-    output channels 3..15 are outside the loss."""
+    """Feature channel 15 is zero, so its weights get zero gradient; output
+    channels 3..15 are outside the loss."""
     rng = np.random.default_rng(8)
     p = N.init_params(2)
     x = np.zeros((1, 8, 16, 16))
@@ -166,8 +166,19 @@ def test_feature_closed_forms():
     np.testing.assert_allclose(f[:, 10], 0.0, atol=1e-12)
     np.testing.assert_allclose(f[:, 11], np.log2(ns) / 16)
     assert np.all(f[:, 12:] == 0)
+    # the a-trous channels: a base film (samples = 1) of radiance albedo * (e^2 - 1) -> 2
+    base = np.zeros((3, 16))
+    base[:, 3] = 1.0
+    base[:, 0:3] = np.maximum(al, 1e-3) * (np.e ** 2 - 1.0)
+    fb = N.features(a, base)
+    np.testing.assert_allclose(fb[:, 12:15], 2.0, rtol=1e-12)
+    np.testing.assert_array_equal(fb[:, :12], f[:, :12])
+    assert np.all(fb[:, 15] == 0)
     # target of the input against itself is feature 0..2; compose inverts it
+    # once the a-trous channels are added back (here: zero output + base 2)
     np.testing.assert_allclose(N.target(a, a), f[:, 0:3], rtol=1e-12)
-    comp = N.compose(f[:, 0:3], a)
+    comp = N.compose(f[:, 0:3], a, f)
     np.testing.assert_allclose(comp[:, 0:3], mean_rad, rtol=1e-12)
     assert np.all(comp[:, 3] == 1.0)
+    comp2 = N.compose(np.zeros((3, 16)), a, fb)
+    np.testing.assert_allclose(comp2[:, 0:3], base[:, 0:3], rtol=1e-12)
