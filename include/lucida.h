@@ -369,11 +369,56 @@ typedef struct lucida_scene_desc {
 
 lucida_status lucida_scene_create(lucida_ctx* ctx, const lucida_scene_desc* desc, lucida_scene** out);
 void lucida_scene_destroy(lucida_scene* scene);
+
+/* One mesh of an instanced scene: a range of desc->triangles. */
+typedef struct lucida_mesh_range {
+    uint32_t first_triangle;
+    uint32_t triangle_count;
+} lucida_mesh_range;
+
+/* One placement (Instance::new, instance.rs:77): the forward object-to-world
+ * transform as a row-major 3x4 [R | t], the mesh it places, and a material
+ * override (0xFFFFFFFF keeps the triangles' own materials). */
+typedef struct lucida_instance_placement {
+    float object_to_world[12];
+    uint32_t mesh;
+    uint32_t material;
+} lucida_instance_placement;
+
+/* Scene::build_instances (crates/core/src/scene.rs:252), on the device: one
+ * LBVH per mesh range (the BLASes, their triangles permuted into leaf order),
+ * an LBVH over the instances' padded world bounds (the TLAS, built over two-
+ * point "triangles" spanning each box), the node arrays rebased and
+ * concatenated BLAS first, TLAS last (tlas_root = its first node), and the
+ * instances appended to the primitives in TLAS leaf order. `desc` carries the
+ * shared geometry and must have no BVH nodes and no instances of its own.
+ * The reference builds both levels with binned SAH on the host; the tree
+ * differs, the hits do not. */
+lucida_status lucida_scene_create_instanced(lucida_ctx* ctx, const lucida_scene_desc* desc,
+                                            const lucida_mesh_range* meshes, size_t num_meshes,
+                                            const lucida_instance_placement* placements, size_t num_placements,
+                                            lucida_scene** out);
+/* Counts and the device's instance records (GpuInstance, in TLAS leaf order)
+ * and TLAS root of a scene, so a host tracer can walk the same structure. */
+lucida_status lucida_scene_counts(lucida_scene* scene, size_t* num_triangles, size_t* num_bvh_nodes,
+                                  size_t* num_instances, uint32_t* tlas_root);
+lucida_status lucida_scene_download_instances(lucida_scene* scene, GpuInstance* out);
 /* Number of BVH nodes the scene holds (after an on-device build). */
 lucida_status lucida_scene_num_bvh_nodes(lucida_scene* scene, size_t* out);
 /* Copies the scene's device BVH nodes and triangles back (host arrays sized
  * by lucida_scene_num_bvh_nodes and the triangle count). */
 lucida_status lucida_scene_download_bvh(lucida_scene* scene, GpuBvhNode* nodes, GpuTriangle* triangles);
+
+/* Batch ray queries against a resident scene without shading: the closest
+ * hit of Scene::intersect (scene.rs:452) or the occlusion test of
+ * Scene::occluded (scene.rs:577), through the same traversal EXTEND and
+ * CONNECT use. rays: n x 8 floats (origin.xyz, unused, direction.xyz, t_max),
+ * t_min is the renderer's T_MIN. hits: n x 4 (t, u, v, id bits), id = triangle
+ * (leaf order), 0x80000000 | analytic primitive, or 0xFFFFFFFF; with any_hit
+ * id is 0 for an occluded ray. instances (optional): the instance primitive
+ * index of a triangle hit in an instanced scene. Host arrays. */
+lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, const float* rays, size_t n, int any_hit,
+                                float* hits, uint32_t* instances);
 
 /* A device-resident accumulation buffer (width*height GpuAccum). */
 lucida_status lucida_film_create(lucida_ctx* ctx, uint32_t width, uint32_t height, lucida_film** out);

@@ -33,7 +33,14 @@ typedef struct lucida_blob_params {
  *   "box-c1"        the 1x1x1 triangle-quad Cornell box with two spheres
  *   "sphere-c2"     the noise-displaced geodesic sphere in a 1,024-triangle
  *                   room lit by 16 emissive quads
- *   "icosphere-L"   a bare icosphere of subdivision level L (mesh.rs:169) */
+ *   "icosphere-L"   a bare icosphere of subdivision level L (mesh.rs:169)
+ *   "instances-c3"  4,096 placements of a 9,680-triangle displaced sphere
+ *                   (frequency 22) in a 100^3 box, 30% rough dielectric, lit
+ *                   by a 2048 x 1024 log-normal environment with three hot spots
+ *   "instances-small" the same recipe at 64 placements of a 320-triangle
+ *                   sphere (frequency 4) in a 10^3 box, environment 256 x 128
+ * Instanced scenes carry meshes and placements (lucida_blob_instancing) and
+ * are uploaded with lucida_scene_create_instanced. */
 lucida_status lucida_blob_create(const char* name, const lucida_blob_params* params, lucida_blob** out);
 void lucida_blob_destroy(lucida_blob* blob);
 
@@ -49,6 +56,10 @@ lucida_status lucida_blob_uniforms(const lucida_blob* blob, uint32_t width, uint
  * triangles directly) and the triangles in leaf order. */
 lucida_status lucida_blob_set_bvh(lucida_blob* blob, const GpuBvhNode* nodes, size_t num_nodes,
                                   const GpuTriangle* triangles_in_leaf_order);
+
+/* The meshes and placements of an instanced scene (empty otherwise). */
+lucida_status lucida_blob_instancing(const lucida_blob* blob, const lucida_mesh_range** meshes, size_t* num_meshes,
+                                     const lucida_instance_placement** placements, size_t* num_placements);
 
 /* Replaces the blob's environment map (RGBA32F, w*h) and builds its 2D CDF
  * (envmap.rs:112 EnvMap::new). */

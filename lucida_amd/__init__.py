@@ -36,8 +36,12 @@ ACCUM = np.dtype([("radiance", "<f4", 3), ("samples", "<f4"), ("albedo", "<f4", 
 LIGHT = np.dtype([("origin", "<f4", 3), ("area", "<f4"), ("edge_u", "<f4", 3), ("kind", "<u4"),
                   ("edge_v", "<f4", 3), ("material", "<u4"), ("normal", "<f4", 3), ("_pad0", "<f4")])
 AABB = np.dtype([("lo", "<f4", 3), ("_p0", "<f4"), ("hi", "<f4", 3), ("_p1", "<f4")])
+INSTANCE = np.dtype([("inv_row0", "<f4", 3), ("blas_root", "<u4"), ("inv_row1", "<f4", 3), ("kind", "<u4"),
+                     ("inv_row2", "<f4", 3), ("material", "<u4"), ("inv_translation", "<f4", 3), ("_pad0", "<f4")])
+MESH_RANGE = np.dtype([("first_triangle", "<u4"), ("triangle_count", "<u4")])
+PLACEMENT = np.dtype([("object_to_world", "<f4", 12), ("mesh", "<u4"), ("material", "<u4")])
 for _dt, _size in ((MATERIAL, 80), (PRIMITIVE, 64), (VERTEX_ATTR, 32), (TRIANGLE, 16), (BVH_NODE, 32),
-                   (ACCUM, 64), (LIGHT, 64), (AABB, 32)):
+                   (ACCUM, 64), (LIGHT, 64), (AABB, 32), (INSTANCE, 64), (MESH_RANGE, 8), (PLACEMENT, 56)):
     assert _dt.itemsize == _size
 
 
@@ -154,6 +158,11 @@ def lib() -> C.CDLL:
             "lucida_scene_destroy": ([P], None),
             "lucida_scene_num_bvh_nodes": ([P, C.POINTER(S)], I),
             "lucida_scene_download_bvh": ([P, P, P], I),
+            "lucida_scene_create_instanced": ([P, C.POINTER(SceneDesc), P, S, P, S, C.POINTER(P)], I),
+            "lucida_scene_counts": ([P, C.POINTER(S), C.POINTER(S), C.POINTER(S), C.POINTER(U32)], I),
+            "lucida_scene_download_instances": ([P, P], I),
+            "lucida_scene_cast": ([P, P, P, S, I, P, P], I),
+            "lucida_blob_instancing": ([P, C.POINTER(P), C.POINTER(S), C.POINTER(P), C.POINTER(S)], I),
             "lucida_film_create": ([P, U32, U32, C.POINTER(P)], I),
             "lucida_film_destroy": ([P], None),
             "lucida_film_clear": ([P], I),
@@ -266,6 +275,12 @@ class Blob:
             "env_width": d.env_width, "env_height": d.env_height, "env_total_weight": d.env_total_weight,
             "tlas_root": d.tlas_root,
         }
+
+    def instancing(self) -> tuple[np.ndarray, np.ndarray]:
+        """Copies of an instanced scene's mesh ranges and placements (empty otherwise)."""
+        mp, mn, pp, pn = C.c_void_p(), C.c_size_t(), C.c_void_p(), C.c_size_t()
+        _check(lib().lucida_blob_instancing(self._h, C.byref(mp), C.byref(mn), C.byref(pp), C.byref(pn)))
+        return self._arr(mp.value, mn.value, MESH_RANGE), self._arr(pp.value, pn.value, PLACEMENT)
 
     def uniforms(self, width: int, height: int, *, samples: int = 1, max_depth: int = 8,
                  frame_seed: int = 0x5EED1234, sampling: int = SAMPLING_MIS, sampler: int = SAMPLER_SOBOL,
@@ -392,10 +407,41 @@ class Scene:
         d = blob.desc()
         d.build_lbvh = 1 if build_lbvh else 0
         h = C.c_void_p()
-        _check(lib().lucida_scene_create(gpu._h, C.byref(d), C.byref(h)))
+        meshes, placements = blob.instancing()
+        if len(placements):
+            # Scene::build_instances on the device: BLAS per mesh + TLAS
+            _check(lib().lucida_scene_create_instanced(gpu._h, C.byref(d), _ptr(meshes), len(meshes),
+                                                       _ptr(placements), len(placements), C.byref(h)))
+        else:
+            _check(lib().lucida_scene_create(gpu._h, C.byref(d), C.byref(h)))
         self._h = h
         gpu._adopt(self)
         self.num_triangles = d.num_triangles
+
+    def counts(self) -> dict:
+        nt, nn, ni, root = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_uint32()
+        _check(lib().lucida_scene_counts(self._h, C.byref(nt), C.byref(nn), C.byref(ni), C.byref(root)))
+        return {"triangles": nt.value, "bvh_nodes": nn.value, "instances": ni.value, "tlas_root": root.value}
+
+    def download_instances(self) -> np.ndarray:
+        out = np.empty(self.counts()["instances"], INSTANCE)
+        _check(lib().lucida_scene_download_instances(self._h, _ptr(out)))
+        return out
+
+    def host_desc(self, blob: Blob) -> tuple[SceneDesc, list]:
+        """The blob's desc with the device's BVH nodes, leaf-ordered triangles,
+        instances and TLAS root, for a host tracer to walk the same structure.
+        Returns the desc and the arrays it points into (keep them alive)."""
+        d = blob.desc()
+        c = self.counts()
+        nodes, tris = self.download_bvh()
+        inst = self.download_instances()
+        keep = [nodes, tris, inst]
+        d.bvh_nodes, d.num_bvh_nodes = nodes.ctypes.data, len(nodes)
+        d.triangles = tris.ctypes.data
+        d.instances, d.num_instances = (inst.ctypes.data if len(inst) else None), len(inst)
+        d.tlas_root = c["tlas_root"]
+        return d, keep
 
     def _release(self) -> None:
         if getattr(self, "_h", None):
@@ -458,6 +504,19 @@ def render_wavefront(gpu: Gpu, scene: Scene, uniforms: Uniforms, samples: int, f
     _check(lib().lucida_render_wavefront(gpu._h, scene._h, C.byref(uniforms), samples, film._h, cfg,
                                          C.byref(st)))
     return st
+
+
+def scene_cast(gpu: Gpu, scene: Scene, rays: np.ndarray, any_hit: bool = False):
+    """Batch closest-hit (or occlusion) queries through the renderer's own
+    traversal: rays (n, 8) float32 = origin, unused, direction, t_max. Returns
+    hits (n, 4) float32 = t, u, v, id bits (triangle, 0x80000000 | primitive or
+    0xFFFFFFFF) and the instance of each instanced hit."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    n = len(rays)
+    hits = np.zeros((n, 4), np.float32)
+    inst = np.zeros(n, np.uint32)
+    _check(lib().lucida_scene_cast(gpu._h, scene._h, _ptr(rays), n, 1 if any_hit else 0, _ptr(hits), _ptr(inst)))
+    return hits, inst
 
 
 def energy_table(gpu: Gpu) -> np.ndarray:

@@ -18,6 +18,9 @@ struct Blob {
     std::vector<GpuBvhNode> bvh_nodes;
     std::vector<GpuLight> lights;
     uint32_t tlas_root = 0;
+    // Instanced scenes: meshes and placements, built into BLAS/TLAS at upload.
+    std::vector<lucida_mesh_range> meshes;
+    std::vector<lucida_instance_placement> placements;
 
     uint32_t env_width = 0, env_height = 0;
     std::vector<float> env_pixels, env_conditional, env_marginal;

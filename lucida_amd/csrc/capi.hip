@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "context.hpp"
 
@@ -31,6 +32,27 @@ void clear_error() { g_last_error.clear(); }
 lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
                                 GpuBvhNode* nodes, size_t* num_nodes, uint32_t* prim_indices,
                                 GpuTriangle* tris_sorted, float* stage_us);
+
+// Walks the tree at `root` (at most n nodes, so a cycle ends the walk) and
+// checks that children stay inside the array and leaves inside [lo, hi).
+static bool tree_is_sound(const GpuBvhNode* nodes, size_t n, uint32_t root, size_t lo, size_t hi) {
+    if (root >= n) return false;
+    std::vector<uint32_t> stack{root};
+    size_t visited = 0;
+    while (!stack.empty()) {
+        if (++visited > n) return false;
+        const GpuBvhNode& nd = nodes[stack.back()];
+        stack.pop_back();
+        if (nd.count == 0) {
+            if ((size_t)nd.left_first + 1 >= n) return false;
+            stack.push_back(nd.left_first);
+            stack.push_back(nd.left_first + 1);
+        } else if (nd.left_first < lo || (size_t)nd.left_first + nd.count > hi) {
+            return false;
+        }
+    }
+    return true;
+}
 
 template <class T>
 static lucida_status upload(lucida_ctx* ctx, T** dst, const T* src, size_t count, size_t min_count = 1) {
@@ -174,21 +196,50 @@ lucida_status lucida_scene_create(lucida_ctx* ctx, const lucida_scene_desc* d, l
             return set_error(LUCIDA_INVALID_ARG, "primitive material out of range");
     for (size_t i = 0; i < d->num_lights; ++i)
         if (d->lights[i].material >= d->num_materials) return set_error(LUCIDA_INVALID_ARG, "light material out of range");
-    for (size_t i = 0; i < d->num_bvh_nodes; ++i) {
-        const GpuBvhNode& nd = d->bvh_nodes[i];
-        if (nd.count == 0 ? (size_t)nd.left_first + 1 >= d->num_bvh_nodes
-                          : (size_t)nd.left_first + nd.count > (d->num_instances ? d->num_primitives + d->num_instances
-                                                                                 : d->num_triangles))
-            return set_error(LUCIDA_INVALID_ARG, "BVH node references out of range");
+    if (d->num_instances == 0) {
+        for (size_t i = 0; i < d->num_bvh_nodes; ++i) {
+            const GpuBvhNode& nd = d->bvh_nodes[i];
+            if (nd.count == 0 ? (size_t)nd.left_first + 1 >= d->num_bvh_nodes
+                              : (size_t)nd.left_first + nd.count > d->num_triangles)
+                return set_error(LUCIDA_INVALID_ARG, "BVH node references out of range");
+        }
+    } else {
+        // Two-level layout (Scene::build_instances, scene.rs:252): the TLAS at
+        // tlas_root indexes the instances that follow the primitives; each
+        // instance's BLAS indexes the triangles.
+        if (!d->instances || d->num_bvh_nodes == 0)
+            return set_error(LUCIDA_INVALID_ARG, "instanced scene without instances or BVH nodes");
+        if (!tree_is_sound(d->bvh_nodes, d->num_bvh_nodes, d->tlas_root, d->num_primitives,
+                           d->num_primitives + d->num_instances))
+            return set_error(LUCIDA_INVALID_ARG, "TLAS references out of range");
+        std::vector<uint32_t> roots;
+        for (size_t i = 0; i < d->num_instances; ++i) {
+            const GpuInstance& in = d->instances[i];
+            if (in.kind != LUCIDA_PRIM_KIND_INSTANCE || in.blas_root >= d->num_bvh_nodes ||
+                (in.material != 0xFFFFFFFFu && in.material >= d->num_materials))
+                return set_error(LUCIDA_INVALID_ARG, "instance record out of range");
+            roots.push_back(in.blas_root);
+        }
+        std::sort(roots.begin(), roots.end());
+        roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
+        for (uint32_t r : roots)
+            if (!tree_is_sound(d->bvh_nodes, d->num_bvh_nodes, r, 0, d->num_triangles))
+                return set_error(LUCIDA_INVALID_ARG, "BLAS references out of range");
     }
-    if (d->num_instances) return set_error(LUCIDA_INVALID_ARG, "instanced scenes are not supported by this build yet");
     auto* s = new lucida_scene();
     s->ctx = ctx;
     lucida_status st = LUCIDA_OK;
 #define UP(dst, src, cnt)                                  \
     if (st == LUCIDA_OK) st = upload(ctx, &s->dst, src, cnt);
     UP(materials, d->materials, d->num_materials);
+    // Instances follow the analytic primitives in one array, as uploaded by
+    // SceneBlob::primitives_upload (gpu_layout.rs:743).
     std::vector<GpuPrimitive> prims(d->primitives, d->primitives + d->num_primitives);
+    for (size_t i = 0; i < d->num_instances; ++i) {
+        GpuPrimitive pr;
+        std::memcpy(&pr, &d->instances[i], sizeof pr);
+        prims.push_back(pr);
+    }
     UP(primitives, prims.data(), prims.size());
     UP(positions, d->positions, d->num_positions * 4);
     UP(vertex_attrs, d->vertex_attrs, d->num_positions);

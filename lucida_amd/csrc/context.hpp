@@ -96,6 +96,11 @@ struct lucida_film {
 namespace lucida {
 // Ensures the context's GGX energy table is resident (computed on the device).
 lucida_status ensure_ggx_table(lucida_ctx* ctx);
+// Device LBVH build (lbvh.hip): nodes in GpuBvhNode layout, optional prim
+// order and triangles permuted into leaf order.
+lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
+                                GpuBvhNode* nodes, size_t* num_nodes, uint32_t* prim_indices,
+                                GpuTriangle* tris_sorted, float* stage_us);
 // Fills scene->tri_edges from its (final, leaf-ordered) triangles.
 lucida_status build_tri_edges(lucida_ctx* ctx, lucida_scene* scene);
 // A-trous filter (atrous.hip) without the final wait: reserve the planes for

@@ -8,6 +8,8 @@
 // (envmap.rs:112-180), cornell_box / cornell_mesh (scenes.rs:56-214). The two
 // benchmark scenes ("box-c1", "sphere-c2") are new generators written in the
 // same idioms.
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -475,34 +477,12 @@ static void room_face(Blob& b, f3 origin, f3 eu, f3 ev, uint32_t nu, uint32_t nv
         }
 }
 
-// Room [-2.5, 2.5] x [-1.5, 2.5] x [-2.5, 2.5], closed, inward-facing:
-// floor and ceiling 12 x 12 quads, four walls 8 x 7 quads (512 quads, 1,024
-// triangles). Unit sphere at the origin, geodesic frequency f (229 gives
-// 1,048,820 triangles), displaced radially by amplitude * noise. Materials:
-// GGX over a diffuse base, roughness U[0.05, 1], albedo U[0.1, 0.9]^3, one per
-// room face and one per sphere group (80), drawn from a PCG stream seeded by
-// `seed`. Sixteen 0.25 x 0.25 analytic quads under the ceiling emit 20.
-static void sphere_c2(Blob& b, uint32_t f, float amplitude, uint32_t seed) {
-    // materials: 0..5 room faces, 6..85 sphere groups, 86 emitter
-    uint32_t st = pcg_hash(seed * 0x9e3779b9u + 3u);
-    for (int m = 0; m < 86; ++m) {
-        float r = 0.05f + 0.95f * rng_f32(st);
-        float cr = 0.1f + 0.8f * rng_f32(st);
-        float cg = 0.1f + 0.8f * rng_f32(st);
-        float cb = 0.1f + 0.8f * rng_f32(st);
-        b.materials.push_back(mat_glossy(f3(cr, cg, cb), r));
-    }
-    b.materials.push_back(mat_emissive(f3(20.0f, 18.0f, 15.0f)));
-    const uint32_t LIGHT = 86;
-
-    const float X0 = -2.5f, X1 = 2.5f, Y0 = -1.5f, Y1 = 2.5f, Z0 = -2.5f, Z1 = 2.5f;
-    room_face(b, f3(X0, Y0, Z0), f3(0, 0, Z1 - Z0), f3(X1 - X0, 0, 0), 12, 12, 0);  // floor, +y
-    room_face(b, f3(X0, Y1, Z0), f3(X1 - X0, 0, 0), f3(0, 0, Z1 - Z0), 12, 12, 1);  // ceiling, -y
-    room_face(b, f3(X0, Y0, Z1), f3(0, Y1 - Y0, 0), f3(X1 - X0, 0, 0), 8, 7, 2);    // z = Z1, -z
-    room_face(b, f3(X0, Y0, Z0), f3(X1 - X0, 0, 0), f3(0, Y1 - Y0, 0), 8, 7, 3);    // z = Z0, +z
-    room_face(b, f3(X1, Y0, Z0), f3(0, 0, Z1 - Z0), f3(0, Y1 - Y0, 0), 8, 7, 4);    // x = X1, -x
-    room_face(b, f3(X0, Y0, Z0), f3(0, Y1 - Y0, 0), f3(0, 0, Z1 - Z0), 8, 7, 5);    // x = X0, +x
-
+// Unit geodesic sphere of frequency f displaced radially by amplitude *
+// noise3 (seeded by `seed`), with area-weighted smooth normals. Triangle
+// materials: `material` + the triangle's group (80 groups) when `by_group`,
+// else `material`.
+static void append_displaced_sphere(Blob& b, uint32_t f, float amplitude, uint32_t seed, uint32_t material,
+                                    bool by_group) {
     std::vector<f3> unit;
     std::vector<uint32_t> tris, groups;
     geodesic(f, unit, tris, groups);
@@ -539,7 +519,39 @@ static void sphere_c2(Blob& b, uint32_t f, float amplitude, uint32_t seed) {
         b.vertex_attrs.push_back(a);
     }
     for (size_t t = 0, g = 0; t < tris.size(); t += 3, ++g)
-        b.triangles.push_back(GpuTriangle{base + tris[t], base + tris[t + 1], base + tris[t + 2], 6u + groups[g]});
+        b.triangles.push_back(GpuTriangle{base + tris[t], base + tris[t + 1], base + tris[t + 2], by_group ? material + groups[g] : material});
+
+}
+
+// Room [-2.5, 2.5] x [-1.5, 2.5] x [-2.5, 2.5], closed, inward-facing:
+// floor and ceiling 12 x 12 quads, four walls 8 x 7 quads (512 quads, 1,024
+// triangles). Unit sphere at the origin, geodesic frequency f (229 gives
+// 1,048,820 triangles), displaced radially by amplitude * noise. Materials:
+// GGX over a diffuse base, roughness U[0.05, 1], albedo U[0.1, 0.9]^3, one per
+// room face and one per sphere group (80), drawn from a PCG stream seeded by
+// `seed`. Sixteen 0.25 x 0.25 analytic quads under the ceiling emit 20.
+static void sphere_c2(Blob& b, uint32_t f, float amplitude, uint32_t seed) {
+    // materials: 0..5 room faces, 6..85 sphere groups, 86 emitter
+    uint32_t st = pcg_hash(seed * 0x9e3779b9u + 3u);
+    for (int m = 0; m < 86; ++m) {
+        float r = 0.05f + 0.95f * rng_f32(st);
+        float cr = 0.1f + 0.8f * rng_f32(st);
+        float cg = 0.1f + 0.8f * rng_f32(st);
+        float cb = 0.1f + 0.8f * rng_f32(st);
+        b.materials.push_back(mat_glossy(f3(cr, cg, cb), r));
+    }
+    b.materials.push_back(mat_emissive(f3(20.0f, 18.0f, 15.0f)));
+    const uint32_t LIGHT = 86;
+
+    const float X0 = -2.5f, X1 = 2.5f, Y0 = -1.5f, Y1 = 2.5f, Z0 = -2.5f, Z1 = 2.5f;
+    room_face(b, f3(X0, Y0, Z0), f3(0, 0, Z1 - Z0), f3(X1 - X0, 0, 0), 12, 12, 0);  // floor, +y
+    room_face(b, f3(X0, Y1, Z0), f3(X1 - X0, 0, 0), f3(0, 0, Z1 - Z0), 12, 12, 1);  // ceiling, -y
+    room_face(b, f3(X0, Y0, Z1), f3(0, Y1 - Y0, 0), f3(X1 - X0, 0, 0), 8, 7, 2);    // z = Z1, -z
+    room_face(b, f3(X0, Y0, Z0), f3(X1 - X0, 0, 0), f3(0, Y1 - Y0, 0), 8, 7, 3);    // z = Z0, +z
+    room_face(b, f3(X1, Y0, Z0), f3(0, 0, Z1 - Z0), f3(0, Y1 - Y0, 0), 8, 7, 4);    // x = X1, -x
+    room_face(b, f3(X0, Y0, Z0), f3(0, Y1 - Y0, 0), f3(0, 0, Z1 - Z0), 8, 7, 5);    // x = X0, +x
+
+    append_displaced_sphere(b, f, amplitude, seed, 6u, true);
 
     for (int gx = 0; gx < 4; ++gx)
         for (int gz = 0; gz < 4; ++gz) {
@@ -548,6 +560,83 @@ static void sphere_c2(Blob& b, uint32_t f, float amplitude, uint32_t seed) {
                 prim_quad(f3(cx - 0.125f, Y1 - 1.0e-3f, cz - 0.125f), f3(0.25f, 0, 0), f3(0, 0, 0.25f), LIGHT));
         }
     camera_look_at(b, f3(0.0f, 0.45f, 2.3f), f3(0.0f, 0.0f, 0.0f), 60.0f);
+    b.background = f3(0, 0, 0);
+}
+
+// Instanced field (the config-3 scene at count 4096, f 22, extent 100, env
+// 2048 x 1024): one noise-displaced geodesic sphere mesh (20 f^2 triangles,
+// amplitude 0.05, noise seed 4) placed `count` times with a uniformly random
+// rotation (unit quaternion from three uniforms, Shoemake 1992), uniform scale
+// U[0.5, 2] and position U[0, extent]^3. Each placement overrides the
+// material: 30% rough dielectric (IOR 1.5, roughness U[0, 0.3]), the rest GGX
+// (roughness U[0.05, 1], albedo U[0.1, 0.9]^3). Lit only by an environment
+// map: log-normal radiance (mu 0, sigma 1.5) per texel plus three Gaussian
+// hot spots of peak 1e4 (seed 5).
+static void instance_field(Blob& b, uint32_t f, uint32_t count, float extent, uint32_t env_w, uint32_t env_h,
+                           uint32_t seed) {
+    b.materials.push_back(mat_glossy(f3(0.5f, 0.5f, 0.5f), 0.5f));  // the mesh's own (always overridden)
+    const uint32_t first = (uint32_t)b.triangles.size();
+    append_displaced_sphere(b, f, 0.05f, 4u, 0u, false);
+    b.meshes.push_back(lucida_mesh_range{first, (uint32_t)b.triangles.size() - first});
+    uint32_t st = pcg_hash(seed * 0x9e3779b9u + 7u);
+    const float TWO_PI = 6.2831853071795864f;
+    for (uint32_t i = 0; i < count; ++i) {
+        const float u1 = rng_f32(st), u2 = rng_f32(st), u3 = rng_f32(st);
+        const float a = std::sqrt(1.0f - u1), c = std::sqrt(u1);
+        const float qx = a * std::sin(TWO_PI * u2), qy = a * std::cos(TWO_PI * u2);
+        const float qz = c * std::sin(TWO_PI * u3), qw = c * std::cos(TWO_PI * u3);
+        const float scale = 0.5f + 1.5f * rng_f32(st);
+        const f3 pos(extent * rng_f32(st), extent * rng_f32(st), extent * rng_f32(st));
+        const float R[3][3] = {{1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)},
+                               {2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)},
+                               {2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)}};
+        lucida_instance_placement pl{};
+        for (int r = 0; r < 3; ++r) {
+            for (int k = 0; k < 3; ++k) pl.object_to_world[4 * r + k] = scale * R[r][k];
+        }
+        pl.object_to_world[3] = pos.x;
+        pl.object_to_world[7] = pos.y;
+        pl.object_to_world[11] = pos.z;
+        pl.mesh = 0;
+        pl.material = (uint32_t)b.materials.size();
+        if (rng_f32(st) < 0.3f) {
+            GpuMaterial m{};
+            f3(1.0f, 1.0f, 1.0f).store(m.base_color);
+            m.roughness = 0.3f * rng_f32(st);
+            m.ior = 1.5f;
+            m.transmission = 1.0f;
+            b.materials.push_back(m);
+        } else {
+            const float r = 0.05f + 0.95f * rng_f32(st);
+            const float cr = 0.1f + 0.8f * rng_f32(st), cg = 0.1f + 0.8f * rng_f32(st), cb = 0.1f + 0.8f * rng_f32(st);
+            b.materials.push_back(mat_glossy(f3(cr, cg, cb), r));
+        }
+        b.placements.push_back(pl);
+    }
+    // environment
+    std::vector<float> rgba((size_t)env_w * env_h * 4);
+    uint32_t es = pcg_hash(5u * 0x9e3779b9u + 11u);
+    for (size_t i = 0; i < (size_t)env_w * env_h; ++i) {
+        const float z = std::sqrt(-2.0f * std::log(std::max(rng_f32(es), 1e-12f))) * std::cos(TWO_PI * rng_f32(es));
+        const float L = std::exp(1.5f * z);
+        rgba[4 * i] = rgba[4 * i + 1] = rgba[4 * i + 2] = L;
+        rgba[4 * i + 3] = 1.0f;
+    }
+    for (int k = 0; k < 3; ++k) {
+        const float cu = rng_f32(es) * env_w, cv = (0.15f + 0.5f * rng_f32(es)) * env_h;
+        const float sigma = 0.004f * env_w;
+        const int rad = (int)std::ceil(4.0f * sigma);
+        for (int dv = -rad; dv <= rad; ++dv)
+            for (int du = -rad; du <= rad; ++du) {
+                const int x = ((int)cu + du + (int)env_w) % (int)env_w, y = (int)cv + dv;
+                if (y < 0 || y >= (int)env_h) continue;
+                const float g = 1.0e4f * std::exp(-(float)(du * du + dv * dv) / (2.0f * sigma * sigma));
+                for (int ch = 0; ch < 3; ++ch) rgba[4 * ((size_t)y * env_w + x) + ch] += g;
+            }
+    }
+    env_build(b, env_w, env_h, rgba.data());
+    camera_look_at(b, f3(0.5f * extent, 0.62f * extent, -0.85f * extent), f3(0.5f * extent, 0.45f * extent, 0.5f * extent),
+                   50.0f);
     b.background = f3(0, 0, 0);
 }
 
@@ -573,6 +662,10 @@ lucida_status lucida_blob_create(const char* name, const lucida_blob_params* par
         uint32_t f = p.frequency ? p.frequency : 229u;
         float amp = p.displacement != 0.0f ? p.displacement : 0.05f;
         sphere_c2(*b, f, amp, p.seed);
+    } else if (n == "instances-c3") {
+        instance_field(*b, p.frequency ? p.frequency : 22u, 4096u, 100.0f, 2048u, 1024u, p.seed);
+    } else if (n == "instances-small") {
+        instance_field(*b, p.frequency ? p.frequency : 4u, 64u, 10.0f, 256u, 128u, p.seed);
     } else if (n.rfind("icosphere-", 0) == 0) {
         int lv = std::atoi(n.c_str() + 10);
         if (lv < 0 || lv > 9) {
@@ -654,6 +747,18 @@ lucida_status lucida_blob_set_bvh(lucida_blob* blob, const GpuBvhNode* nodes, si
     Blob& b = *reinterpret_cast<Blob*>(blob);
     b.bvh_nodes.assign(nodes, nodes + num_nodes);
     if (tris) b.triangles.assign(tris, tris + b.triangles.size());
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_blob_instancing(const lucida_blob* blob, const lucida_mesh_range** meshes, size_t* num_meshes,
+                                     const lucida_instance_placement** placements, size_t* num_placements) {
+    if (!blob || !meshes || !num_meshes || !placements || !num_placements)
+        return set_error(LUCIDA_INVALID_ARG, "lucida_blob_instancing: null argument");
+    const Blob& b = *reinterpret_cast<const Blob*>(blob);
+    *meshes = b.meshes.data();
+    *num_meshes = b.meshes.size();
+    *placements = b.placements.data();
+    *num_placements = b.placements.size();
     return LUCIDA_OK;
 }
 

@@ -59,6 +59,7 @@ struct Planes {
     float4* gal;   // guide albedo.xyz, guide depth
     float4* gnr;   // guide normal.xyz, unused
     float4* hit;   // t, u, v, primitive id (uint bits)
+    uint32_t* hin; // instance of a triangle hit (instanced scenes only)
     float4* sro;   // shadow: origin.xyz, t_max
     float4* srd;   // shadow: direction.xyz, path (uint bits)
     float4* src;   // shadow: contribution.xyz, unused
@@ -75,10 +76,12 @@ struct SceneView {
     const GpuTriangle* __restrict__ tris;
     const float4* __restrict__ edges;  // per triangle: p0, p1 - p0, p2 - p0
     const GpuBvhNode* __restrict__ nodes;
+    const GpuInstance* __restrict__ insts;  // the primitive array read as instances (absolute indices)
     const GpuLight* __restrict__ lights;
     const float* __restrict__ ggx;
     EnvView env;
     uint32_t num_prims, num_tris, num_nodes, num_lights;
+    uint32_t num_instances, tlas_root;
 };
 
 __device__ inline uint64_t lanemask_lt() {
@@ -224,11 +227,13 @@ __device__ inline bool tri_test(const float4* __restrict__ edges, uint32_t ti, V
 __device__ inline uint32_t pack_child(uint32_t first, uint32_t count) { return (first << 3) | count; }
 
 // Closest hit (any_hit = false) or occlusion (any_hit = true) against the
-// scene's triangles. Returns hit flag; closest-hit fills t/u/v/tri.
+// triangles of the tree rooted at `root` (intersect_bvh_from, bvh.wgsl:137:
+// the root's own box is not tested). Returns the hit flag; closest-hit fills
+// t/u/v/tri. `stk` is this lane's stack column with `capacity` entries.
 template <bool ANY_HIT>
-__device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk, V3 o, V3 d, float t_min,
-                                float t_max, float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
-                                uint32_t* visits, uint32_t* tests, uint32_t* overflow) {
+__device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk, int capacity, uint32_t root_index,
+                                V3 o, V3 d, float t_min, float t_max, float* t_out, float* u_out, float* v_out,
+                                uint32_t* tri_out, uint32_t* visits, uint32_t* tests, uint32_t* overflow) {
     bool found = false;
     float closest = t_max;
     if (sc.num_nodes == 0) {
@@ -244,7 +249,7 @@ __device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk,
         return found;
     }
     const V3 inv = safe_inv_dir(d);
-    const GpuBvhNode root = sc.nodes[0];
+    const GpuBvhNode root = sc.nodes[root_index];
     uint32_t cur = pack_child(root.left_first, root.count);
     int sp = 0;
     const int lane_stride = TRACE_WG;
@@ -263,6 +268,101 @@ __device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk,
                     if (ANY_HIT) return true;
                     closest = t;
                     *t_out = t; *u_out = u; *v_out = v; *tri_out = ti;
+                    found = true;
+                }
+            }
+        } else {
+            const float4* cn = reinterpret_cast<const float4*>(sc.nodes + first);
+            const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
+            float ea, eb;
+            const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
+            const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
+            const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
+            const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
+            if (ha && hb) {
+                const bool a_near = ANY_HIT ? true : ea <= eb;
+                if (sp < capacity) {
+                    stk[sp * lane_stride] = a_near ? cb : ca;
+                    ++sp;
+                } else {
+                    *overflow += 1;
+                }
+                cur = a_near ? ca : cb;
+                continue;
+            } else if (ha) {
+                cur = ca;
+                continue;
+            } else if (hb) {
+                cur = cb;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = stk[sp * lane_stride];
+    }
+    return found;
+}
+
+// An instance's world-to-object transform (GpuInstance, gpu_layout.rs:496).
+struct InstanceXf {
+    V3 r0, r1, r2, t;
+    uint32_t blas_root, material;
+};
+
+__device__ inline InstanceXf load_instance(const GpuInstance* __restrict__ insts, uint32_t i) {
+    const float4* q = reinterpret_cast<const float4*>(insts + i);
+    const float4 a = q[0], b = q[1], c = q[2], e = q[3];
+    return InstanceXf{V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, V3{e.x, e.y, e.z},
+                      __float_as_uint(a.w), __float_as_uint(c.w)};
+}
+
+// Instance::transform_ray (instance.rs:120, bvh.wgsl instance_transform_*):
+// the direction is not normalised, so t means the same in both spaces.
+__device__ inline V3 xf_point(const InstanceXf& I, V3 p) {
+    return V3{dot(I.r0, p) + I.t.x, dot(I.r1, p) + I.t.y, dot(I.r2, p) + I.t.z};
+}
+__device__ inline V3 xf_vector(const InstanceXf& I, V3 v) { return V3{dot(I.r0, v), dot(I.r1, v), dot(I.r2, v)}; }
+// Instance::transform_normal (instance.rs:138): by the transpose of the stored inverse.
+__device__ inline V3 xf_normal(const InstanceXf& I, V3 n) {
+    return normalize_or(V3{I.r0.x * n.x + I.r1.x * n.y + I.r2.x * n.z, I.r0.y * n.x + I.r1.y * n.y + I.r2.y * n.z,
+                           I.r0.z * n.x + I.r1.z * n.y + I.r2.z * n.z},
+                        v3(0, 0, 1));
+}
+
+// Two-level traversal (intersect_instances, bvh.wgsl:305): the TLAS rooted at
+// sc.tlas_root in world space; at each instance leaf the ray goes to object
+// space and walks that instance's BLAS with `closest` carried across levels.
+// The BLAS walk uses the stack entries above the TLAS's own. With ANY_HIT it
+// is the two-level occlusion query (the reference's shadow rays walk only the
+// tree at node 0, see connect.wgsl:192-212; here they see every instance).
+template <bool ANY_HIT>
+__device__ inline bool traverse_instances(const SceneView& sc, uint32_t* __restrict__ stk, V3 o, V3 d, float t_min,
+                                          float t_max, float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
+                                          uint32_t* inst_out, uint32_t* visits, uint32_t* tests,
+                                          uint32_t* overflow) {
+    bool found = false;
+    float closest = t_max;
+    const V3 inv = safe_inv_dir(d);
+    const GpuBvhNode root = sc.nodes[sc.tlas_root];
+    uint32_t cur = pack_child(root.left_first, root.count);
+    int sp = 0;
+    const int lane_stride = TRACE_WG;
+    const uint32_t cap = 2u * sc.num_nodes + 8u;
+    for (uint32_t iter = 0; iter < cap; ++iter) {
+        *visits += 1;
+        const uint32_t first = cur >> 3, count = cur & 7u;
+        if (count > 0) {
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t ii = first + k;
+                const InstanceXf I = load_instance(sc.insts, ii);
+                float t, u, v;
+                uint32_t tri;
+                if (traverse<ANY_HIT>(sc, stk + sp * lane_stride, STACK - sp, I.blas_root, xf_point(I, o),
+                                      xf_vector(I, d), t_min, closest, &t, &u, &v, &tri, visits, tests, overflow)) {
+                    if (ANY_HIT) return true;
+                    closest = t;
+                    *t_out = t; *u_out = u; *v_out = v; *tri_out = tri; *inst_out = ii;
                     found = true;
                 }
             }
@@ -327,9 +427,15 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc, Planes p, con
             }
         }
         float t = 0.0f, u = 0.0f, v = 0.0f;
-        uint32_t tri = 0;
-        bool th = traverse<false>(sc, stack_mem + threadIdx.x, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
-                                  &overflow);
+        uint32_t tri = 0, inst = 0;
+        // Instanced scenes take the two-level path instead of the
+        // single-level one (scene.wgsl:31-75): their triangles live in object space.
+        const bool th = sc.num_instances
+                            ? traverse_instances<false>(sc, stack_mem + threadIdx.x, o, d, T_MIN, best_t, &t, &u, &v,
+                                                        &tri, &inst, &visits, &tests, &overflow)
+                            : traverse<false>(sc, stack_mem + threadIdx.x, STACK, 0, o, d, T_MIN, best_t, &t, &u, &v,
+                                              &tri, &visits, &tests, &overflow);
+        if (sc.num_instances) p.hin[i] = inst;
         float4 out;
         if (th) out = make_float4(t, u, v, __uint_as_float(tri));
         else if (best != MISS) out = make_float4(best_t, 0.0f, 0.0f, __uint_as_float(ANALYTIC | best));
@@ -411,12 +517,28 @@ __global__ __launch_bounds__(WG) void k_shade(SceneView sc, const GpuUniforms* _
                                      v3(0, 0, 1));
                 V3 cr = cross(p1 - p0, p2 - p0);
                 V3 g = normalize_or(cr, v3(0, 0, 1));
-                front = dot(g, dir) < 0.0f;
-                const float flip = front ? 1.0f : -1.0f;
-                normal = sh * flip;
-                gnormal = g * flip;
                 material = tri.material;
-                light_area = 0.5f * length(cr);
+                if (sc.num_instances) {
+                    // shade_instanced_triangle (scene.rs:375): orient in object
+                    // space, bring both normals to world space, re-decide the
+                    // face there, apply the material override; instanced
+                    // triangles are not lights.
+                    const InstanceXf I = load_instance(sc.insts, p.hin[i]);
+                    const float oflip = dot(g, xf_vector(I, dir)) < 0.0f ? 1.0f : -1.0f;
+                    const V3 n_w = xf_normal(I, sh * oflip), g_w = xf_normal(I, g * oflip);
+                    front = dot(g_w, dir) < 0.0f;
+                    const float flip = front ? 1.0f : -1.0f;
+                    normal = n_w * flip;
+                    gnormal = g_w * flip;
+                    if (I.material != 0xFFFFFFFFu) material = I.material;
+                    light_area = 0.0f;
+                } else {
+                    front = dot(g, dir) < 0.0f;
+                    const float flip = front ? 1.0f : -1.0f;
+                    normal = sh * flip;
+                    gnormal = g * flip;
+                    light_area = 0.5f * length(cr);
+                }
             }
             const GpuMaterial m = sc.materials[material];
             const V3 emissive = ld3(m.emissive);
@@ -577,9 +699,12 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc, Planes p, co
         }
         if (!blocked) {
             float t, u, v;
-            uint32_t tri;
-            blocked = traverse<true>(sc, stack_mem + threadIdx.x, o, d, T_MIN, t_max, &t, &u, &v, &tri, &visits,
-                                     &tests, &overflow);
+            uint32_t tri, inst;
+            blocked = sc.num_instances
+                          ? traverse_instances<true>(sc, stack_mem + threadIdx.x, o, d, T_MIN, t_max, &t, &u, &v, &tri,
+                                                     &inst, &visits, &tests, &overflow)
+                          : traverse<true>(sc, stack_mem + threadIdx.x, STACK, 0, o, d, T_MIN, t_max, &t, &u, &v,
+                                           &tri, &visits, &tests, &overflow);
         }
         if (!blocked) {
             const uint32_t pid = __float_as_uint(b.w);
@@ -592,6 +717,55 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc, Planes p, co
         }
     }
     flush_traversal_stats(p.counters, visits, tests, overflow, stats_on, 2);
+}
+
+// ---------------------------------------------------------------------------
+// Ray casting without shading (the scene_intersect / scene_occluded queries
+// EXTEND and CONNECT make): rays are (origin, unused, direction, t_max);
+// hits are (t, u, v, id) with id = triangle, ANALYTIC | primitive or MISS
+// (any-hit: id 0 when occluded), plus the instance of an instanced hit.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* __restrict__ rays, uint32_t n,
+                                                   uint32_t any_hit, float4* __restrict__ hits,
+                                                   uint32_t* __restrict__ insts) {
+    __shared__ uint32_t stack_mem[STACK * TRACE_WG];
+    const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = rays[2 * i], b = rays[2 * i + 1];
+    const V3 o = f4xyz(a), d = f4xyz(b);
+    float best_t = b.w;
+    uint32_t best = MISS;
+    for (uint32_t k = 0; k < sc.num_prims; ++k) {
+        const GpuPrimitive& pr = sc.prims[k];
+        float t;
+        const bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, o, d, T_MIN, best_t, &t)
+                                                        : intersect_sphere(pr, o, d, T_MIN, best_t, &t);
+        if (h) {
+            best_t = t;
+            best = k;
+            if (any_hit) break;
+        }
+    }
+    uint32_t visits = 0, tests = 0, overflow = 0, tri = 0, inst = 0;
+    float t = 0.0f, u = 0.0f, v = 0.0f;
+    float4 out = best != MISS ? make_float4(best_t, 0.0f, 0.0f, __uint_as_float(ANALYTIC | best))
+                              : make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(MISS));
+    if (!(any_hit && best != MISS)) {
+        bool got;
+        if (any_hit)
+            got = sc.num_instances ? traverse_instances<true>(sc, stack_mem + threadIdx.x, o, d, T_MIN, best_t, &t, &u,
+                                                              &v, &tri, &inst, &visits, &tests, &overflow)
+                                   : traverse<true>(sc, stack_mem + threadIdx.x, STACK, 0, o, d, T_MIN, best_t, &t,
+                                                    &u, &v, &tri, &visits, &tests, &overflow);
+        else
+            got = sc.num_instances ? traverse_instances<false>(sc, stack_mem + threadIdx.x, o, d, T_MIN, best_t, &t,
+                                                               &u, &v, &tri, &inst, &visits, &tests, &overflow)
+                                   : traverse<false>(sc, stack_mem + threadIdx.x, STACK, 0, o, d, T_MIN, best_t, &t,
+                                                     &u, &v, &tri, &visits, &tests, &overflow);
+        if (got) out = any_hit ? make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u)) : make_float4(t, u, v, __uint_as_float(tri));
+    }
+    hits[i] = out;
+    insts[i] = inst;
 }
 
 // ---------------------------------------------------------------------------
@@ -719,7 +893,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     if (P > ctx->wf_capacity) {
         LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
         LUCIDA_HIP(ctx->wf_planes.ensure(P * 16 * 7), LUCIDA_OOM);
-        LUCIDA_HIP(ctx->wf_hits.ensure(P * 16), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_hits.ensure(P * 20), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * 3), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_queues.ensure(P * 4 * 2), LUCIDA_OOM);
         ctx->wf_capacity = P;
@@ -736,6 +910,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     out->gal = pl + 5 * C;
     out->gnr = pl + 6 * C;
     out->hit = ctx->wf_hits.as<float4>();
+    out->hin = reinterpret_cast<uint32_t*>(ctx->wf_hits.as<float4>() + C);
     float4* sh = ctx->wf_shadow.as<float4>();
     out->sro = sh;
     out->srd = sh + C;
@@ -750,6 +925,34 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
 }  // namespace lucida
 
 using namespace lucida;
+
+// The kernels' view of a resident scene.
+static SceneView make_view(lucida_ctx* ctx, const lucida_scene* scene) {
+    SceneView sc;
+    sc.materials = scene->materials;
+    sc.prims = scene->primitives;
+    sc.pos = reinterpret_cast<const float4*>(scene->positions);
+    sc.attrs = scene->vertex_attrs;
+    sc.tris = scene->triangles;
+    sc.edges = scene->tri_edges;
+    sc.nodes = scene->bvh_nodes;
+    sc.insts = reinterpret_cast<const GpuInstance*>(scene->primitives);
+    sc.num_instances = (uint32_t)scene->num_instances;
+    sc.tlas_root = scene->tlas_root;
+    sc.lights = scene->lights;
+    sc.ggx = ctx->ggx_table.as<float>();
+    sc.env.w = scene->env_width;
+    sc.env.h = scene->env_height;
+    sc.env.total_weight = scene->env_total_weight;
+    sc.env.pixels = reinterpret_cast<const float4*>(scene->env_pixels);
+    sc.env.conditional = scene->env_conditional;
+    sc.env.marginal = scene->env_marginal;
+    sc.num_prims = (uint32_t)scene->num_primitives;
+    sc.num_tris = (uint32_t)scene->num_triangles;
+    sc.num_nodes = (uint32_t)scene->num_bvh_nodes;
+    sc.num_lights = (uint32_t)scene->num_lights;
+    return sc;
+}
 
 extern "C" {
 
@@ -771,7 +974,6 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     if (uniforms->width != film->width || uniforms->height != film->height)
         return set_error(LUCIDA_INVALID_ARG, "uniforms width/height do not match the film");
     if (uniforms->max_depth > MAX_BOUNCES) return set_error(LUCIDA_INVALID_ARG, "max_depth above 64");
-    if (scene->num_instances) return set_error(LUCIDA_INVALID_ARG, "instanced scenes are not supported yet");
     if (scene->num_bvh_nodes >= (1u << 28) || scene->num_triangles >= (1u << 28))
         return set_error(LUCIDA_INVALID_ARG, "scene too large: node and triangle indices are packed in 29 bits");
     if (stats) std::memset(stats, 0, sizeof *stats);
@@ -797,26 +999,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     st = ensure_wavefront(ctx, P, &p, &dU);
     if (st != LUCIDA_OK) return st;
 
-    SceneView sc;
-    sc.materials = scene->materials;
-    sc.prims = scene->primitives;
-    sc.pos = reinterpret_cast<const float4*>(scene->positions);
-    sc.attrs = scene->vertex_attrs;
-    sc.tris = scene->triangles;
-    sc.edges = scene->tri_edges;
-    sc.nodes = scene->bvh_nodes;
-    sc.lights = scene->lights;
-    sc.ggx = ctx->ggx_table.as<float>();
-    sc.env.w = scene->env_width;
-    sc.env.h = scene->env_height;
-    sc.env.total_weight = scene->env_total_weight;
-    sc.env.pixels = reinterpret_cast<const float4*>(scene->env_pixels);
-    sc.env.conditional = scene->env_conditional;
-    sc.env.marginal = scene->env_marginal;
-    sc.num_prims = (uint32_t)scene->num_primitives;
-    sc.num_tris = (uint32_t)scene->num_triangles;
-    sc.num_nodes = (uint32_t)scene->num_bvh_nodes;
-    sc.num_lights = (uint32_t)scene->num_lights;
+    const SceneView sc = make_view(ctx, scene);
 
     hipStream_t s = ctx->stream;
     const uint32_t depth = uniforms->max_depth;
@@ -902,5 +1085,37 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     return LUCIDA_OK;
 }
 
+
+lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, const float* rays, size_t n, int any_hit,
+                                float* hits, uint32_t* instances) {
+    if (!ctx || !scene || (n && (!rays || !hits))) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_cast: bad argument");
+    if (n == 0) return LUCIDA_OK;
+    if (n > (1u << 26)) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_cast: more than 2^26 rays");
+    clear_error();
+    const SceneView sc = make_view(ctx, scene);
+    DevBuf d_rays, d_hits, d_inst;
+    LUCIDA_HIP(d_rays.ensure(n * 32), LUCIDA_OOM);
+    LUCIDA_HIP(d_hits.ensure(n * 16), LUCIDA_OOM);
+    LUCIDA_HIP(d_inst.ensure(n * 4), LUCIDA_OOM);
+    lucida_status st = LUCIDA_OK;
+    if (hipMemcpyAsync(d_rays.ptr, rays, n * 32, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        st = set_error(LUCIDA_VALIDATION, "lucida_scene_cast: upload failed");
+    if (st == LUCIDA_OK) {
+        hipLaunchKernelGGL(k_cast, dim3(grid_for(n, TRACE_WG)), dim3(TRACE_WG), 0, ctx->stream, sc,
+                           d_rays.as<float4>(), (uint32_t)n, any_hit ? 1u : 0u, d_hits.as<float4>(),
+                           d_inst.as<uint32_t>());
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(hits, d_hits.ptr, n * 16, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            (instances &&
+             hipMemcpyAsync(instances, d_inst.ptr, n * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            st = set_error(LUCIDA_READBACK, "lucida_scene_cast: launch or readback failed");
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    d_rays.release();
+    d_hits.release();
+    d_inst.release();
+    return st;
+}
 
 }  // extern "C"

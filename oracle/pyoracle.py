@@ -57,6 +57,7 @@ def lib() -> C.CDLL:
             "oracle_render": ([P, P, P, U32, U32, U32, P, P, P], None),
             "oracle_denoise": ([P, P, P, P, U32, U32, P, P], None),
             "oracle_compare": ([P, P, S, P], None),
+            "oracle_scene_cast": ([P, P, S, I, P, P], None),
             "oracle_bsdf_sample": ([P, U32, P, F, F, F, P], F),
             "oracle_bsdf_eval_pdf": ([P, U32, P, P, P], F),
             "oracle_bsdf_albedo": ([P, F, U32, U32], C.c_double),
@@ -219,6 +220,18 @@ def denoise(film, albedo, normal, depth, params) -> np.ndarray:
     out = np.empty_like(f)
     lib().oracle_denoise(_p(f), _p(a), _p(n), _p(d), w, h, C.byref(params), _p(out))
     return out
+
+
+def scene_cast(desc, rays, any_hit: bool = False):
+    """Scene::intersect / occluded per ray without shading: rays (n, 8) float32
+    (origin, unused, direction, t_max) -> hits (n, 4) float32 (t, u, v, id
+    bits) and the instance of each triangle hit (instanced scenes)."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    n = len(rays)
+    hits = np.zeros((n, 4), np.float32)
+    inst = np.zeros(n, np.uint32)
+    lib().oracle_scene_cast(C.byref(desc), _p(rays), n, 1 if any_hit else 0, _p(hits), _p(inst))
+    return hits, inst
 
 
 def compare(a, b) -> dict:

@@ -669,6 +669,9 @@ struct OScene {
     const GpuMaterial* materials;
     const GpuPrimitive* prims;
     size_t num_prims;
+    const GpuInstance* instances = nullptr;  // instance k is primitive num_prims + k
+    size_t num_instances = 0;
+    uint32_t tlas_root = 0;
     BvhView bvh;
     const GpuVertexAttr* attrs;
     const GpuLight* lights;
@@ -714,6 +717,92 @@ static bool shade_analytic(const OScene& sc, long best, float best_t, const Ray&
     return true;
 }
 
+// --- instancing (instance.rs:120-160, :342-410; scene.rs:375-402) -------------------
+
+static Vec3 inst_point(const GpuInstance& I, Vec3 p) {
+    return Vec3(dot(Vec3::from(I.inv_row0), p) + I.inv_translation[0],
+                dot(Vec3::from(I.inv_row1), p) + I.inv_translation[1],
+                dot(Vec3::from(I.inv_row2), p) + I.inv_translation[2]);
+}
+static Vec3 inst_vector(const GpuInstance& I, Vec3 v) {
+    return Vec3(dot(Vec3::from(I.inv_row0), v), dot(Vec3::from(I.inv_row1), v), dot(Vec3::from(I.inv_row2), v));
+}
+static Vec3 inst_normal(const GpuInstance& I, Vec3 n) {
+    const float* r0 = I.inv_row0;
+    const float* r1 = I.inv_row1;
+    const float* r2 = I.inv_row2;
+    return normalize_or(Vec3(r0[0] * n.x + r1[0] * n.y + r2[0] * n.z, r0[1] * n.x + r1[1] * n.y + r2[1] * n.z,
+                             r0[2] * n.x + r1[2] * n.y + r2[2] * n.z),
+                        Vec3(0, 0, 1));
+}
+
+// InstanceSet::intersect (instance.rs:342): the TLAS walked in world space,
+// every node's own box tested, left child first; an instance leaf sends the
+// unnormalised object-space ray down its BLAS with `closest` carried over.
+// With any_hit set it answers occlusion through both levels.
+static bool instances_intersect(const OScene& sc, const Ray& ray, float t_min, float t_max, bool any_hit,
+                                TriHit* out, uint32_t* inst_out, TraversalStats* stats) {
+    const Vec3 inv = safe_inv_dir(ray.dir);
+    float closest = t_max;
+    bool found = false;
+    uint32_t stack[32];
+    size_t sp = 0;
+    uint32_t node = sc.tlas_root;
+    for (;;) {
+        const GpuBvhNode& n = sc.bvh.nodes[node];
+        if (stats) stats->node_visits += 1;
+        Aabb box;
+        box.min = Vec3::from(n.bounds_min);
+        box.max = Vec3::from(n.bounds_max);
+        float entry;
+        if (box.hit(ray.origin, inv, t_min, closest, &entry)) {
+            if (n.count > 0) {
+                for (uint32_t k = 0; k < n.count; ++k) {
+                    const uint32_t ii = n.left_first + k;
+                    const GpuInstance& I = sc.instances[ii - sc.num_prims];
+                    const Ray local{inst_point(I, ray.origin), inst_vector(I, ray.dir)};
+                    if (any_hit) {
+                        if (bvh_occluded(sc.bvh, I.blas_root, local, t_min, closest)) return true;
+                        continue;
+                    }
+                    TriHit h;
+                    if (bvh_intersect(sc.bvh, I.blas_root, local, t_min, closest, &h, stats)) {
+                        closest = h.t;
+                        *out = h;
+                        *inst_out = ii;
+                        found = true;
+                    }
+                }
+            } else {
+                if (sp + 1 < 32) stack[sp++] = n.left_first + 1;
+                node = n.left_first;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    return found;
+}
+
+static Hit shade_triangle(const OScene& sc, const TriHit& th, const Ray& ray);
+
+static Hit shade_instanced_triangle(const OScene& sc, const TriHit& th, uint32_t ii, const Ray& ray) {
+    const GpuInstance& I = sc.instances[ii - sc.num_prims];
+    Ray local{inst_point(I, ray.origin), inst_vector(I, ray.dir)};
+    Hit h = shade_triangle(sc, th, local);
+    h.normal = inst_normal(I, h.normal);
+    h.geometric_normal = inst_normal(I, h.geometric_normal);
+    h.position = ray.origin + th.t * ray.dir;
+    h.front_face = dot(h.geometric_normal, ray.dir) < 0.0f;
+    const float flip = h.front_face ? 1.0f : -1.0f;
+    h.normal = h.normal * flip;
+    h.geometric_normal = h.geometric_normal * flip;
+    if (I.material != 0xFFFFFFFFu) h.material = I.material;
+    h.light_area = 0.0f;
+    return h;
+}
+
 static bool scene_intersect(const OScene& sc, const Ray& ray, Hit* out, TraversalStats* stats) {
     float best_t = T_MAX;
     long best = -1;
@@ -728,6 +817,15 @@ static bool scene_intersect(const OScene& sc, const Ray& ray, Hit* out, Traversa
         }
     }
     TriHit th;
+    if (sc.num_instances) {
+        // instead of the single-level path (scene.rs:475-497)
+        uint32_t ii = 0;
+        if (instances_intersect(sc, ray, T_MIN, best_t, false, &th, &ii, stats)) {
+            *out = shade_instanced_triangle(sc, th, ii, ray);
+            return true;
+        }
+        return shade_analytic(sc, best, best_t, ray, out);
+    }
     bool tri_hit = sc.bvh.num_nodes == 0 ? brute_force_intersect(sc.bvh, ray, T_MIN, best_t, &th)
                                          : bvh_intersect(sc.bvh, 0, ray, T_MIN, best_t, &th, stats);
     if (tri_hit) {
@@ -744,6 +842,15 @@ static bool scene_occluded(const OScene& sc, const Ray& ray, float t_max) {
         bool hit = p.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(p, ray, T_MIN, t_max, &t)
                                                    : intersect_sphere(p, ray, T_MIN, t_max, &t);
         if (hit) return true;
+    }
+    if (sc.num_instances) {
+        // Two-level any-hit. The reference's CPU twin ignores instances here
+        // (scene.rs:577-606) and its GPU walks only the tree at node 0
+        // (connect.wgsl:192-212); both levels are the occluders a shadow ray
+        // meets in the instanced scene, and the device path does the same.
+        TriHit th;
+        uint32_t ii;
+        return instances_intersect(sc, ray, T_MIN, t_max, true, &th, &ii, nullptr);
     }
     if (sc.bvh.num_nodes == 0) {
         TriHit th;
@@ -977,12 +1084,14 @@ static void render_pixel(const OScene& sc, const GpuUniforms& u, uint32_t x, uin
 }
 
 // integrator.rs:635-699 — 32-row bands handed out through an atomic counter.
-void render(const lucida_scene_desc& d, const GpuUniforms& u, const float background[3], uint32_t threads,
-            uint32_t row_begin, uint32_t row_end, float* rgb, float* guides, RenderCounts* counts) {
+static OScene make_scene(const lucida_scene_desc& d, const float background[3]) {
     OScene sc;
     sc.materials = d.materials;
     sc.prims = d.primitives;
     sc.num_prims = d.num_primitives;
+    sc.instances = d.instances;
+    sc.num_instances = d.num_instances;
+    sc.tlas_root = d.tlas_root;
     sc.bvh.nodes = d.bvh_nodes;
     sc.bvh.num_nodes = d.num_bvh_nodes;
     sc.bvh.prim_indices = nullptr;
@@ -999,6 +1108,73 @@ void render(const lucida_scene_desc& d, const GpuUniforms& u, const float backgr
     sc.env.marginal = d.env_marginal;
     sc.env.total_weight = d.env_total_weight;
     sc.background = Vec3::from(background);
+    return sc;
+}
+
+// Closest hit (or occlusion) of one ray against the scene, without shading:
+// the analytic primitives first, then the triangles through the instance
+// hierarchy or the single-level BVH, as scene_intersect / scene_occluded.
+// id = triangle, 0x80000000 | primitive, or 0xFFFFFFFF.
+static void cast_one(const OScene& sc, const Ray& ray, float t_max, bool any_hit, float out[4], uint32_t* inst) {
+    float best_t = t_max;
+    long best = -1;
+    *inst = 0;
+    for (size_t i = 0; i < sc.num_prims; ++i) {
+        const GpuPrimitive& p = sc.prims[i];
+        float t;
+        bool hit = p.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(p, ray, T_MIN, best_t, &t)
+                                                   : intersect_sphere(p, ray, T_MIN, best_t, &t);
+        if (hit) {
+            best_t = t;
+            best = (long)i;
+            if (any_hit) break;
+        }
+    }
+    uint32_t id = best >= 0 ? (0x80000000u | (uint32_t)best) : 0xFFFFFFFFu;
+    float t = best >= 0 ? best_t : 0.0f, u = 0.0f, v = 0.0f;
+    if (!(any_hit && best >= 0)) {
+        TriHit th;
+        bool got;
+        if (sc.num_instances)
+            got = instances_intersect(sc, ray, T_MIN, best_t, any_hit, &th, inst, nullptr);
+        else if (any_hit)
+            got = sc.bvh.num_nodes ? bvh_occluded(sc.bvh, 0, ray, T_MIN, best_t)
+                                   : brute_force_intersect(sc.bvh, ray, T_MIN, best_t, &th);
+        else
+            got = sc.bvh.num_nodes ? bvh_intersect(sc.bvh, 0, ray, T_MIN, best_t, &th, nullptr)
+                                   : brute_force_intersect(sc.bvh, ray, T_MIN, best_t, &th);
+        if (got) {
+            if (any_hit) {
+                id = 0;
+                t = 0.0f;
+            } else {
+                id = th.triangle;
+                t = th.t;
+                u = th.u;
+                v = th.v;
+            }
+        }
+    }
+    out[0] = t;
+    out[1] = u;
+    out[2] = v;
+    std::memcpy(&out[3], &id, 4);
+}
+
+void scene_cast(const lucida_scene_desc& d, const float* rays, size_t n, int any_hit, float* hits, uint32_t* insts) {
+    const float bg[3] = {0, 0, 0};
+    const OScene sc = make_scene(d, bg);
+    for (size_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        uint32_t inst = 0;
+        cast_one(sc, Ray{Vec3(r[0], r[1], r[2]), Vec3(r[4], r[5], r[6])}, r[7], any_hit != 0, hits + 4 * i, &inst);
+        if (insts) insts[i] = inst;
+    }
+}
+
+void render(const lucida_scene_desc& d, const GpuUniforms& u, const float background[3], uint32_t threads,
+            uint32_t row_begin, uint32_t row_end, float* rgb, float* guides, RenderCounts* counts) {
+    const OScene sc = make_scene(d, background);
     ggx_table();
     if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
     const uint32_t BAND = 32;
@@ -1133,6 +1309,11 @@ void oracle_offset_ray_origin(const float p[3], const float n[3], float out[3]) 
 void oracle_render(const lucida_scene_desc* d, const GpuUniforms* u, const float background[3], uint32_t threads,
                    uint32_t row_begin, uint32_t row_end, float* rgb, float* guides, RenderCounts* counts) {
     render(*d, *u, background, threads, row_begin, row_end, rgb, guides, counts);
+}
+
+void oracle_scene_cast(const lucida_scene_desc* d, const float* rays, size_t n, int any_hit, float* hits,
+                       uint32_t* instances) {
+    scene_cast(*d, rays, n, any_hit, hits, instances);
 }
 
 void oracle_denoise(const float* film, const float* albedo, const float* normal, const float* depth, uint32_t width,

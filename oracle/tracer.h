@@ -35,6 +35,14 @@ void oracle_offset_ray_origin(const float p[3], const float n[3], float out[3]);
 void oracle_render(const lucida_scene_desc* d, const GpuUniforms* u, const float background[3], uint32_t threads,
                    uint32_t row_begin, uint32_t row_end, float* rgb, float* guides, RenderCounts* counts);
 
+/* One ray per 8 floats (origin.xyz, unused, direction.xyz, t_max) against the
+ * scene without shading: hits receive (t, u, v, id bits) per ray, id =
+ * triangle, 0x80000000 | analytic primitive, or 0xFFFFFFFF for a miss;
+ * instances (optional) the instance primitive index of a triangle hit in an
+ * instanced scene. With any_hit, id is 0 for an occluded ray. */
+void oracle_scene_cast(const lucida_scene_desc* d, const float* rays, size_t n, int any_hit, float* hits,
+                       uint32_t* instances);
+
 /* denoise::denoise (denoise.rs:134) on a film and its guides (3 floats per
  * pixel for film/albedo/normal/out, 1 for depth). */
 void oracle_denoise(const float* film, const float* albedo, const float* normal, const float* depth, uint32_t width,
