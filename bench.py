@@ -188,6 +188,67 @@ def time_to_relmse(L, gpu, scene, blob, width: int, height: int, depth: int, thr
     return out
 
 
+def batch_c4(L, gpu, jobs_n: int = 64, seed: int = 6) -> dict:
+    """The batched request stream: `jobs_n` renders drawn by a PCG64 stream
+    (seed 6) over resolution {128, 256, 512, 1024}^2 x spp {1, 4, 16, 64} x
+    scene {the Cornell box, four displaced-sphere rooms (seeds 1-4, 81,920 +
+    1,024 triangles)}, 8 bounces, all LBVHs resident. Run once job after job
+    (each lucida_render_wavefront waits for the previous) and once as one
+    lucida_render_batch with the autotuned launch shape; latency = time from
+    the start until a job's film is complete, all jobs arriving at once."""
+    from lucida_amd import autotune
+    rng = np.random.default_rng(seed)
+    blobs = [L.Blob("box-c1")] + [L.Blob("sphere-c2", seed=k, frequency=64) for k in range(1, 5)]
+    scenes = [L.Scene(gpu, b) for b in blobs]
+    reqs = []
+    for k in range(jobs_n):
+        res = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(rng.choice([1, 4, 16, 64]))
+        si = int(rng.integers(0, len(blobs)))
+        reqs.append((si, res, spp, 0x1000 + k))
+    paths = sum(r * r * spp for _, r, spp, _ in reqs)
+
+    def jobs():
+        out = []
+        for si, res, spp, fs in reqs:
+            u = blobs[si].uniforms(res, res, samples=spp, max_depth=8, frame_seed=fs)
+            out.append((scenes[si], u, spp, L.Film(gpu, res, res)))
+        return out
+
+    seq_jobs = jobs()
+    L.render_wavefront(gpu, seq_jobs[0][0], seq_jobs[0][1], 1, L.Film(gpu, seq_jobs[0][1].width,
+                                                                       seq_jobs[0][1].height))  # warm-up
+    gpu.check_errors()
+    t0 = time.perf_counter()
+    seq_done = []
+    for scene, u, spp, film in seq_jobs:
+        L.render_wavefront(gpu, scene, u, spp, film)
+        gpu.check_errors()
+        seq_done.append((time.perf_counter() - t0) * 1e3)
+    seq_ms = seq_done[-1]
+    del seq_jobs
+    bjobs = jobs()
+    t_tune = time.perf_counter()
+    cfg = autotune.tune(gpu, bjobs)
+    tune_s = time.perf_counter() - t_tune
+    for _, _, _, f in bjobs:
+        f.clear()
+    gpu.check_errors()
+    t0 = time.perf_counter()
+    st, done = L.render_batch(gpu, bjobs, cfg)
+    gpu.check_errors()
+    batch_ms = (time.perf_counter() - t0) * 1e3
+    q = lambda a, p: round(float(np.percentile(a, p)), 2)
+    return {"jobs": jobs_n, "paths": int(paths), "max_depth": 8,
+            "sequential": {"ms": round(seq_ms, 2), "mpaths_per_s": round(paths / seq_ms / 1e3, 2),
+                           "latency_p50_ms": q(seq_done, 50), "latency_p99_ms": q(seq_done, 99)},
+            "batched": {"ms": round(batch_ms, 2), "device_ms": round(st.elapsed_ms, 2),
+                        "mpaths_per_s": round(paths / batch_ms / 1e3, 2), "rounds": int(st.kernel_launches[0]),
+                        "latency_p50_ms": q(done, 50), "latency_p99_ms": q(done, 99),
+                        "config": {"max_paths": cfg.max_paths, "use_graph": cfg.use_graph},
+                        "autotune_s": round(tune_s, 2)}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +268,7 @@ def main() -> int:
     ap.add_argument("--net-scenes", type=int, default=32, help="training scenes for the learned denoiser")
     ap.add_argument("--net-ref-spp", type=int, default=1024, help="target samples per pixel of the training pairs")
     ap.add_argument("--net-steps", type=int, default=3000, help="Adam steps (0: no learned denoiser)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched request stream")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the library's streaming copy of 256 MiB three times (the known-size kernel "
                          "the FETCH_SIZE / WRITE_SIZE counters are scaled by) and exit")
@@ -345,6 +407,8 @@ def main() -> int:
                 "held_out": "the benchmark scene (seed 0, frequency 229) is not in the training set"}
         out["time_to_relmse"] = time_to_relmse(L, gpu, scene, blob, a.width, a.height, a.depth, 0.01,
                                                a.relmse_ref_spp, a.relmse_max_spp, net)
+    if not a.no_batch:
+        out["batch"] = batch_c4(L, gpu)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(blob, u, a.cpu_seconds)
     else:

@@ -447,12 +447,12 @@ typedef struct lucida_render_stats {
 
 /* Tunables of the wavefront. Zero fields take the defaults. */
 typedef struct lucida_wavefront_config {
-    uint32_t samples_per_batch; /* samples of every pixel in flight at once  */
-    uint32_t chunk;             /* queue items a wave claims per grab         */
-    uint32_t blocks_per_cu;     /* persistent workgroups per compute unit     */
-    uint32_t use_graph;         /* capture the bounce loop in a hipGraph      */
-    uint32_t count_traversal;   /* keep node/triangle counters (costs a little) */
-    uint32_t profile_kernels;   /* time every kernel with HIP events          */
+    uint32_t samples_per_batch; /* samples of every pixel in flight at once (0: about 2^22 paths) */
+    uint32_t max_paths;         /* batched renders: paths per round at most (0: 2^24)            */
+    uint32_t reserved;          /* must be 0                                                      */
+    uint32_t use_graph;         /* batched renders: capture every round's launches in one hipGraph */
+    uint32_t count_traversal;   /* keep node/triangle counters (costs a little)                   */
+    uint32_t profile_kernels;   /* time every kernel with HIP events (single renders)             */
 } lucida_wavefront_config;
 
 /* wavefront::render_native_on (crates/gpu/src/wavefront.rs:134): renders
@@ -463,6 +463,31 @@ typedef struct lucida_wavefront_config {
 lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene, const GpuUniforms* uniforms,
                                       uint32_t samples, lucida_film* film, const lucida_wavefront_config* config,
                                       lucida_render_stats* stats);
+
+/* One render request of a batch: like one lucida_render_wavefront call. */
+typedef struct lucida_job {
+    const lucida_scene* scene;
+    GpuUniforms uniforms; /* width/height must match the film; sample_offset, max_depth, seeds as usual */
+    uint32_t samples;
+    lucida_film* film;    /* distinct per job; the render ADDS into it */
+} lucida_job;
+
+/* Many renders of uneven resolution, samples and scene in one wavefront.
+ * The reference renders one request at a time (wavefront.rs:163-212 batches
+ * only the samples of one render); this entry point has no counterpart.
+ * Rounds gather one chunk of samples from as many jobs as fit in
+ * config->max_paths paths; all of a round's paths share one launch of each
+ * stage, every kernel finding a path's job (scene, uniforms, film) in a
+ * device table. Each job is cut into the chunks its single render would use,
+ * so its film is bit-identical to a lucida_render_wavefront of the same job.
+ * With config->use_graph the whole batch is captured once and replayed as one
+ * hipGraph launch. job_done_ms (optional, n floats) receives each job's
+ * completion time from the start of the batch (device clock). stats as for a
+ * single render; kernel_launches[0] = rounds. */
+lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_t n,
+                                  const lucida_wavefront_config* config, float* job_done_ms,
+                                  lucida_render_stats* stats);
+
 
 /* The GGX directional-albedo table the BSDF's energy compensation reads
  * (energy::compute_table, crates/core/src/bsdf.rs:1200-1227), built on the

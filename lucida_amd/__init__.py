@@ -90,8 +90,13 @@ class RenderStats(C.Structure):
 
 
 class WavefrontConfig(C.Structure):
-    _fields_ = [("samples_per_batch", C.c_uint32), ("chunk", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+    _fields_ = [("samples_per_batch", C.c_uint32), ("max_paths", C.c_uint32), ("reserved", C.c_uint32),
                 ("use_graph", C.c_uint32), ("count_traversal", C.c_uint32), ("profile_kernels", C.c_uint32)]
+
+
+class Job(C.Structure):
+    """lucida_job: one render request of a batch."""
+    _fields_ = [("scene", C.c_void_p), ("uniforms", Uniforms), ("samples", C.c_uint32), ("film", C.c_void_p)]
 
 
 class DenoiseParams(C.Structure):
@@ -162,6 +167,7 @@ def lib() -> C.CDLL:
             "lucida_scene_counts": ([P, C.POINTER(S), C.POINTER(S), C.POINTER(S), C.POINTER(U32)], I),
             "lucida_scene_download_instances": ([P, P], I),
             "lucida_scene_cast": ([P, P, P, S, I, P, P], I),
+            "lucida_render_batch": ([P, P, S, C.POINTER(WavefrontConfig), P, C.POINTER(RenderStats)], I),
             "lucida_blob_instancing": ([P, C.POINTER(P), C.POINTER(S), C.POINTER(P), C.POINTER(S)], I),
             "lucida_film_create": ([P, U32, U32, C.POINTER(P)], I),
             "lucida_film_destroy": ([P], None),
@@ -504,6 +510,23 @@ def render_wavefront(gpu: Gpu, scene: Scene, uniforms: Uniforms, samples: int, f
     _check(lib().lucida_render_wavefront(gpu._h, scene._h, C.byref(uniforms), samples, film._h, cfg,
                                          C.byref(st)))
     return st
+
+
+def render_batch(gpu: Gpu, jobs, config: WavefrontConfig | None = None):
+    """Many render requests in one wavefront (lucida_render_batch). `jobs` is a
+    list of (scene, uniforms, samples, film); returns (stats, per-job completion
+    times in ms from the start of the batch)."""
+    arr = (Job * len(jobs))()
+    for k, (scene, u, samples, film) in enumerate(jobs):
+        arr[k].scene = scene._h.value
+        arr[k].uniforms = u
+        arr[k].samples = samples
+        arr[k].film = film._h.value
+    done = np.zeros(len(jobs), np.float32)
+    st = RenderStats()
+    cfg = C.byref(config) if config is not None else None
+    _check(lib().lucida_render_batch(gpu._h, arr, len(jobs), cfg, _ptr(done), C.byref(st)))
+    return st, done
 
 
 def scene_cast(gpu: Gpu, scene: Scene, rays: np.ndarray, any_hit: bool = False):

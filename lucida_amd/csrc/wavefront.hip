@@ -66,6 +66,7 @@ struct Planes {
     uint32_t* q0;
     uint32_t* q1;
     uint32_t* counters;  // see SHADOW_AT / STATS_AT / TOTALS_AT
+    uint32_t* job;       // job of each path (batched renders only)
 };
 
 struct SceneView {
@@ -83,6 +84,22 @@ struct SceneView {
     uint32_t num_prims, num_tris, num_nodes, num_lights;
     uint32_t num_instances, tlas_root;
 };
+
+// One job of a batched render (lucida_render_batch): its scene, this round's
+// uniforms (sample_offset and samples_per_launch of the round's chunk of its
+// samples), its film, and where its paths and pixels start in the round.
+struct BatchJob {
+    SceneView sc;
+    GpuUniforms u;
+    GpuAccum* film;
+    uint32_t path_base, pixel_base, pixels, _pad;
+};
+
+// The wave-uniform job of a lane: kernels of a batched render process the
+// distinct jobs of a wave one at a time (a waterfall over readfirstlane), so
+// each job's scene and uniforms are read once per wave through the scalar
+// unit. Paths of one job are contiguous, so a wave almost always holds one.
+__device__ inline uint32_t first_job(uint32_t jid) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)jid); }
 
 __device__ inline uint64_t lanemask_lt() {
     const int lane = threadIdx.x & 63;
@@ -131,15 +148,12 @@ __device__ inline void flush_traversal_stats(uint32_t* counters, uint32_t visits
 // GENERATE (generate.wgsl:36-88): camera ray per (pixel, sample); identity
 // queue; bounce-0 count.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_generate(const GpuUniforms* __restrict__ U, Planes p) {
-    const GpuUniforms u = *U;
+// Path `index` of the batch; `local` is its index within its own render
+// (sample-major: local = sample * pixels + pixel).
+__device__ inline void generate_path(const GpuUniforms& u, uint32_t index, uint32_t local, Planes& p) {
     const uint32_t pixels = u.width * u.height;
-    const uint32_t total = pixels * u.samples_per_launch;
-    const uint32_t index = blockIdx.x * 256 + threadIdx.x;
-    if (index == 0) p.counters[0] = total;
-    if (index >= total) return;
-    const uint32_t pixel = index % pixels;
-    const uint32_t sample = u.sample_offset + index / pixels;
+    const uint32_t pixel = local % pixels;
+    const uint32_t sample = u.sample_offset + local / pixels;
     const uint32_t x = pixel % u.width, y = pixel / u.width;
     Sampler s = make_sampler(u, pixel, sample);
     float pu, pv, lu, lv;
@@ -167,6 +181,44 @@ __global__ __launch_bounds__(256) void k_generate(const GpuUniforms* __restrict_
     p.gal[index] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     p.gnr[index] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     p.q0[index] = index;
+}
+
+__global__ __launch_bounds__(256) void k_generate(const GpuUniforms* __restrict__ U, Planes p) {
+    const GpuUniforms u = *U;
+    const uint32_t total = u.width * u.height * u.samples_per_launch;
+    const uint32_t index = blockIdx.x * 256 + threadIdx.x;
+    if (index == 0) p.counters[0] = total;
+    if (index >= total) return;
+    generate_path(u, index, index, p);
+}
+
+// Largest j < n with base(j) <= x (bases ascending, base(0) = 0).
+template <class F>
+__device__ inline uint32_t find_job(uint32_t n, uint32_t x, F base) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (base(mid) <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// GENERATE for one round of a batched render: the round's paths are the
+// jobs' chunks laid end to end, each in the single render's order.
+__global__ __launch_bounds__(256) void k_generate_batch(const BatchJob* __restrict__ jobs, uint32_t njobs,
+                                                        uint32_t total, Planes p) {
+    const uint32_t index = blockIdx.x * 256 + threadIdx.x;
+    if (index == 0) p.counters[0] = total;
+    if (index >= total) return;
+    const uint32_t jid = find_job(njobs, index, [&](uint32_t k) { return jobs[k].path_base; });
+    for (;;) {
+        const uint32_t j = first_job(jid);
+        if (jid != j) continue;
+        generate_path(jobs[j].u, index, index - jobs[j].path_base, p);
+        p.job[index] = j;
+        break;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -404,14 +456,21 @@ __device__ inline bool traverse_instances(const SceneView& sc, uint32_t* __restr
 // then the BVH with t_max = the best analytic hit. Writes (t, u, v, id):
 // id = triangle, ANALYTIC | primitive, or MISS.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc, Planes p, const uint32_t* __restrict__ queue,
+template <bool BATCH>
+__global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJob* __restrict__ jobs, Planes p,
+                                                     const uint32_t* __restrict__ queue,
                                                      const uint32_t* __restrict__ count_ptr, uint32_t stats_on) {
     __shared__ uint32_t stack_mem[STACK * TRACE_WG];
     const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (i < count) {
-        const uint32_t pid = queue[i];
+      const uint32_t pid = queue[i];
+      const uint32_t jid = BATCH ? p.job[pid] : 0u;
+      for (;;) {
+        const uint32_t j = BATCH ? first_job(jid) : 0u;
+        if (jid != j) continue;
+        const SceneView& sc = BATCH ? jobs[j].sc : sc0;
         const float4 of = p.org[pid], df = p.dir[pid];
         const V3 o = f4xyz(of), d = f4xyz(df);
         float best_t = T_MAX;
@@ -441,6 +500,8 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc, Planes p, con
         else if (best != MISS) out = make_float4(best_t, 0.0f, 0.0f, __uint_as_float(ANALYTIC | best));
         else out = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(MISS));
         p.hit[i] = out;
+        break;
+      }
     }
     flush_traversal_stats(p.counters, visits, tests, overflow, stats_on, 0);
 }
@@ -448,7 +509,9 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc, Planes p, con
 // ---------------------------------------------------------------------------
 // SHADE (shade.wgsl:120-348 = integrator.rs radiance_with_guide, one bounce).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(WG) void k_shade(SceneView sc, const GpuUniforms* __restrict__ U, Planes p,
+template <bool BATCH>
+__global__ __launch_bounds__(WG) void k_shade(SceneView sc0, const GpuUniforms* __restrict__ U,
+                                              const BatchJob* __restrict__ jobs, Planes p,
                                               const uint32_t* __restrict__ queue_in,
                                               const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ queue_out,
                                               uint32_t* __restrict__ next_count, uint32_t* __restrict__ shadow_count) {
@@ -459,8 +522,14 @@ __global__ __launch_bounds__(WG) void k_shade(SceneView sc, const GpuUniforms* _
     V3 sr_o{0, 0, 0}, sr_d{0, 0, 0}, sr_c{0, 0, 0};
     float sr_t = 0.0f;
     if (i < count) {
-        const GpuUniforms& u = *U;
-        pid = queue_in[i];
+      pid = queue_in[i];
+      const uint32_t jid = BATCH ? p.job[pid] : 0u;
+      for (;;) {
+        const uint32_t j = BATCH ? first_job(jid) : 0u;
+        if (jid != j) continue;
+        const SceneView& sc = BATCH ? jobs[j].sc : sc0;
+        const GpuUniforms& u = BATCH ? jobs[j].u : *U;
+        const uint32_t local = pid - (BATCH ? jobs[j].path_base : 0u);  // index within its own job
         const float4 hit = p.hit[i];
         float4 of = p.org[pid], df = p.dir[pid], tf = p.thr[pid], pf = p.prv[pid], rf = p.rad[pid];
         const uint32_t depth = __float_as_uint(of.w);
@@ -471,8 +540,8 @@ __global__ __launch_bounds__(WG) void k_shade(SceneView sc, const GpuUniforms* _
         Sampler s;
         s.sobol = u.sampler_kind == LUCIDA_SAMPLER_SOBOL;
         s.state = __float_as_uint(df.w);
-        s.index = u.sample_offset + pid / pixels;
-        s.seed = hash_u32(pid % pixels + u.frame_seed * 0x9e3779b9u);
+        s.index = u.sample_offset + local / pixels;
+        s.seed = hash_u32(local % pixels + u.frame_seed * 0x9e3779b9u);
         s.dim = __float_as_uint(pf.w);
         const uint32_t mode = u.sampling_mode > 2 ? 2 : u.sampling_mode;
         const uint32_t total = (uint32_t)sc.num_lights + (sc.env.present() ? 1u : 0u);
@@ -663,6 +732,8 @@ __global__ __launch_bounds__(WG) void k_shade(SceneView sc, const GpuUniforms* _
             }
             p.rad[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
         }
+        break;
+      }
     }
     // Compaction: one atomic per wave for each queue.
     const uint32_t sslot = wave_append(shadow, shadow_count);
@@ -680,14 +751,20 @@ __global__ __launch_bounds__(WG) void k_shade(SceneView sc, const GpuUniforms* _
 // analytic primitives and the BVH; an unoccluded ray adds its contribution
 // to its path's radiance (one shadow ray per path per bounce: no races).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc, Planes p, const uint32_t* __restrict__ count_ptr,
-                                                      uint32_t stats_on) {
+template <bool BATCH>
+__global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const BatchJob* __restrict__ jobs, Planes p,
+                                                      const uint32_t* __restrict__ count_ptr, uint32_t stats_on) {
     __shared__ uint32_t stack_mem[STACK * TRACE_WG];
     const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (i < count) {
-        const float4 a = p.sro[i], b = p.srd[i];
+      const float4 a = p.sro[i], b = p.srd[i];
+      const uint32_t jid = BATCH ? p.job[__float_as_uint(b.w)] : 0u;
+      for (;;) {
+        const uint32_t j = BATCH ? first_job(jid) : 0u;
+        if (jid != j) continue;
+        const SceneView& sc = BATCH ? jobs[j].sc : sc0;
         const V3 o = f4xyz(a), d = f4xyz(b);
         const float t_max = a.w;
         bool blocked = false;
@@ -715,6 +792,8 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc, Planes p, co
             r.z += c.z;
             p.rad[pid] = r;
         }
+        break;
+      }
     }
     flush_traversal_stats(p.counters, visits, tests, overflow, stats_on, 2);
 }
@@ -772,15 +851,14 @@ __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* _
 // RESOLVE (resolve.wgsl:305-338): per pixel, sum the batch's samples in sample
 // order and add them to the film.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_resolve(const GpuUniforms* __restrict__ U, Planes p, GpuAccum* accum) {
-    const GpuUniforms& u = *U;
+// Pixel px of a render whose paths start at `base`: its samples in order.
+__device__ inline void resolve_pixel(const GpuUniforms& u, const Planes& p, uint32_t base, uint32_t px,
+                                     GpuAccum* accum) {
     const uint32_t pixels = u.width * u.height;
-    const uint32_t px = blockIdx.x * 256 + threadIdx.x;
-    if (px >= pixels) return;
     V3 sum{0, 0, 0}, sq{0, 0, 0}, al{0, 0, 0}, nr{0, 0, 0};
     float dp = 0.0f;
     for (uint32_t j = 0; j < u.samples_per_launch; ++j) {
-        const uint32_t q = px + j * pixels;
+        const uint32_t q = base + px + j * pixels;
         const V3 r = f4xyz(p.rad[q]);
         const float4 ga = p.gal[q];
         sum = sum + r;
@@ -796,6 +874,27 @@ __global__ __launch_bounds__(256) void k_resolve(const GpuUniforms* __restrict__
     a2.x += nr.x; a2.y += nr.y; a2.z += nr.z;
     a3.x += sq.x; a3.y += sq.y; a3.z += sq.z;
     a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
+}
+
+__global__ __launch_bounds__(256) void k_resolve(const GpuUniforms* __restrict__ U, Planes p, GpuAccum* accum) {
+    const GpuUniforms& u = *U;
+    const uint32_t px = blockIdx.x * 256 + threadIdx.x;
+    if (px >= u.width * u.height) return;
+    resolve_pixel(u, p, 0u, px, accum);
+}
+
+// RESOLVE for one round of a batched render: every job's pixels, each into its film.
+__global__ __launch_bounds__(256) void k_resolve_batch(const BatchJob* __restrict__ jobs, uint32_t njobs,
+                                                       uint32_t total_pixels, Planes p) {
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= total_pixels) return;
+    const uint32_t jid = find_job(njobs, g, [&](uint32_t k) { return jobs[k].pixel_base; });
+    for (;;) {
+        const uint32_t j = first_job(jid);
+        if (jid != j) continue;
+        resolve_pixel(jobs[j].u, p, jobs[j].path_base, g - jobs[j].pixel_base, jobs[j].film);
+        break;
+    }
 }
 
 // Adds one batch's counters into the render's running totals (u64 at word
@@ -819,6 +918,11 @@ __global__ void k_fold_counters(uint32_t* counters, uint32_t depth) {
 // be captured in a graph and replayed.
 __global__ void k_clear_counters(uint32_t* counters, uint32_t n) {
     for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) counters[k] = 0;
+}
+
+// Device wall-clock stamp (constant-rate counter) after everything queued before it.
+__global__ void k_stamp(unsigned long long* out, uint32_t slot) {
+    if (threadIdx.x == 0) out[slot] = (unsigned long long)wall_clock64();
 }
 
 // bsdf.rs:1200-1227 energy::compute_table on the device: 32 x 32 entries,
@@ -895,7 +999,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
         LUCIDA_HIP(ctx->wf_planes.ensure(P * 16 * 7), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_hits.ensure(P * 20), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * 3), LUCIDA_OOM);
-        LUCIDA_HIP(ctx->wf_queues.ensure(P * 4 * 2), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_queues.ensure(P * 4 * 3), LUCIDA_OOM);
         ctx->wf_capacity = P;
     }
     LUCIDA_HIP(ctx->wf_counters.ensure(4096), LUCIDA_OOM);
@@ -917,6 +1021,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     out->src = sh + 2 * C;
     out->q0 = ctx->wf_queues.as<uint32_t>();
     out->q1 = ctx->wf_queues.as<uint32_t>() + C;
+    out->job = ctx->wf_queues.as<uint32_t>() + 2 * C;
     out->counters = ctx->wf_counters.as<uint32_t>();
     *dU = ctx->wf_uniforms.as<GpuUniforms>();
     return LUCIDA_OK;
@@ -1040,15 +1145,15 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
             uint32_t* qin = (b & 1) ? p.q1 : p.q0;
             uint32_t* qout = (b & 1) ? p.q0 : p.q1;
             mark(1, true);
-            hipLaunchKernelGGL(k_extend, dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc, p, qin,
-                               p.counters + b, stats_on);
+            hipLaunchKernelGGL(k_extend<false>, dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc, nullptr, p,
+                               qin, p.counters + b, stats_on);
             mark(1, false);
             mark(2, true);
-            hipLaunchKernelGGL(k_shade, dim3(grid_for(Pb, WG)), dim3(WG), 0, s, sc, dU, p, qin, p.counters + b, qout,
-                               p.counters + b + 1, p.counters + SHADOW_AT + b);
+            hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, WG)), dim3(WG), 0, s, sc, dU, nullptr, p, qin,
+                               p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
             mark(2, false);
             mark(3, true);
-            hipLaunchKernelGGL(k_connect, dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc, p,
+            hipLaunchKernelGGL(k_connect<false>, dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc, nullptr, p,
                                p.counters + SHADOW_AT + b, stats_on);
             mark(3, false);
         }
@@ -1115,6 +1220,189 @@ lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, cons
     d_rays.release();
     d_hits.release();
     d_inst.release();
+    return st;
+}
+
+lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_t n,
+                                  const lucida_wavefront_config* config, float* job_done_ms,
+                                  lucida_render_stats* stats) {
+    if (!ctx || (n && !jobs)) return set_error(LUCIDA_INVALID_ARG, "lucida_render_batch: null argument");
+    clear_error();
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    if (n == 0) return LUCIDA_OK;
+    if (n > 65536) return set_error(LUCIDA_INVALID_ARG, "lucida_render_batch: more than 65,536 jobs");
+    lucida_wavefront_config cfg{};
+    if (config) cfg = *config;
+    for (size_t j = 0; j < n; ++j) {
+        const lucida_job& J = jobs[j];
+        if (!J.scene || !J.film) return set_error(LUCIDA_INVALID_ARG, "lucida_render_batch: a job has no scene or film");
+        if (J.uniforms.width != J.film->width || J.uniforms.height != J.film->height)
+            return set_error(LUCIDA_INVALID_ARG, "lucida_render_batch: a job's uniforms do not match its film");
+        if (J.uniforms.max_depth > MAX_BOUNCES) return set_error(LUCIDA_INVALID_ARG, "max_depth above 64");
+        if (J.scene->num_bvh_nodes >= (1u << 28) || J.scene->num_triangles >= (1u << 28))
+            return set_error(LUCIDA_INVALID_ARG, "scene too large: indices are packed in 29 bits");
+        for (size_t k = 0; k < j; ++k)
+            if (jobs[k].film == J.film) return set_error(LUCIDA_INVALID_ARG, "lucida_render_batch: two jobs share a film");
+    }
+    lucida_status st = ensure_ggx_table(ctx);
+    if (st != LUCIDA_OK) return st;
+
+    // Each job is rendered in chunks of the single render's batch size, so
+    // its film is summed exactly as a lucida_render_wavefront call would.
+    std::vector<uint32_t> spb(n), chunks(n), next(n, 0), last_round(n, 0);
+    std::vector<size_t> pixels(n);
+    for (size_t j = 0; j < n; ++j) {
+        pixels[j] = (size_t)jobs[j].uniforms.width * jobs[j].uniforms.height;
+        uint32_t b = cfg.samples_per_batch ? cfg.samples_per_batch
+                                           : (uint32_t)std::max<size_t>(1, ((size_t)1 << 22) / pixels[j]);
+        spb[j] = std::max(1u, std::min(b, jobs[j].samples));
+        chunks[j] = (jobs[j].samples == 0 || jobs[j].uniforms.max_depth == 0)
+                        ? 0u
+                        : (jobs[j].samples + spb[j] - 1) / spb[j];
+    }
+    const size_t cap = cfg.max_paths ? cfg.max_paths : ((size_t)1 << 24);
+    struct Round {
+        size_t first, count, paths, pixels;
+        uint32_t depth;
+    };
+    std::vector<Round> rounds;
+    std::vector<BatchJob> table;
+    std::vector<SceneView> views(n);
+    for (size_t j = 0; j < n; ++j) views[j] = make_view(ctx, jobs[j].scene);
+    for (;;) {
+        Round r{table.size(), 0, 0, 0, 0};
+        for (size_t j = 0; j < n; ++j) {
+            if (next[j] >= chunks[j]) continue;
+            const uint32_t done = next[j] * spb[j];
+            const uint32_t this_spb = std::min(spb[j], jobs[j].samples - done);
+            const size_t paths = pixels[j] * this_spb;
+            if (r.count > 0 && r.paths + paths > cap) continue;
+            BatchJob b{};
+            b.sc = views[j];
+            b.u = jobs[j].uniforms;
+            b.u.sample_offset = jobs[j].uniforms.sample_offset + done;
+            b.u.samples_per_launch = this_spb;
+            b.film = jobs[j].film->accum;
+            b.path_base = (uint32_t)r.paths;
+            b.pixel_base = (uint32_t)r.pixels;
+            b.pixels = (uint32_t)pixels[j];
+            table.push_back(b);
+            r.paths += paths;
+            r.pixels += pixels[j];
+            r.depth = std::max(r.depth, jobs[j].uniforms.max_depth);
+            ++r.count;
+            ++next[j];
+            if (next[j] == chunks[j]) last_round[j] = (uint32_t)rounds.size();
+        }
+        if (r.count == 0) break;
+        if (r.paths >= 0x7FFFFFFFull / 4) return set_error(LUCIDA_INVALID_ARG, "a job's chunk exceeds 2^29 paths");
+        rounds.push_back(r);
+    }
+    if (rounds.empty()) return LUCIDA_OK;
+    size_t max_paths = 0;
+    for (const Round& r : rounds) max_paths = std::max(max_paths, r.paths);
+
+    Planes p;
+    GpuUniforms* dU = nullptr;
+    st = ensure_wavefront(ctx, max_paths, &p, &dU);
+    if (st != LUCIDA_OK) return st;
+    DevBuf d_table, d_stamps;
+    LUCIDA_HIP(d_table.ensure(table.size() * sizeof(BatchJob)), LUCIDA_OOM);
+    LUCIDA_HIP(d_stamps.ensure((rounds.size() + 1) * 8), LUCIDA_OOM);
+    hipStream_t s = ctx->stream;
+    st = LUCIDA_OK;
+    if (hipMemcpyAsync(d_table.ptr, table.data(), table.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s) !=
+        hipSuccess)
+        st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: job table upload failed");
+    const BatchJob* dt = d_table.as<BatchJob>();
+    unsigned long long* stamps = d_stamps.as<unsigned long long>();
+    const uint32_t stats_on = (stats && cfg.count_traversal) ? 1u : 0u;
+    const SceneView none{};
+
+    auto enqueue_all = [&]() {
+        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT + 16u);
+        hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, 0u);
+        for (size_t ri = 0; ri < rounds.size(); ++ri) {
+            const Round& r = rounds[ri];
+            const BatchJob* jr = dt + r.first;
+            const uint32_t nj = (uint32_t)r.count, P = (uint32_t)r.paths;
+            hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT);
+            hipLaunchKernelGGL(k_generate_batch, dim3(grid_for(P, 256)), dim3(256), 0, s, jr, nj, P, p);
+            for (uint32_t b = 0; b < r.depth; ++b) {
+                uint32_t* qin = (b & 1) ? p.q1 : p.q0;
+                uint32_t* qout = (b & 1) ? p.q0 : p.q1;
+                hipLaunchKernelGGL(k_extend<true>, dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s, none, jr, p,
+                                   qin, p.counters + b, stats_on);
+                hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, WG)), dim3(WG), 0, s, none, nullptr, jr, p, qin,
+                                   p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
+                hipLaunchKernelGGL(k_connect<true>, dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s, none, jr, p,
+                                   p.counters + SHADOW_AT + b, stats_on);
+            }
+            hipLaunchKernelGGL(k_resolve_batch, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, jr, nj,
+                               (uint32_t)r.pixels, p);
+            hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
+            hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, (uint32_t)(ri + 1));
+        }
+    };
+
+    if (st == LUCIDA_OK) {
+        LUCIDA_HIP(hipEventRecord(ctx->ev0, s), LUCIDA_VALIDATION);
+        if (cfg.use_graph) {
+            // One graph for the whole batch: every launch above is a kernel
+            // node (no memset, no host copy), replayed with one launch.
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: stream capture failed to start");
+            } else {
+                enqueue_all();
+                const hipError_t ec = hipStreamEndCapture(s, &graph);
+                if (ec != hipSuccess || !graph)
+                    st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: stream capture failed");
+                else if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+                    st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: graph instantiation failed");
+                else if (hipGraphLaunch(exec, s) != hipSuccess)
+                    st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: graph launch failed");
+            }
+            (void)hipStreamSynchronize(s);
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+        } else {
+            enqueue_all();
+            if (hipGetLastError() != hipSuccess) st = set_error(LUCIDA_SHADER, "lucida_render_batch: launch failed");
+        }
+        LUCIDA_HIP(hipEventRecord(ctx->ev1, s), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipEventSynchronize(ctx->ev1), LUCIDA_VALIDATION);
+    }
+    if (st == LUCIDA_OK) LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
+    if (st == LUCIDA_OK && (stats || job_done_ms)) {
+        std::vector<unsigned long long> h_stamps(rounds.size() + 1);
+        uint64_t totals[8];
+        LUCIDA_HIP(hipMemcpy(h_stamps.data(), stamps, h_stamps.size() * 8, hipMemcpyDeviceToHost), LUCIDA_READBACK);
+        LUCIDA_HIP(hipMemcpy(totals, p.counters + TOTALS_AT, sizeof totals, hipMemcpyDeviceToHost), LUCIDA_READBACK);
+        int rate_khz = 0;
+        LUCIDA_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device), LUCIDA_VALIDATION);
+        if (job_done_ms)
+            for (size_t j = 0; j < n; ++j)
+                job_done_ms[j] = chunks[j] == 0 || rate_khz <= 0
+                                     ? 0.0f
+                                     : (float)((double)(h_stamps[last_round[j] + 1] - h_stamps[0]) / rate_khz);
+        if (stats) {
+            float ms = 0.0f;
+            LUCIDA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), LUCIDA_VALIDATION);
+            stats->extension_rays = totals[0];
+            stats->shadow_rays = totals[1];
+            stats->extend_node_visits = totals[2];
+            stats->extend_triangle_tests = totals[3];
+            stats->node_visits = totals[2] + totals[4];
+            stats->triangle_tests = totals[3] + totals[5];
+            stats->stack_overflows = totals[6];
+            stats->elapsed_ms = ms;
+            stats->kernel_launches[0] = (uint32_t)rounds.size();
+        }
+    }
+    d_table.release();
+    d_stamps.release();
     return st;
 }
 
