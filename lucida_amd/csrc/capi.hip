@@ -111,7 +111,7 @@ void lucida_ctx_destroy(lucida_ctx* ctx) {
                       &ctx->lb_subtree, &ctx->lb_flags,  &ctx->lb_misc,  &ctx->lb_rank,   &ctx->lb_nodes,
                       &ctx->lb_tris,   &ctx->lb_pos,     &ctx->lb_prims, &ctx->lb_level, &ctx->ggx_table, &ctx->wf_planes,
                       &ctx->wf_hits,   &ctx->wf_shadow,  &ctx->wf_queues, &ctx->wf_counters, &ctx->wf_uniforms,
-                      &ctx->dn_buf};
+                      &ctx->wf_spill,  &ctx->dn_buf};
     for (DevBuf* b : bufs) b->release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);

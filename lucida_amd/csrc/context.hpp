@@ -62,7 +62,7 @@ struct lucida_ctx {
     lucida::DevBuf ggx_table;
     bool ggx_ready = false;
     // Wavefront path state, hits, shadow rays, queues, counters, uniforms.
-    lucida::DevBuf wf_planes, wf_hits, wf_shadow, wf_queues, wf_counters, wf_uniforms;
+    lucida::DevBuf wf_planes, wf_hits, wf_shadow, wf_queues, wf_counters, wf_uniforms, wf_spill;
     size_t wf_capacity = 0;
     // Denoiser planes.
     lucida::DevBuf dn_buf;

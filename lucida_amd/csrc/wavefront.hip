@@ -40,6 +40,7 @@ constexpr uint32_t ANALYTIC = 0x80000000u;
 constexpr int WG = 64;          // one wave per workgroup for the per-path kernels
 constexpr int TRACE_WG = 128;   // traversal workgroups (two waves)
 constexpr int STACK = 32;       // traversal stack entries per lane (shaders/common/bvh.wgsl:133)
+constexpr int LDS_STACK = 16;   // the newest entries, in LDS; older ones spill to memory
 constexpr int MAX_BOUNCES = 64;
 // Counter words: queue length per bounce at [0, 72) (bounce b+1's count is
 // written by bounce b's SHADE), shadow-ray count per bounce at [SHADOW_AT,
@@ -67,6 +68,7 @@ struct Planes {
     uint32_t* q1;
     uint32_t* counters;  // see SHADOW_AT / STATS_AT / TOTALS_AT
     uint32_t* job;       // job of each path (batched renders only)
+    uint32_t* spill;     // traversal stack spill columns (STACK - LDS_STACK per lane)
 };
 
 struct SceneView {
@@ -278,14 +280,50 @@ __device__ inline bool tri_test(const float4* __restrict__ edges, uint32_t ti, V
 
 __device__ inline uint32_t pack_child(uint32_t first, uint32_t count) { return (first << 3) | count; }
 
+// A lane's traversal stack of STACK entries (bvh.wgsl:126-133). The newest
+// LDS_STACK entries live in an LDS ring (one column per lane of the block),
+// older ones in a per-lane column of device memory, so the LDS held per wave
+// is halved and twice the waves fit on a CU; deep stacks are rare, so the
+// spill column is rarely touched. A push onto a full stack is dropped (and
+// counted), as in the reference.
+struct RayStack {
+    uint32_t* ring;        // LDS, entry k at ring[(k % LDS_STACK) * TRACE_WG]
+    uint32_t* spill;       // device memory, entry k at spill[k * spill_stride]
+    uint32_t spill_stride;
+    int sp, lo;            // entries [0, lo) spilled, [lo, sp) in the ring
+    __device__ inline bool push(uint32_t v) {
+        if (sp >= STACK) return false;
+        if (sp - lo == LDS_STACK) {
+            spill[(size_t)lo * spill_stride] = ring[(lo % LDS_STACK) * TRACE_WG];
+            ++lo;
+        }
+        ring[(sp % LDS_STACK) * TRACE_WG] = v;
+        ++sp;
+        return true;
+    }
+    __device__ inline uint32_t pop() {
+        --sp;
+        if (sp < lo) {
+            lo = sp;
+            return spill[(size_t)sp * spill_stride];
+        }
+        return ring[(sp % LDS_STACK) * TRACE_WG];
+    }
+};
+
+__device__ inline RayStack make_stack(uint32_t* lds_base, uint32_t* spill_base) {
+    const uint32_t lanes = gridDim.x * TRACE_WG;
+    return RayStack{lds_base + threadIdx.x, spill_base + blockIdx.x * TRACE_WG + threadIdx.x, lanes, 0, 0};
+}
+
 // Closest hit (any_hit = false) or occlusion (any_hit = true) against the
 // triangles of the tree rooted at `root` (intersect_bvh_from, bvh.wgsl:137:
 // the root's own box is not tested). Returns the hit flag; closest-hit fills
-// t/u/v/tri. `stk` is this lane's stack column with `capacity` entries.
+// t/u/v/tri. Entries the walk pushes sit above the stack's current top.
 template <bool ANY_HIT>
-__device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk, int capacity, uint32_t root_index,
-                                V3 o, V3 d, float t_min, float t_max, float* t_out, float* u_out, float* v_out,
-                                uint32_t* tri_out, uint32_t* visits, uint32_t* tests, uint32_t* overflow) {
+__device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root_index, V3 o, V3 d, float t_min,
+                                float t_max, float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
+                                uint32_t* visits, uint32_t* tests, uint32_t* overflow) {
     bool found = false;
     float closest = t_max;
     if (sc.num_nodes == 0) {
@@ -303,8 +341,7 @@ __device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk,
     const V3 inv = safe_inv_dir(d);
     const GpuBvhNode root = sc.nodes[root_index];
     uint32_t cur = pack_child(root.left_first, root.count);
-    int sp = 0;
-    const int lane_stride = TRACE_WG;
+    const int base = st.sp;
     // A tree visits each node at most once per ray; the cap only guarantees
     // termination on a malformed (cyclic) node array.
     const uint32_t cap = 2u * sc.num_nodes + 8u;
@@ -333,12 +370,7 @@ __device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk,
             const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
             if (ha && hb) {
                 const bool a_near = ANY_HIT ? true : ea <= eb;
-                if (sp < capacity) {
-                    stk[sp * lane_stride] = a_near ? cb : ca;
-                    ++sp;
-                } else {
-                    *overflow += 1;
-                }
+                if (!st.push(a_near ? cb : ca)) *overflow += 1;
                 cur = a_near ? ca : cb;
                 continue;
             } else if (ha) {
@@ -349,10 +381,11 @@ __device__ inline bool traverse(const SceneView& sc, uint32_t* __restrict__ stk,
                 continue;
             }
         }
-        if (sp == 0) break;
-        --sp;
-        cur = stk[sp * lane_stride];
+        if (st.sp == base) break;
+        cur = st.pop();
     }
+    st.sp = base;  // an any-hit return leaves entries behind
+    if (st.lo > base) st.lo = base;
     return found;
 }
 
@@ -389,8 +422,8 @@ __device__ inline V3 xf_normal(const InstanceXf& I, V3 n) {
 // is the two-level occlusion query (the reference's shadow rays walk only the
 // tree at node 0, see connect.wgsl:192-212; here they see every instance).
 template <bool ANY_HIT>
-__device__ inline bool traverse_instances(const SceneView& sc, uint32_t* __restrict__ stk, V3 o, V3 d, float t_min,
-                                          float t_max, float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
+__device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 o, V3 d, float t_min, float t_max,
+                                          float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
                                           uint32_t* inst_out, uint32_t* visits, uint32_t* tests,
                                           uint32_t* overflow) {
     bool found = false;
@@ -398,8 +431,6 @@ __device__ inline bool traverse_instances(const SceneView& sc, uint32_t* __restr
     const V3 inv = safe_inv_dir(d);
     const GpuBvhNode root = sc.nodes[sc.tlas_root];
     uint32_t cur = pack_child(root.left_first, root.count);
-    int sp = 0;
-    const int lane_stride = TRACE_WG;
     const uint32_t cap = 2u * sc.num_nodes + 8u;
     for (uint32_t iter = 0; iter < cap; ++iter) {
         *visits += 1;
@@ -410,8 +441,8 @@ __device__ inline bool traverse_instances(const SceneView& sc, uint32_t* __restr
                 const InstanceXf I = load_instance(sc.insts, ii);
                 float t, u, v;
                 uint32_t tri;
-                if (traverse<ANY_HIT>(sc, stk + sp * lane_stride, STACK - sp, I.blas_root, xf_point(I, o),
-                                      xf_vector(I, d), t_min, closest, &t, &u, &v, &tri, visits, tests, overflow)) {
+                if (traverse<ANY_HIT>(sc, st, I.blas_root, xf_point(I, o), xf_vector(I, d), t_min, closest, &t, &u,
+                                      &v, &tri, visits, tests, overflow)) {
                     if (ANY_HIT) return true;
                     closest = t;
                     *t_out = t; *u_out = u; *v_out = v; *tri_out = tri; *inst_out = ii;
@@ -428,12 +459,7 @@ __device__ inline bool traverse_instances(const SceneView& sc, uint32_t* __restr
             const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
             if (ha && hb) {
                 const bool a_near = ANY_HIT ? true : ea <= eb;
-                if (sp < STACK) {
-                    stk[sp * lane_stride] = a_near ? cb : ca;
-                    ++sp;
-                } else {
-                    *overflow += 1;
-                }
+                if (!st.push(a_near ? cb : ca)) *overflow += 1;
                 cur = a_near ? ca : cb;
                 continue;
             } else if (ha) {
@@ -444,9 +470,8 @@ __device__ inline bool traverse_instances(const SceneView& sc, uint32_t* __restr
                 continue;
             }
         }
-        if (sp == 0) break;
-        --sp;
-        cur = stk[sp * lane_stride];
+        if (st.sp == 0) break;
+        cur = st.pop();
     }
     return found;
 }
@@ -456,15 +481,18 @@ __device__ inline bool traverse_instances(const SceneView& sc, uint32_t* __restr
 // then the BVH with t_max = the best analytic hit. Writes (t, u, v, id):
 // id = triangle, ANALYTIC | primitive, or MISS.
 // ---------------------------------------------------------------------------
-template <bool BATCH>
+// INST = false compiles out the two-level path (single-level scenes keep
+// the traversal kernels' registers low).
+template <bool BATCH, bool INST>
 __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJob* __restrict__ jobs, Planes p,
                                                      const uint32_t* __restrict__ queue,
                                                      const uint32_t* __restrict__ count_ptr, uint32_t stats_on) {
-    __shared__ uint32_t stack_mem[STACK * TRACE_WG];
+    __shared__ uint32_t stack_mem[LDS_STACK * TRACE_WG];
     const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (i < count) {
+      RayStack st = make_stack(stack_mem, p.spill);
       const uint32_t pid = queue[i];
       const uint32_t jid = BATCH ? p.job[pid] : 0u;
       for (;;) {
@@ -489,12 +517,12 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJ
         uint32_t tri = 0, inst = 0;
         // Instanced scenes take the two-level path instead of the
         // single-level one (scene.wgsl:31-75): their triangles live in object space.
-        const bool th = sc.num_instances
-                            ? traverse_instances<false>(sc, stack_mem + threadIdx.x, o, d, T_MIN, best_t, &t, &u, &v,
-                                                        &tri, &inst, &visits, &tests, &overflow)
-                            : traverse<false>(sc, stack_mem + threadIdx.x, STACK, 0, o, d, T_MIN, best_t, &t, &u, &v,
-                                              &tri, &visits, &tests, &overflow);
-        if (sc.num_instances) p.hin[i] = inst;
+        const bool two_level = INST && sc.num_instances;
+        const bool th = two_level ? traverse_instances<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &inst,
+                                                              &visits, &tests, &overflow)
+                                  : traverse<false>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
+                                                    &overflow);
+        if (two_level) p.hin[i] = inst;
         float4 out;
         if (th) out = make_float4(t, u, v, __uint_as_float(tri));
         else if (best != MISS) out = make_float4(best_t, 0.0f, 0.0f, __uint_as_float(ANALYTIC | best));
@@ -751,14 +779,15 @@ __global__ __launch_bounds__(WG) void k_shade(SceneView sc0, const GpuUniforms* 
 // analytic primitives and the BVH; an unoccluded ray adds its contribution
 // to its path's radiance (one shadow ray per path per bounce: no races).
 // ---------------------------------------------------------------------------
-template <bool BATCH>
+template <bool BATCH, bool INST>
 __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const BatchJob* __restrict__ jobs, Planes p,
                                                       const uint32_t* __restrict__ count_ptr, uint32_t stats_on) {
-    __shared__ uint32_t stack_mem[STACK * TRACE_WG];
+    __shared__ uint32_t stack_mem[LDS_STACK * TRACE_WG];
     const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (i < count) {
+      RayStack st = make_stack(stack_mem, p.spill);
       const float4 a = p.sro[i], b = p.srd[i];
       const uint32_t jid = BATCH ? p.job[__float_as_uint(b.w)] : 0u;
       for (;;) {
@@ -777,11 +806,11 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
         if (!blocked) {
             float t, u, v;
             uint32_t tri, inst;
-            blocked = sc.num_instances
-                          ? traverse_instances<true>(sc, stack_mem + threadIdx.x, o, d, T_MIN, t_max, &t, &u, &v, &tri,
-                                                     &inst, &visits, &tests, &overflow)
-                          : traverse<true>(sc, stack_mem + threadIdx.x, STACK, 0, o, d, T_MIN, t_max, &t, &u, &v,
-                                           &tri, &visits, &tests, &overflow);
+            blocked = (INST && sc.num_instances)
+                          ? traverse_instances<true>(sc, st, o, d, T_MIN, t_max, &t, &u, &v, &tri, &inst, &visits,
+                                                     &tests, &overflow)
+                          : traverse<true>(sc, st, 0, o, d, T_MIN, t_max, &t, &u, &v, &tri, &visits, &tests,
+                                           &overflow);
         }
         if (!blocked) {
             const uint32_t pid = __float_as_uint(b.w);
@@ -806,10 +835,11 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* __restrict__ rays, uint32_t n,
                                                    uint32_t any_hit, float4* __restrict__ hits,
-                                                   uint32_t* __restrict__ insts) {
-    __shared__ uint32_t stack_mem[STACK * TRACE_WG];
+                                                   uint32_t* __restrict__ insts, uint32_t* __restrict__ spill) {
+    __shared__ uint32_t stack_mem[LDS_STACK * TRACE_WG];
     const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
     if (i >= n) return;
+    RayStack st = make_stack(stack_mem, spill);
     const float4 a = rays[2 * i], b = rays[2 * i + 1];
     const V3 o = f4xyz(a), d = f4xyz(b);
     float best_t = b.w;
@@ -832,15 +862,15 @@ __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* _
     if (!(any_hit && best != MISS)) {
         bool got;
         if (any_hit)
-            got = sc.num_instances ? traverse_instances<true>(sc, stack_mem + threadIdx.x, o, d, T_MIN, best_t, &t, &u,
-                                                              &v, &tri, &inst, &visits, &tests, &overflow)
-                                   : traverse<true>(sc, stack_mem + threadIdx.x, STACK, 0, o, d, T_MIN, best_t, &t,
-                                                    &u, &v, &tri, &visits, &tests, &overflow);
+            got = sc.num_instances ? traverse_instances<true>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &inst,
+                                                              &visits, &tests, &overflow)
+                                   : traverse<true>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
+                                                    &overflow);
         else
-            got = sc.num_instances ? traverse_instances<false>(sc, stack_mem + threadIdx.x, o, d, T_MIN, best_t, &t,
-                                                               &u, &v, &tri, &inst, &visits, &tests, &overflow)
-                                   : traverse<false>(sc, stack_mem + threadIdx.x, STACK, 0, o, d, T_MIN, best_t, &t,
-                                                     &u, &v, &tri, &visits, &tests, &overflow);
+            got = sc.num_instances ? traverse_instances<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &inst,
+                                                               &visits, &tests, &overflow)
+                                   : traverse<false>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits,
+                                                     &tests, &overflow);
         if (got) out = any_hit ? make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u)) : make_float4(t, u, v, __uint_as_float(tri));
     }
     hits[i] = out;
@@ -1000,6 +1030,8 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
         LUCIDA_HIP(ctx->wf_hits.ensure(P * 20), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * 3), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_queues.ensure(P * 4 * 3), LUCIDA_OOM);
+        // spill columns for every lane of a traversal grid over P items
+        LUCIDA_HIP(ctx->wf_spill.ensure((P + TRACE_WG) * (STACK - LDS_STACK) * 4), LUCIDA_OOM);
         ctx->wf_capacity = P;
     }
     LUCIDA_HIP(ctx->wf_counters.ensure(4096), LUCIDA_OOM);
@@ -1022,6 +1054,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     out->q0 = ctx->wf_queues.as<uint32_t>();
     out->q1 = ctx->wf_queues.as<uint32_t>() + C;
     out->job = ctx->wf_queues.as<uint32_t>() + 2 * C;
+    out->spill = ctx->wf_spill.as<uint32_t>();
     out->counters = ctx->wf_counters.as<uint32_t>();
     *dU = ctx->wf_uniforms.as<GpuUniforms>();
     return LUCIDA_OK;
@@ -1145,16 +1178,24 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
             uint32_t* qin = (b & 1) ? p.q1 : p.q0;
             uint32_t* qout = (b & 1) ? p.q0 : p.q1;
             mark(1, true);
-            hipLaunchKernelGGL(k_extend<false>, dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc, nullptr, p,
-                               qin, p.counters + b, stats_on);
+            if (sc.num_instances)
+                hipLaunchKernelGGL((k_extend<false, true>), dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc,
+                                   nullptr, p, qin, p.counters + b, stats_on);
+            else
+                hipLaunchKernelGGL((k_extend<false, false>), dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc,
+                                   nullptr, p, qin, p.counters + b, stats_on);
             mark(1, false);
             mark(2, true);
             hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, WG)), dim3(WG), 0, s, sc, dU, nullptr, p, qin,
                                p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
             mark(2, false);
             mark(3, true);
-            hipLaunchKernelGGL(k_connect<false>, dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc, nullptr, p,
-                               p.counters + SHADOW_AT + b, stats_on);
+            if (sc.num_instances)
+                hipLaunchKernelGGL((k_connect<false, true>), dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc,
+                                   nullptr, p, p.counters + SHADOW_AT + b, stats_on);
+            else
+                hipLaunchKernelGGL((k_connect<false, false>), dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc,
+                                   nullptr, p, p.counters + SHADOW_AT + b, stats_on);
             mark(3, false);
         }
         mark(4, true);
@@ -1198,17 +1239,18 @@ lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, cons
     if (n > (1u << 26)) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_cast: more than 2^26 rays");
     clear_error();
     const SceneView sc = make_view(ctx, scene);
-    DevBuf d_rays, d_hits, d_inst;
+    DevBuf d_rays, d_hits, d_inst, d_spill;
     LUCIDA_HIP(d_rays.ensure(n * 32), LUCIDA_OOM);
     LUCIDA_HIP(d_hits.ensure(n * 16), LUCIDA_OOM);
     LUCIDA_HIP(d_inst.ensure(n * 4), LUCIDA_OOM);
+    LUCIDA_HIP(d_spill.ensure((n + TRACE_WG) * (STACK - LDS_STACK) * 4), LUCIDA_OOM);
     lucida_status st = LUCIDA_OK;
     if (hipMemcpyAsync(d_rays.ptr, rays, n * 32, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         st = set_error(LUCIDA_VALIDATION, "lucida_scene_cast: upload failed");
     if (st == LUCIDA_OK) {
         hipLaunchKernelGGL(k_cast, dim3(grid_for(n, TRACE_WG)), dim3(TRACE_WG), 0, ctx->stream, sc,
                            d_rays.as<float4>(), (uint32_t)n, any_hit ? 1u : 0u, d_hits.as<float4>(),
-                           d_inst.as<uint32_t>());
+                           d_inst.as<uint32_t>(), d_spill.as<uint32_t>());
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(hits, d_hits.ptr, n * 16, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
             (instances &&
@@ -1220,6 +1262,7 @@ lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, cons
     d_rays.release();
     d_hits.release();
     d_inst.release();
+    d_spill.release();
     return st;
 }
 
@@ -1268,7 +1311,11 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     std::vector<Round> rounds;
     std::vector<BatchJob> table;
     std::vector<SceneView> views(n);
-    for (size_t j = 0; j < n; ++j) views[j] = make_view(ctx, jobs[j].scene);
+    bool any_instanced = false;
+    for (size_t j = 0; j < n; ++j) {
+        views[j] = make_view(ctx, jobs[j].scene);
+        any_instanced = any_instanced || jobs[j].scene->num_instances > 0;
+    }
     for (;;) {
         Round r{table.size(), 0, 0, 0, 0};
         for (size_t j = 0; j < n; ++j) {
@@ -1331,12 +1378,20 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
             for (uint32_t b = 0; b < r.depth; ++b) {
                 uint32_t* qin = (b & 1) ? p.q1 : p.q0;
                 uint32_t* qout = (b & 1) ? p.q0 : p.q1;
-                hipLaunchKernelGGL(k_extend<true>, dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s, none, jr, p,
-                                   qin, p.counters + b, stats_on);
+                if (any_instanced)
+                    hipLaunchKernelGGL((k_extend<true, true>), dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s, none,
+                                       jr, p, qin, p.counters + b, stats_on);
+                else
+                    hipLaunchKernelGGL((k_extend<true, false>), dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s,
+                                       none, jr, p, qin, p.counters + b, stats_on);
                 hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, WG)), dim3(WG), 0, s, none, nullptr, jr, p, qin,
                                    p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
-                hipLaunchKernelGGL(k_connect<true>, dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s, none, jr, p,
-                                   p.counters + SHADOW_AT + b, stats_on);
+                if (any_instanced)
+                    hipLaunchKernelGGL((k_connect<true, true>), dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s,
+                                       none, jr, p, p.counters + SHADOW_AT + b, stats_on);
+                else
+                    hipLaunchKernelGGL((k_connect<true, false>), dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s,
+                                       none, jr, p, p.counters + SHADOW_AT + b, stats_on);
             }
             hipLaunchKernelGGL(k_resolve_batch, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, jr, nj,
                                (uint32_t)r.pixels, p);
