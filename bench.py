@@ -260,6 +260,8 @@ def main() -> int:
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--frequency", type=int, default=229)
     ap.add_argument("--samples-per-batch", type=int, default=0)
+    ap.add_argument("--bvh-width", type=int, default=0, choices=(0, 2, 4),
+                    help="0/2: the binary walk (default), 4: the 4-wide walk")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--relmse-ref-spp", type=int, default=4096)
@@ -306,7 +308,7 @@ def main() -> int:
     film = L.Film(gpu, a.width, a.height)
     u = blob.uniforms(a.width, a.height, samples=a.spp, max_depth=a.depth, sampling=L.SAMPLING_MIS,
                       sampler=L.SAMPLER_SOBOL)
-    cfg = L.WavefrontConfig(a.samples_per_batch, 0, 0, 0, 1, 1)
+    cfg = L.WavefrontConfig(a.samples_per_batch, 0, a.bvh_width, 0, 1, 1)
 
     for _ in range(a.warmup):
         film.clear()
@@ -341,9 +343,11 @@ def main() -> int:
 
     # Roofline of the dominant kernel. EXTEND's algorithmic bytes per ray:
     # queue index 4 + origin/direction planes 32 + hit write 16, plus 64 per
-    # node visit (the two 32-byte children) and 64 per triangle test (16-byte
+    # node visit (the two 32-byte children of a binary node; 128 for a 4-wide
+    # node's four boxes and child words) and 64 per triangle test (16-byte
     # triangle + three 16-byte positions).
-    ext_bytes = ext_rays * (4 + 32 + 16) + 64 * (ext_visits + ext_tests)
+    node_bytes = 128 if a.bvh_width == 4 else 64
+    ext_bytes = ext_rays * (4 + 32 + 16) + node_bytes * ext_visits + 64 * ext_tests
     ext_ms = agg["extend"]
     dominant = max(agg, key=agg.get)
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
@@ -378,6 +382,7 @@ def main() -> int:
             "device": dev_name, "compute_units": cus,
             "mrays_per_s": round((ext_rays + shadow_rays) / a.steps / sec / 1e6, 2),
             "extension_rays_per_step": ext_rays // a.steps, "shadow_rays_per_step": shadow_rays // a.steps,
+            "bvh_walk": "4-wide" if a.bvh_width == 4 else "binary",
             "node_visits_per_ray": round(visits / max(ext_rays + shadow_rays, 1), 2),
             "triangle_tests_per_ray": round(tests / max(ext_rays + shadow_rays, 1), 2),
             "kernel_ms_per_step": {k: round(v / a.steps, 3) for k, v in agg.items()},

@@ -412,13 +412,16 @@ lucida_status lucida_scene_download_bvh(lucida_scene* scene, GpuBvhNode* nodes, 
 /* Batch ray queries against a resident scene without shading: the closest
  * hit of Scene::intersect (scene.rs:452) or the occlusion test of
  * Scene::occluded (scene.rs:577), through the same traversal EXTEND and
- * CONNECT use. rays: n x 8 floats (origin.xyz, unused, direction.xyz, t_max),
+ * CONNECT use (flags below). rays: n x 8 floats (origin.xyz, unused, direction.xyz, t_max),
  * t_min is the renderer's T_MIN. hits: n x 4 (t, u, v, id bits), id = triangle
  * (leaf order), 0x80000000 | analytic primitive, or 0xFFFFFFFF; with any_hit
  * id is 0 for an occluded ray. instances (optional): the instance primitive
  * index of a triangle hit in an instanced scene. Host arrays. */
-lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, const float* rays, size_t n, int any_hit,
+lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, const float* rays, size_t n, int flags,
                                 float* hits, uint32_t* instances);
+/* flags for lucida_scene_cast: */
+#define LUCIDA_CAST_ANY_HIT 1 /* occlusion query */
+#define LUCIDA_CAST_WIDE 2    /* walk the scene's 4-wide BVH instead of the binary tree */
 
 /* A device-resident accumulation buffer (width*height GpuAccum). */
 lucida_status lucida_film_create(lucida_ctx* ctx, uint32_t width, uint32_t height, lucida_film** out);
@@ -449,7 +452,8 @@ typedef struct lucida_render_stats {
 typedef struct lucida_wavefront_config {
     uint32_t samples_per_batch; /* samples of every pixel in flight at once (0: about 2^22 paths) */
     uint32_t max_paths;         /* batched renders: paths per round at most (0: 2^24)            */
-    uint32_t reserved;          /* must be 0                                                      */
+    uint32_t bvh_width;         /* 0 or 2: the binary tree (the reference's order, whose node     *
+                                 * visit counts the oracle repeats); 4: the scene's 4-wide BVH    */
     uint32_t use_graph;         /* batched renders: capture every round's launches in one hipGraph */
     uint32_t count_traversal;   /* keep node/triangle counters (costs a little)                   */
     uint32_t profile_kernels;   /* time every kernel with HIP events (single renders)             */

@@ -65,8 +65,8 @@ static lucida_status upload(lucida_ctx* ctx, T** dst, const T* src, size_t count
 }
 
 static void scene_free(lucida_scene* s) {
-    void* ptrs[] = {s->materials, s->primitives, s->positions,       s->vertex_attrs, s->triangles, s->tri_edges,
-                    s->bvh_nodes, s->lights,     s->env_pixels,      s->env_conditional, s->env_marginal};
+    void* ptrs[] = {s->materials, s->primitives, s->positions,  s->vertex_attrs,    s->triangles,   s->tri_edges,
+                    s->wide,      s->bvh_nodes,  s->lights,     s->env_pixels,      s->env_conditional, s->env_marginal};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -196,6 +196,8 @@ lucida_status lucida_scene_create(lucida_ctx* ctx, const lucida_scene_desc* d, l
             return set_error(LUCIDA_INVALID_ARG, "primitive material out of range");
     for (size_t i = 0; i < d->num_lights; ++i)
         if (d->lights[i].material >= d->num_materials) return set_error(LUCIDA_INVALID_ARG, "light material out of range");
+    for (size_t i = 0; i < d->num_bvh_nodes; ++i)
+        if (d->bvh_nodes[i].count > 31) return set_error(LUCIDA_INVALID_ARG, "BVH leaf holds more than 31 primitives");
     if (d->num_instances == 0) {
         for (size_t i = 0; i < d->num_bvh_nodes; ++i) {
             const GpuBvhNode& nd = d->bvh_nodes[i];
@@ -288,6 +290,7 @@ lucida_status lucida_scene_create(lucida_ctx* ctx, const lucida_scene_desc* d, l
         }
     }
     if (st == LUCIDA_OK && s->num_triangles) st = build_tri_edges(ctx, s);
+    if (st == LUCIDA_OK && s->num_bvh_nodes) st = build_wide_bvh(ctx, s);
     if (st == LUCIDA_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
         st = set_error(LUCIDA_VALIDATION, "scene upload failed");
     if (st != LUCIDA_OK) {

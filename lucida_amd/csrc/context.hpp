@@ -76,6 +76,8 @@ struct lucida_scene {
     GpuVertexAttr* vertex_attrs = nullptr;
     GpuTriangle* triangles = nullptr;
     float4* tri_edges = nullptr;  // per triangle: p0, p1 - p0, p2 - p0 (the intersection's own operands)
+    void* wide = nullptr;         // 4-wide BVH collapsed from bvh_nodes (128-byte nodes), or null
+    size_t num_wide = 0;
     GpuBvhNode* bvh_nodes = nullptr;
     GpuLight* lights = nullptr;
     float* env_pixels = nullptr;
@@ -101,6 +103,12 @@ lucida_status ensure_ggx_table(lucida_ctx* ctx);
 lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
                                 GpuBvhNode* nodes, size_t* num_nodes, uint32_t* prim_indices,
                                 GpuTriangle* tris_sorted, float* stage_us);
+// Exclusive prefix sum of device words in place (lbvh.hip); `partials` holds
+// ceil(count / 2048) words.
+lucida_status scan_u32_device(lucida_ctx* ctx, uint32_t* data, uint32_t count, uint32_t* partials);
+// Builds scene->wide (the 4-wide BVH traversed by default) from its binary
+// BVH; a no-op for instanced scenes and scenes without a BVH.
+lucida_status build_wide_bvh(lucida_ctx* ctx, lucida_scene* scene);
 // Fills scene->tri_edges from its (final, leaf-ordered) triangles.
 lucida_status build_tri_edges(lucida_ctx* ctx, lucida_scene* scene);
 // A-trous filter (atrous.hip) without the final wait: reserve the planes for

@@ -634,6 +634,18 @@ inline uint32_t blocks_for(size_t n, int block) { return (uint32_t)((n + block -
 
 }  // namespace
 
+// Exclusive prefix sum of `count` device words in place; `partials` holds
+// ceil(count / 2048) words of scratch.
+lucida_status scan_u32_device(lucida_ctx* ctx, uint32_t* data, uint32_t count, uint32_t* partials) {
+    if (count == 0) return LUCIDA_OK;
+    const uint32_t segs = blocks_for(count, SCAN_SEG);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, data, count, partials);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, partials, segs);
+    hipLaunchKernelGGL(k_scan_down, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, data, count, partials);
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    return LUCIDA_OK;
+}
+
 // Radix-sorts device arrays keys/vals in place over bits [0, bits), using
 // tmp arrays of the same length and ctx's histogram scratch.
 static lucida_status sort_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,

@@ -29,6 +29,7 @@
 
 #include "context.hpp"
 #include "shading.hpp"
+#include "widebvh.hpp"
 
 namespace lucida {
 namespace {
@@ -41,6 +42,9 @@ constexpr int WG = 64;          // one wave per workgroup for the per-path kerne
 constexpr int TRACE_WG = 128;   // traversal workgroups (two waves)
 constexpr int STACK = 32;       // traversal stack entries per lane (shaders/common/bvh.wgsl:133)
 constexpr int LDS_STACK = 16;   // the newest entries, in LDS; older ones spill to memory
+constexpr int WIDE_STACK = 48;  // the 4-wide walk pushes up to three entries per visit
+// Traversal variants a kernel is compiled for.
+constexpr int TRAV_BINARY = 0, TRAV_WIDE = 1, TRAV_INSTANCED = 2;
 constexpr int MAX_BOUNCES = 64;
 // Counter words: queue length per bounce at [0, 72) (bounce b+1's count is
 // written by bounce b's SHADE), shadow-ray count per bounce at [SHADOW_AT,
@@ -80,6 +84,8 @@ struct SceneView {
     const float4* __restrict__ edges;  // per triangle: p0, p1 - p0, p2 - p0
     const GpuBvhNode* __restrict__ nodes;
     const GpuInstance* __restrict__ insts;  // the primitive array read as instances (absolute indices)
+    const WideNode* __restrict__ wide;      // 4-wide BVH (null: walk the binary tree)
+    uint32_t num_wide;
     const GpuLight* __restrict__ lights;
     const float* __restrict__ ggx;
     EnvView env;
@@ -278,7 +284,9 @@ __device__ inline bool tri_test(const float4* __restrict__ edges, uint32_t ti, V
     return true;
 }
 
-__device__ inline uint32_t pack_child(uint32_t first, uint32_t count) { return (first << 3) | count; }
+// A child carried as (first << 5) | count: count <= 31 (0: internal), first < 2^27.
+constexpr uint32_t COUNT_BITS = 5, COUNT_MASK = 31u;
+__device__ inline uint32_t pack_child(uint32_t first, uint32_t count) { return (first << COUNT_BITS) | count; }
 
 // A lane's traversal stack of STACK entries (bvh.wgsl:126-133). The newest
 // LDS_STACK entries live in an LDS ring (one column per lane of the block),
@@ -291,8 +299,9 @@ struct RayStack {
     uint32_t* spill;       // device memory, entry k at spill[k * spill_stride]
     uint32_t spill_stride;
     int sp, lo;            // entries [0, lo) spilled, [lo, sp) in the ring
+    int cap;               // STACK for the binary walks, WIDE_STACK for the 4-wide one
     __device__ inline bool push(uint32_t v) {
-        if (sp >= STACK) return false;
+        if (sp >= cap) return false;
         if (sp - lo == LDS_STACK) {
             spill[(size_t)lo * spill_stride] = ring[(lo % LDS_STACK) * TRACE_WG];
             ++lo;
@@ -311,9 +320,9 @@ struct RayStack {
     }
 };
 
-__device__ inline RayStack make_stack(uint32_t* lds_base, uint32_t* spill_base) {
+__device__ inline RayStack make_stack(uint32_t* lds_base, uint32_t* spill_base, int cap) {
     const uint32_t lanes = gridDim.x * TRACE_WG;
-    return RayStack{lds_base + threadIdx.x, spill_base + blockIdx.x * TRACE_WG + threadIdx.x, lanes, 0, 0};
+    return RayStack{lds_base + threadIdx.x, spill_base + blockIdx.x * TRACE_WG + threadIdx.x, lanes, 0, 0, cap};
 }
 
 // Closest hit (any_hit = false) or occlusion (any_hit = true) against the
@@ -347,7 +356,7 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
     const uint32_t cap = 2u * sc.num_nodes + 8u;
     for (uint32_t iter = 0; iter < cap; ++iter) {
         *visits += 1;
-        const uint32_t first = cur >> 3, count = cur & 7u;
+        const uint32_t first = cur >> COUNT_BITS, count = cur & COUNT_MASK;
         if (count > 0) {
             for (uint32_t k = 0; k < count; ++k) {
                 const uint32_t ti = first + k;
@@ -369,7 +378,7 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
             const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
             const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
             if (ha && hb) {
-                const bool a_near = ANY_HIT ? true : ea <= eb;
+                const bool a_near = ea <= eb;
                 if (!st.push(a_near ? cb : ca)) *overflow += 1;
                 cur = a_near ? ca : cb;
                 continue;
@@ -386,6 +395,95 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
     }
     st.sp = base;  // an any-hit return leaves entries behind
     if (st.lo > base) st.lo = base;
+    return found;
+}
+
+// The 4-wide walk (widebvh.hpp) over the same leaves: a visit tests the four
+// child boxes of one 128-byte node; hit leaves are intersected at once (so
+// `closest` tightens before the inner children are tested), hit inner
+// children are visited nearest first, the others pushed. Hits equal the
+// binary walk's except between triangles at exactly equal t.
+__device__ inline void cswap(float& ea, uint32_t& ra, float& eb, uint32_t& rb) {
+    if (eb < ea) {
+        const float e = ea;
+        ea = eb;
+        eb = e;
+        const uint32_t r = ra;
+        ra = rb;
+        rb = r;
+    }
+}
+
+template <bool ANY_HIT>
+__device__ inline bool traverse_wide(const SceneView& sc, RayStack& st, V3 o, V3 d, float t_min, float t_max,
+                                     float* t_out, float* u_out, float* v_out, uint32_t* tri_out, uint32_t* visits,
+                                     uint32_t* tests, uint32_t* overflow) {
+    bool found = false;
+    float closest = t_max;
+    const V3 inv = safe_inv_dir(d);
+    uint32_t cur = 0;
+    const uint32_t cap = 2u * sc.num_wide + 8u;
+    for (uint32_t iter = 0; iter < cap; ++iter) {
+        *visits += 1;
+        const float4* q = reinterpret_cast<const float4*>(sc.wide + cur);
+        const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
+        const uint4 ch = *reinterpret_cast<const uint4*>(q + 6);
+        const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, LY[4] = {ly.x, ly.y, ly.z, ly.w}, LZ[4] = {lz.x, lz.y, lz.z, lz.w};
+        const float HX[4] = {hx.x, hx.y, hx.z, hx.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+        const uint32_t CH[4] = {ch.x, ch.y, ch.z, ch.w};
+        // leaves first
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t c = CH[k];
+            if (c == WIDE_EMPTY || !(c & WIDE_LEAF)) continue;
+            float en;
+            if (!slab(make_float4(LX[k], LY[k], LZ[k], 0.0f), make_float4(HX[k], HY[k], HZ[k], 0.0f), o, inv, t_min,
+                      closest, &en))
+                continue;
+            const uint32_t first = (c & ~WIDE_LEAF) >> COUNT_BITS, count = c & COUNT_MASK;
+            for (uint32_t m = 0; m < count; ++m) {
+                const uint32_t ti = first + m;
+                *tests += 1;
+                float t, u, v;
+                if (tri_test(sc.edges, ti, o, d, t_min, closest, &t, &u, &v)) {
+                    if (ANY_HIT) return true;
+                    closest = t;
+                    *t_out = t; *u_out = u; *v_out = v; *tri_out = ti;
+                    found = true;
+                }
+            }
+        }
+        // inner children, tested against the tightened `closest`
+        float E[4];
+        uint32_t R[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t c = CH[k];
+            float en = INFINITY;
+            if (c != WIDE_EMPTY && !(c & WIDE_LEAF)) {
+                float e;
+                if (slab(make_float4(LX[k], LY[k], LZ[k], 0.0f), make_float4(HX[k], HY[k], HZ[k], 0.0f), o, inv,
+                         t_min, closest, &e))
+                    en = e;
+            }
+            E[k] = en;
+            R[k] = c;
+        }
+        cswap(E[0], R[0], E[1], R[1]);
+        cswap(E[2], R[2], E[3], R[3]);
+        cswap(E[0], R[0], E[2], R[2]);
+        cswap(E[1], R[1], E[3], R[3]);
+        cswap(E[1], R[1], E[2], R[2]);
+#pragma unroll
+        for (int k = 3; k >= 1; --k)
+            if (E[k] != INFINITY && !st.push(R[k])) *overflow += 1;
+        if (E[0] != INFINITY) {
+            cur = R[0];
+            continue;
+        }
+        if (st.sp == 0) break;
+        cur = st.pop();
+    }
     return found;
 }
 
@@ -434,7 +532,7 @@ __device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 
     const uint32_t cap = 2u * sc.num_nodes + 8u;
     for (uint32_t iter = 0; iter < cap; ++iter) {
         *visits += 1;
-        const uint32_t first = cur >> 3, count = cur & 7u;
+        const uint32_t first = cur >> COUNT_BITS, count = cur & COUNT_MASK;
         if (count > 0) {
             for (uint32_t k = 0; k < count; ++k) {
                 const uint32_t ii = first + k;
@@ -458,7 +556,7 @@ __device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 
             const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
             const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
             if (ha && hb) {
-                const bool a_near = ANY_HIT ? true : ea <= eb;
+                const bool a_near = ea <= eb;
                 if (!st.push(a_near ? cb : ca)) *overflow += 1;
                 cur = a_near ? ca : cb;
                 continue;
@@ -481,9 +579,9 @@ __device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 
 // then the BVH with t_max = the best analytic hit. Writes (t, u, v, id):
 // id = triangle, ANALYTIC | primitive, or MISS.
 // ---------------------------------------------------------------------------
-// INST = false compiles out the two-level path (single-level scenes keep
-// the traversal kernels' registers low).
-template <bool BATCH, bool INST>
+// MODE (TRAV_*) selects the walk at compile time, so each kernel carries
+// only its own traversal's registers.
+template <bool BATCH, int MODE>
 __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJob* __restrict__ jobs, Planes p,
                                                      const uint32_t* __restrict__ queue,
                                                      const uint32_t* __restrict__ count_ptr, uint32_t stats_on) {
@@ -492,7 +590,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJ
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (i < count) {
-      RayStack st = make_stack(stack_mem, p.spill);
+      RayStack st = make_stack(stack_mem, p.spill, MODE == TRAV_WIDE ? WIDE_STACK : STACK);
       const uint32_t pid = queue[i];
       const uint32_t jid = BATCH ? p.job[pid] : 0u;
       for (;;) {
@@ -517,11 +615,15 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJ
         uint32_t tri = 0, inst = 0;
         // Instanced scenes take the two-level path instead of the
         // single-level one (scene.wgsl:31-75): their triangles live in object space.
-        const bool two_level = INST && sc.num_instances;
-        const bool th = two_level ? traverse_instances<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &inst,
-                                                              &visits, &tests, &overflow)
-                                  : traverse<false>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
-                                                    &overflow);
+        const bool two_level = MODE == TRAV_INSTANCED && sc.num_instances;
+        bool th;
+        if (MODE == TRAV_WIDE)
+            th = traverse_wide<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests, &overflow);
+        else if (two_level)
+            th = traverse_instances<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &inst, &visits, &tests,
+                                           &overflow);
+        else
+            th = traverse<false>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests, &overflow);
         if (two_level) p.hin[i] = inst;
         float4 out;
         if (th) out = make_float4(t, u, v, __uint_as_float(tri));
@@ -779,7 +881,7 @@ __global__ __launch_bounds__(WG) void k_shade(SceneView sc0, const GpuUniforms* 
 // analytic primitives and the BVH; an unoccluded ray adds its contribution
 // to its path's radiance (one shadow ray per path per bounce: no races).
 // ---------------------------------------------------------------------------
-template <bool BATCH, bool INST>
+template <bool BATCH, int MODE>
 __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const BatchJob* __restrict__ jobs, Planes p,
                                                       const uint32_t* __restrict__ count_ptr, uint32_t stats_on) {
     __shared__ uint32_t stack_mem[LDS_STACK * TRACE_WG];
@@ -787,7 +889,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (i < count) {
-      RayStack st = make_stack(stack_mem, p.spill);
+      RayStack st = make_stack(stack_mem, p.spill, MODE == TRAV_WIDE ? WIDE_STACK : STACK);
       const float4 a = p.sro[i], b = p.srd[i];
       const uint32_t jid = BATCH ? p.job[__float_as_uint(b.w)] : 0u;
       for (;;) {
@@ -806,11 +908,13 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
         if (!blocked) {
             float t, u, v;
             uint32_t tri, inst;
-            blocked = (INST && sc.num_instances)
-                          ? traverse_instances<true>(sc, st, o, d, T_MIN, t_max, &t, &u, &v, &tri, &inst, &visits,
-                                                     &tests, &overflow)
-                          : traverse<true>(sc, st, 0, o, d, T_MIN, t_max, &t, &u, &v, &tri, &visits, &tests,
-                                           &overflow);
+            if (MODE == TRAV_WIDE)
+                blocked = traverse_wide<true>(sc, st, o, d, T_MIN, t_max, &t, &u, &v, &tri, &visits, &tests, &overflow);
+            else if (MODE == TRAV_INSTANCED && sc.num_instances)
+                blocked = traverse_instances<true>(sc, st, o, d, T_MIN, t_max, &t, &u, &v, &tri, &inst, &visits,
+                                                   &tests, &overflow);
+            else
+                blocked = traverse<true>(sc, st, 0, o, d, T_MIN, t_max, &t, &u, &v, &tri, &visits, &tests, &overflow);
         }
         if (!blocked) {
             const uint32_t pid = __float_as_uint(b.w);
@@ -835,11 +939,13 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* __restrict__ rays, uint32_t n,
                                                    uint32_t any_hit, float4* __restrict__ hits,
-                                                   uint32_t* __restrict__ insts, uint32_t* __restrict__ spill) {
+                                                   uint32_t* __restrict__ insts, uint32_t* __restrict__ spill,
+                                                   uint32_t wide) {
     __shared__ uint32_t stack_mem[LDS_STACK * TRACE_WG];
     const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
     if (i >= n) return;
-    RayStack st = make_stack(stack_mem, spill);
+    const bool use_wide = wide && sc.wide && !sc.num_instances;
+    RayStack st = make_stack(stack_mem, spill, use_wide ? WIDE_STACK : STACK);
     const float4 a = rays[2 * i], b = rays[2 * i + 1];
     const V3 o = f4xyz(a), d = f4xyz(b);
     float best_t = b.w;
@@ -864,13 +970,17 @@ __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* _
         if (any_hit)
             got = sc.num_instances ? traverse_instances<true>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &inst,
                                                               &visits, &tests, &overflow)
-                                   : traverse<true>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
-                                                    &overflow);
+                  : use_wide ? traverse_wide<true>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
+                                                   &overflow)
+                             : traverse<true>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
+                                              &overflow);
         else
             got = sc.num_instances ? traverse_instances<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &inst,
                                                                &visits, &tests, &overflow)
-                                   : traverse<false>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits,
-                                                     &tests, &overflow);
+                  : use_wide ? traverse_wide<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
+                                                    &overflow)
+                             : traverse<false>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
+                                               &overflow);
         if (got) out = any_hit ? make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u)) : make_float4(t, u, v, __uint_as_float(tri));
     }
     hits[i] = out;
@@ -1031,7 +1141,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
         LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * 3), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_queues.ensure(P * 4 * 3), LUCIDA_OOM);
         // spill columns for every lane of a traversal grid over P items
-        LUCIDA_HIP(ctx->wf_spill.ensure((P + TRACE_WG) * (STACK - LDS_STACK) * 4), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_spill.ensure((P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
         ctx->wf_capacity = P;
     }
     LUCIDA_HIP(ctx->wf_counters.ensure(4096), LUCIDA_OOM);
@@ -1064,6 +1174,31 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
 
 using namespace lucida;
 
+// EXTEND / CONNECT for the traversal variant `mode` (TRAV_*).
+template <bool BATCH>
+static void launch_extend(int mode, dim3 grid, hipStream_t s, const SceneView& sc, const BatchJob* jobs,
+                          const Planes& p, const uint32_t* queue, const uint32_t* count, uint32_t stats_on) {
+    if (mode == TRAV_WIDE)
+        hipLaunchKernelGGL((k_extend<BATCH, TRAV_WIDE>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, queue, count, stats_on);
+    else if (mode == TRAV_INSTANCED)
+        hipLaunchKernelGGL((k_extend<BATCH, TRAV_INSTANCED>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, queue, count,
+                           stats_on);
+    else
+        hipLaunchKernelGGL((k_extend<BATCH, TRAV_BINARY>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, queue, count,
+                           stats_on);
+}
+
+template <bool BATCH>
+static void launch_connect(int mode, dim3 grid, hipStream_t s, const SceneView& sc, const BatchJob* jobs,
+                           const Planes& p, const uint32_t* count, uint32_t stats_on) {
+    if (mode == TRAV_WIDE)
+        hipLaunchKernelGGL((k_connect<BATCH, TRAV_WIDE>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, stats_on);
+    else if (mode == TRAV_INSTANCED)
+        hipLaunchKernelGGL((k_connect<BATCH, TRAV_INSTANCED>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, stats_on);
+    else
+        hipLaunchKernelGGL((k_connect<BATCH, TRAV_BINARY>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, stats_on);
+}
+
 // The kernels' view of a resident scene.
 static SceneView make_view(lucida_ctx* ctx, const lucida_scene* scene) {
     SceneView sc;
@@ -1075,6 +1210,8 @@ static SceneView make_view(lucida_ctx* ctx, const lucida_scene* scene) {
     sc.edges = scene->tri_edges;
     sc.nodes = scene->bvh_nodes;
     sc.insts = reinterpret_cast<const GpuInstance*>(scene->primitives);
+    sc.wide = reinterpret_cast<const WideNode*>(scene->wide);
+    sc.num_wide = (uint32_t)scene->num_wide;
     sc.num_instances = (uint32_t)scene->num_instances;
     sc.tlas_root = scene->tlas_root;
     sc.lights = scene->lights;
@@ -1112,8 +1249,8 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     if (uniforms->width != film->width || uniforms->height != film->height)
         return set_error(LUCIDA_INVALID_ARG, "uniforms width/height do not match the film");
     if (uniforms->max_depth > MAX_BOUNCES) return set_error(LUCIDA_INVALID_ARG, "max_depth above 64");
-    if (scene->num_bvh_nodes >= (1u << 28) || scene->num_triangles >= (1u << 28))
-        return set_error(LUCIDA_INVALID_ARG, "scene too large: node and triangle indices are packed in 29 bits");
+    if (scene->num_bvh_nodes >= (1u << 26) || scene->num_triangles + scene->num_primitives + scene->num_instances >= (1u << 26))
+        return set_error(LUCIDA_INVALID_ARG, "scene too large: node and primitive indices are packed in 26 bits");
     if (stats) std::memset(stats, 0, sizeof *stats);
     if (samples == 0 || uniforms->max_depth == 0) return LUCIDA_OK;
     lucida_status st = ensure_ggx_table(ctx);
@@ -1138,6 +1275,9 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     if (st != LUCIDA_OK) return st;
 
     const SceneView sc = make_view(ctx, scene);
+    const int mode = scene->num_instances ? TRAV_INSTANCED
+                     : (scene->wide && cfg.bvh_width == 4) ? TRAV_WIDE
+                                                           : TRAV_BINARY;
 
     hipStream_t s = ctx->stream;
     const uint32_t depth = uniforms->max_depth;
@@ -1178,24 +1318,16 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
             uint32_t* qin = (b & 1) ? p.q1 : p.q0;
             uint32_t* qout = (b & 1) ? p.q0 : p.q1;
             mark(1, true);
-            if (sc.num_instances)
-                hipLaunchKernelGGL((k_extend<false, true>), dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc,
-                                   nullptr, p, qin, p.counters + b, stats_on);
-            else
-                hipLaunchKernelGGL((k_extend<false, false>), dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc,
-                                   nullptr, p, qin, p.counters + b, stats_on);
+            launch_extend<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s, sc, nullptr, p, qin, p.counters + b,
+                                 stats_on);
             mark(1, false);
             mark(2, true);
             hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, WG)), dim3(WG), 0, s, sc, dU, nullptr, p, qin,
                                p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
             mark(2, false);
             mark(3, true);
-            if (sc.num_instances)
-                hipLaunchKernelGGL((k_connect<false, true>), dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc,
-                                   nullptr, p, p.counters + SHADOW_AT + b, stats_on);
-            else
-                hipLaunchKernelGGL((k_connect<false, false>), dim3(grid_for(Pb, TRACE_WG)), dim3(TRACE_WG), 0, s, sc,
-                                   nullptr, p, p.counters + SHADOW_AT + b, stats_on);
+            launch_connect<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s, sc, nullptr, p, p.counters + SHADOW_AT + b,
+                                  stats_on);
             mark(3, false);
         }
         mark(4, true);
@@ -1232,8 +1364,9 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
 }
 
 
-lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, const float* rays, size_t n, int any_hit,
+lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, const float* rays, size_t n, int flags,
                                 float* hits, uint32_t* instances) {
+    const int any_hit = flags & 1;
     if (!ctx || !scene || (n && (!rays || !hits))) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_cast: bad argument");
     if (n == 0) return LUCIDA_OK;
     if (n > (1u << 26)) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_cast: more than 2^26 rays");
@@ -1243,14 +1376,14 @@ lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, cons
     LUCIDA_HIP(d_rays.ensure(n * 32), LUCIDA_OOM);
     LUCIDA_HIP(d_hits.ensure(n * 16), LUCIDA_OOM);
     LUCIDA_HIP(d_inst.ensure(n * 4), LUCIDA_OOM);
-    LUCIDA_HIP(d_spill.ensure((n + TRACE_WG) * (STACK - LDS_STACK) * 4), LUCIDA_OOM);
+    LUCIDA_HIP(d_spill.ensure((n + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
     lucida_status st = LUCIDA_OK;
     if (hipMemcpyAsync(d_rays.ptr, rays, n * 32, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         st = set_error(LUCIDA_VALIDATION, "lucida_scene_cast: upload failed");
     if (st == LUCIDA_OK) {
         hipLaunchKernelGGL(k_cast, dim3(grid_for(n, TRACE_WG)), dim3(TRACE_WG), 0, ctx->stream, sc,
                            d_rays.as<float4>(), (uint32_t)n, any_hit ? 1u : 0u, d_hits.as<float4>(),
-                           d_inst.as<uint32_t>(), d_spill.as<uint32_t>());
+                           d_inst.as<uint32_t>(), d_spill.as<uint32_t>(), (flags & 2) ? 1u : 0u);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(hits, d_hits.ptr, n * 16, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
             (instances &&
@@ -1282,8 +1415,9 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
         if (J.uniforms.width != J.film->width || J.uniforms.height != J.film->height)
             return set_error(LUCIDA_INVALID_ARG, "lucida_render_batch: a job's uniforms do not match its film");
         if (J.uniforms.max_depth > MAX_BOUNCES) return set_error(LUCIDA_INVALID_ARG, "max_depth above 64");
-        if (J.scene->num_bvh_nodes >= (1u << 28) || J.scene->num_triangles >= (1u << 28))
-            return set_error(LUCIDA_INVALID_ARG, "scene too large: indices are packed in 29 bits");
+        if (J.scene->num_bvh_nodes >= (1u << 26) ||
+            J.scene->num_triangles + J.scene->num_primitives + J.scene->num_instances >= (1u << 26))
+            return set_error(LUCIDA_INVALID_ARG, "scene too large: node and primitive indices are packed in 26 bits");
         for (size_t k = 0; k < j; ++k)
             if (jobs[k].film == J.film) return set_error(LUCIDA_INVALID_ARG, "lucida_render_batch: two jobs share a film");
     }
@@ -1311,11 +1445,13 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     std::vector<Round> rounds;
     std::vector<BatchJob> table;
     std::vector<SceneView> views(n);
-    bool any_instanced = false;
+    bool any_instanced = false, all_wide = true;
     for (size_t j = 0; j < n; ++j) {
         views[j] = make_view(ctx, jobs[j].scene);
         any_instanced = any_instanced || jobs[j].scene->num_instances > 0;
+        all_wide = all_wide && jobs[j].scene->wide != nullptr;
     }
+    const int mode = any_instanced ? TRAV_INSTANCED : (all_wide && cfg.bvh_width == 4) ? TRAV_WIDE : TRAV_BINARY;
     for (;;) {
         Round r{table.size(), 0, 0, 0, 0};
         for (size_t j = 0; j < n; ++j) {
@@ -1378,20 +1514,11 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
             for (uint32_t b = 0; b < r.depth; ++b) {
                 uint32_t* qin = (b & 1) ? p.q1 : p.q0;
                 uint32_t* qout = (b & 1) ? p.q0 : p.q1;
-                if (any_instanced)
-                    hipLaunchKernelGGL((k_extend<true, true>), dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s, none,
-                                       jr, p, qin, p.counters + b, stats_on);
-                else
-                    hipLaunchKernelGGL((k_extend<true, false>), dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s,
-                                       none, jr, p, qin, p.counters + b, stats_on);
+                launch_extend<true>(mode, dim3(grid_for(P, TRACE_WG)), s, none, jr, p, qin, p.counters + b, stats_on);
                 hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, WG)), dim3(WG), 0, s, none, nullptr, jr, p, qin,
                                    p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
-                if (any_instanced)
-                    hipLaunchKernelGGL((k_connect<true, true>), dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s,
-                                       none, jr, p, p.counters + SHADOW_AT + b, stats_on);
-                else
-                    hipLaunchKernelGGL((k_connect<true, false>), dim3(grid_for(P, TRACE_WG)), dim3(TRACE_WG), 0, s,
-                                       none, jr, p, p.counters + SHADOW_AT + b, stats_on);
+                launch_connect<true>(mode, dim3(grid_for(P, TRACE_WG)), s, none, jr, p, p.counters + SHADOW_AT + b,
+                                     stats_on);
             }
             hipLaunchKernelGGL(k_resolve_batch, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, jr, nj,
                                (uint32_t)r.pixels, p);

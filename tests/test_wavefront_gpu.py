@@ -124,12 +124,17 @@ def test_config2_geometry_matches_oracle(gpu, oracle):
     """The 1.05M-triangle benchmark scene, rendered small (the bench runs it at 1024^2 x 16)."""
     blob, scene = scene_for(gpu, oracle, "sphere-c2")
     u = blob.uniforms(128, 128, samples=8, max_depth=8)
-    say("case: sphere-c2 (1.05M tris) 128x128 8spp")
-    cfg = L.WavefrontConfig(0, 0, 0, 0, 1)
+    say("case: sphere-c2 (1.05M tris) 128x128 8spp, binary and 4-wide walks")
+    cfg = L.WavefrontConfig(0, 0, 2, 0, 1)  # the binary walk: the oracle repeats its node-visit counts
     acc, st = gpu_render(gpu, scene, u, 8, cfg)
     assert st.stack_overflows == 0
     cpu, _, counts = oracle.render(blob.desc(), u, guides=False)
     check_agreement(oracle, means(acc, u)[0], cpu, "sphere-c2")
+    wide, wst = gpu_render(gpu, scene, u, 8, L.WavefrontConfig(0, 0, 4, 0, 1))
+    assert wst.stack_overflows == 0
+    check_agreement(oracle, means(wide, u)[0], cpu, "sphere-c2 (4-wide)")
+    say(f"node visits per extension ray: binary {st.extend_node_visits / st.extension_rays:.2f}, "
+        f"4-wide {wst.extend_node_visits / wst.extension_rays:.2f}")
     # the same rays were traced: counts agree to the handful of paths that
     # flip at silhouettes
     assert abs(st.extension_rays - counts.extension_rays) <= 2e-3 * counts.extension_rays
