@@ -479,11 +479,13 @@ typedef struct lucida_job {
 /* Many renders of uneven resolution, samples and scene in one wavefront.
  * The reference renders one request at a time (wavefront.rs:163-212 batches
  * only the samples of one render); this entry point has no counterpart.
- * Rounds gather one chunk of samples from as many jobs as fit in
- * config->max_paths paths; all of a round's paths share one launch of each
- * stage, every kernel finding a path's job (scene, uniforms, film) in a
- * device table. Each job is cut into the chunks its single render would use,
- * so its film is bit-identical to a lucida_render_wavefront of the same job.
+ * Each job is cut into the chunks its single render would use, so its film
+ * is bit-identical to a lucida_render_wavefront of the same job. Jobs whose
+ * chunks hold fewer than 2^21 paths go first, in mixed rounds that gather one
+ * chunk from as many of them as fit in config->max_paths paths: all of a
+ * round's paths share one launch of each stage, every kernel finding a path's
+ * job (scene, uniforms, film) in a device table. The larger jobs follow,
+ * shortest first, one chunk per round with the single render's kernels.
  * With config->use_graph the whole batch is captured once and replayed as one
  * hipGraph launch. job_done_ms (optional, n floats) receives each job's
  * completion time from the start of the batch (device clock). stats as for a

@@ -1441,45 +1441,77 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     struct Round {
         size_t first, count, paths, pixels;
         uint32_t depth;
+        int solo_mode;  // -1: mixed round (batch kernels); else the traversal mode of its one chunk
     };
     std::vector<Round> rounds;
     std::vector<BatchJob> table;
     std::vector<SceneView> views(n);
+    std::vector<int> solo_mode(n);
     bool any_instanced = false, all_wide = true;
     for (size_t j = 0; j < n; ++j) {
         views[j] = make_view(ctx, jobs[j].scene);
         any_instanced = any_instanced || jobs[j].scene->num_instances > 0;
         all_wide = all_wide && jobs[j].scene->wide != nullptr;
+        solo_mode[j] = jobs[j].scene->num_instances                        ? TRAV_INSTANCED
+                       : (jobs[j].scene->wide && cfg.bvh_width == 4) ? TRAV_WIDE
+                                                                         : TRAV_BINARY;
     }
     const int mode = any_instanced ? TRAV_INSTANCED : (all_wide && cfg.bvh_width == 4) ? TRAV_WIDE : TRAV_BINARY;
+    // A job whose chunks hold kSoloPaths paths or more fills the device by
+    // itself: its chunks run as rounds of their own with the single-render
+    // kernels (no per-path job lookup). The other jobs' chunks are packed
+    // into mixed rounds, one chunk per job per round, and go first, so small
+    // requests finish early; the large jobs follow, shortest first.
+    const size_t kSoloPaths = (size_t)1 << 21;
+    std::vector<char> solo(n, 0);
+    for (size_t j = 0; j < n; ++j) solo[j] = chunks[j] > 0 && pixels[j] * spb[j] >= kSoloPaths;
+    auto add_chunk = [&](size_t j, Round& r) {
+        const uint32_t done = next[j] * spb[j];
+        const uint32_t this_spb = std::min(spb[j], jobs[j].samples - done);
+        const size_t paths = pixels[j] * this_spb;
+        BatchJob b{};
+        b.sc = views[j];
+        b.u = jobs[j].uniforms;
+        b.u.sample_offset = jobs[j].uniforms.sample_offset + done;
+        b.u.samples_per_launch = this_spb;
+        b.film = jobs[j].film->accum;
+        b.path_base = (uint32_t)r.paths;
+        b.pixel_base = (uint32_t)r.pixels;
+        b.pixels = (uint32_t)pixels[j];
+        table.push_back(b);
+        r.paths += paths;
+        r.pixels += pixels[j];
+        r.depth = std::max(r.depth, jobs[j].uniforms.max_depth);
+        ++r.count;
+        ++next[j];
+        if (next[j] == chunks[j]) last_round[j] = (uint32_t)rounds.size();
+    };
     for (;;) {
-        Round r{table.size(), 0, 0, 0, 0};
+        Round r{table.size(), 0, 0, 0, 0, -1};
         for (size_t j = 0; j < n; ++j) {
-            if (next[j] >= chunks[j]) continue;
-            const uint32_t done = next[j] * spb[j];
-            const uint32_t this_spb = std::min(spb[j], jobs[j].samples - done);
-            const size_t paths = pixels[j] * this_spb;
-            if (r.count > 0 && r.paths + paths > cap) continue;
-            BatchJob b{};
-            b.sc = views[j];
-            b.u = jobs[j].uniforms;
-            b.u.sample_offset = jobs[j].uniforms.sample_offset + done;
-            b.u.samples_per_launch = this_spb;
-            b.film = jobs[j].film->accum;
-            b.path_base = (uint32_t)r.paths;
-            b.pixel_base = (uint32_t)r.pixels;
-            b.pixels = (uint32_t)pixels[j];
-            table.push_back(b);
-            r.paths += paths;
-            r.pixels += pixels[j];
-            r.depth = std::max(r.depth, jobs[j].uniforms.max_depth);
-            ++r.count;
-            ++next[j];
-            if (next[j] == chunks[j]) last_round[j] = (uint32_t)rounds.size();
+            if (solo[j] || next[j] >= chunks[j]) continue;
+            const uint32_t this_spb = std::min(spb[j], jobs[j].samples - next[j] * spb[j]);
+            if (r.count > 0 && r.paths + pixels[j] * this_spb > cap) continue;
+            add_chunk(j, r);
         }
         if (r.count == 0) break;
         if (r.paths >= 0x7FFFFFFFull / 4) return set_error(LUCIDA_INVALID_ARG, "a job's chunk exceeds 2^29 paths");
         rounds.push_back(r);
+    }
+    std::vector<size_t> order;
+    for (size_t j = 0; j < n; ++j)
+        if (solo[j]) order.push_back(j);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        return pixels[a] * jobs[a].samples * jobs[a].uniforms.max_depth <
+               pixels[b] * jobs[b].samples * jobs[b].uniforms.max_depth;
+    });
+    for (size_t j : order) {
+        while (next[j] < chunks[j]) {
+            Round r{table.size(), 0, 0, 0, 0, solo_mode[j]};
+            add_chunk(j, r);
+            if (r.paths >= 0x7FFFFFFFull / 4) return set_error(LUCIDA_INVALID_ARG, "a job's chunk exceeds 2^29 paths");
+            rounds.push_back(r);
+        }
     }
     if (rounds.empty()) return LUCIDA_OK;
     size_t max_paths = 0;
@@ -1510,6 +1542,29 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
             const BatchJob* jr = dt + r.first;
             const uint32_t nj = (uint32_t)r.count, P = (uint32_t)r.paths;
             hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT);
+            if (r.solo_mode >= 0) {
+                // One chunk: exactly the launches of a single render's batch,
+                // with the chunk's uniforms read from its table entry.
+                const GpuUniforms* uj =
+                    reinterpret_cast<const GpuUniforms*>(reinterpret_cast<const char*>(jr) + offsetof(BatchJob, u));
+                const SceneView& sc = table[r.first].sc;
+                hipLaunchKernelGGL(k_generate, dim3(grid_for(P, 256)), dim3(256), 0, s, uj, p);
+                for (uint32_t b = 0; b < r.depth; ++b) {
+                    uint32_t* qin = (b & 1) ? p.q1 : p.q0;
+                    uint32_t* qout = (b & 1) ? p.q0 : p.q1;
+                    launch_extend<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p, qin,
+                                         p.counters + b, stats_on);
+                    hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, WG)), dim3(WG), 0, s, sc, uj, nullptr, p, qin,
+                                       p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
+                    launch_connect<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p,
+                                          p.counters + SHADOW_AT + b, stats_on);
+                }
+                hipLaunchKernelGGL(k_resolve, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, uj, p,
+                                   table[r.first].film);
+                hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
+                hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, (uint32_t)(ri + 1));
+                continue;
+            }
             hipLaunchKernelGGL(k_generate_batch, dim3(grid_for(P, 256)), dim3(256), 0, s, jr, nj, P, p);
             for (uint32_t b = 0; b < r.depth; ++b) {
                 uint32_t* qin = (b & 1) ? p.q1 : p.q0;

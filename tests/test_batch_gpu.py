@@ -39,6 +39,10 @@ JOBS = [
     ("instances-small", 64, 48, 5, 12, 0, 99),
     ("box-c1", 32, 32, 16, 3, 2, 5),
     ("sphere-c2-f24", 48, 40, 2, 8, 0, 0x5EED1234),
+    # chunks of 2^21 paths or more: rounds of their own, single-render kernels
+    ("box-c1", 1024, 1024, 3, 4, 0, 7),
+    ("sphere-c2-f24", 1024, 1024, 6, 8, 1, 3),  # two chunks: 4 + 2 spp
+    ("instances-small", 1024, 768, 3, 6, 0, 21),
 ]
 
 
@@ -74,6 +78,8 @@ def test_batch_equals_solo_renders(gpu, scenes, use_graph, max_paths):
     assert np.all(done > 0) and st.elapsed_ms > 0
     assert st.kernel_launches[0] >= (1 if max_paths == 0 else 3)  # rounds
     assert st.stack_overflows == 0
+    # the large jobs finish after every small one, shortest first
+    assert done[:6].max() <= done[6] <= done[8] <= done[7]
     say(f"rounds {st.kernel_launches[0]}, batch {st.elapsed_ms:.2f} ms, done {np.round(done, 2)}")
 
 
