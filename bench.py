@@ -308,7 +308,10 @@ def main() -> int:
     film = L.Film(gpu, a.width, a.height)
     u = blob.uniforms(a.width, a.height, samples=a.spp, max_depth=a.depth, sampling=L.SAMPLING_MIS,
                       sampler=L.SAMPLER_SOBOL)
-    cfg = L.WavefrontConfig(a.samples_per_batch, 0, a.bvh_width, 0, 1, 1)
+    # Timed renders run as a caller would (no traversal counters; per-kernel
+    # HIP events only); node visits and triangle tests come from one more,
+    # untimed, counted render of the same (deterministic) step.
+    cfg = L.WavefrontConfig(a.samples_per_batch, 0, a.bvh_width, 0, 0, 1)
 
     for _ in range(a.warmup):
         film.clear()
@@ -317,7 +320,6 @@ def main() -> int:
 
     agg = {k: 0.0 for k in L.RenderStats.KERNELS}
     launches = {k: 0 for k in L.RenderStats.KERNELS}
-    ext_rays = shadow_rays = ext_visits = ext_tests = visits = tests = 0
     device_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -326,18 +328,21 @@ def main() -> int:
         for i, k in enumerate(L.RenderStats.KERNELS):
             agg[k] += st.kernel_ms[i]
             launches[k] += st.kernel_launches[i]
-        ext_rays += st.extension_rays
-        shadow_rays += st.shadow_rays
-        ext_visits += st.extend_node_visits
-        ext_tests += st.extend_triangle_tests
-        visits += st.node_visits
-        tests += st.triangle_tests
         device_ms += st.elapsed_ms
         if st.stack_overflows:
             print(f"warning: {st.stack_overflows} traversal stack overflows", file=sys.stderr)
     gpu.check_errors()
     t1 = time.perf_counter()
     sec = (t1 - t0) / a.steps
+    film.clear()
+    cst = L.render_wavefront(gpu, scene, u, a.spp, film,
+                             L.WavefrontConfig(a.samples_per_batch, 0, a.bvh_width, 0, 1, 0))
+    gpu.check_errors()
+    n = a.steps  # every step traces the same rays
+    ext_rays, shadow_rays = cst.extension_rays * n, cst.shadow_rays * n
+    ext_visits, ext_tests = cst.extend_node_visits * n, cst.extend_triangle_tests * n
+    visits, tests = cst.node_visits * n, cst.triangle_tests * n
+    ext_slots, con_slots = cst.extend_lane_slots * n, cst.connect_lane_slots * n
     paths = a.width * a.height * a.spp
     value = paths / sec / 1e6
 
@@ -383,6 +388,8 @@ def main() -> int:
             "mrays_per_s": round((ext_rays + shadow_rays) / a.steps / sec / 1e6, 2),
             "extension_rays_per_step": ext_rays // a.steps, "shadow_rays_per_step": shadow_rays // a.steps,
             "bvh_walk": "4-wide" if a.bvh_width == 4 else "binary",
+            "walk_simd_efficiency": {"extend": round(ext_visits / max(ext_slots, 1), 3),
+                                     "connect": round((visits - ext_visits) / max(con_slots, 1), 3)},
             "node_visits_per_ray": round(visits / max(ext_rays + shadow_rays, 1), 2),
             "triangle_tests_per_ray": round(tests / max(ext_rays + shadow_rays, 1), 2),
             "kernel_ms_per_step": {k: round(v / a.steps, 3) for k, v in agg.items()},

@@ -446,6 +446,11 @@ typedef struct lucida_render_stats {
      * set, in the order GENERATE, EXTEND, SHADE, CONNECT, RESOLVE. */
     float kernel_ms[5];
     uint32_t kernel_launches[5];
+    /* With count_traversal: 64 x the visit steps each wave executed, for
+     * closest-hit and any-hit walks; visits / lane slots is the fraction of
+     * lanes a walk kept busy (its SIMD efficiency). */
+    uint64_t extend_lane_slots;
+    uint64_t connect_lane_slots;
 } lucida_render_stats;
 
 /* Tunables of the wavefront. Zero fields take the defaults. */

@@ -84,7 +84,8 @@ class RenderStats(C.Structure):
     _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64),
                 ("triangle_tests", C.c_uint64), ("extend_node_visits", C.c_uint64),
                 ("extend_triangle_tests", C.c_uint64), ("stack_overflows", C.c_uint64), ("elapsed_ms", C.c_float),
-                ("kernel_ms", C.c_float * 5), ("kernel_launches", C.c_uint32 * 5)]
+                ("kernel_ms", C.c_float * 5), ("kernel_launches", C.c_uint32 * 5),
+                ("extend_lane_slots", C.c_uint64), ("connect_lane_slots", C.c_uint64)]
 
     KERNELS = ("generate", "extend", "shade", "connect", "resolve")
 

@@ -48,11 +48,16 @@ constexpr int TRAV_BINARY = 0, TRAV_WIDE = 1, TRAV_INSTANCED = 2;
 constexpr int MAX_BOUNCES = 64;
 // Counter words: queue length per bounce at [0, 72) (bounce b+1's count is
 // written by bounce b's SHADE), shadow-ray count per bounce at [SHADOW_AT,
-// +64), this batch's traversal counters (u64) at STATS_AT, the render's
-// running totals (u64) at TOTALS_AT.
+// +64), this batch's traversal counters (u64) at STATS_AT in STATS_STRIPES
+// copies of 8 (a block adds to copy blockIdx % STATS_STRIPES, so the waves
+// of a launch do not queue on a few addresses), the render's running totals
+// (10 u64) at TOTALS_AT.
 constexpr uint32_t SHADOW_AT = 80;
 constexpr uint32_t STATS_AT = 160;
-constexpr uint32_t TOTALS_AT = 256;
+constexpr uint32_t STATS_STRIPES = 32;
+constexpr uint32_t TOTALS_AT = STATS_AT + STATS_STRIPES * 16;
+constexpr size_t COUNTER_BYTES = 4096;
+static_assert((TOTALS_AT + 24) * 4 <= COUNTER_BYTES, "counter words exceed their buffer");
 
 // Per-render device state. Planes are float4 arrays of P entries.
 struct Planes {
@@ -133,20 +138,29 @@ __device__ inline uint64_t wave_sum_u32(uint32_t v) {
     return s;
 }
 
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
 // Per-wave totals of the traversal counters, one atomic each. Overflows are
 // always counted; visits and tests only when asked for.
 // Per-batch counters live at word STATS_AT as u64: closest-hit visits and
-// tests, any-hit visits and tests, stack overflows. `slot` is 0 for EXTEND and
-// 2 for CONNECT.
+// tests, any-hit visits and tests, stack overflows, then the lane slots of
+// closest-hit and any-hit walks (64 x the wave's longest walk, the visit
+// steps it executed, so visits / slots is the walk's SIMD efficiency).
+// `slot` is 0 for EXTEND and 2 for CONNECT.
 __device__ inline void flush_traversal_stats(uint32_t* counters, uint32_t visits, uint32_t tests, uint32_t overflow,
                                              uint32_t stats_on, int slot) {
-    uint64_t* s64 = reinterpret_cast<uint64_t*>(counters + STATS_AT);
+    uint64_t* s64 = reinterpret_cast<uint64_t*>(counters + STATS_AT + (blockIdx.x % STATS_STRIPES) * 16);
     const uint64_t so = wave_sum_u32(overflow);
     if (stats_on) {
         const uint64_t sv = wave_sum_u32(visits), st = wave_sum_u32(tests);
+        const uint64_t steps = wave_max_u32(visits);
         if ((threadIdx.x & 63) == 0) {
             if (sv) atomicAdd((unsigned long long*)&s64[slot], (unsigned long long)sv);
             if (st) atomicAdd((unsigned long long*)&s64[slot + 1], (unsigned long long)st);
+            if (steps) atomicAdd((unsigned long long*)&s64[5 + slot / 2], (unsigned long long)(64 * steps));
         }
     }
     if ((threadIdx.x & 63) == 0 && so) atomicAdd((unsigned long long*)&s64[4], (unsigned long long)so);
@@ -232,7 +246,7 @@ __global__ __launch_bounds__(256) void k_generate_batch(const BatchJob* __restri
 // ---------------------------------------------------------------------------
 // Traversal (bvh.rs:1045 / bvh.wgsl:137): near-first, children tested as a
 // pair, the far child pushed. A stack entry packs a child's (first, count) so
-// a popped node is never re-read: (first << 3) | count, count <= 4 (leaf) or
+// a popped node is never re-read: (first << 5) | count, count <= 31 (leaf) or
 // 0 (internal). An overflowing push is dropped, as the reference GPU does,
 // and counted.
 // ---------------------------------------------------------------------------
@@ -329,13 +343,91 @@ __device__ inline RayStack make_stack(uint32_t* lds_base, uint32_t* spill_base, 
 // triangles of the tree rooted at `root` (intersect_bvh_from, bvh.wgsl:137:
 // the root's own box is not tested). Returns the hit flag; closest-hit fills
 // t/u/v/tri. Entries the walk pushes sit above the stack's current top.
+// One ray's walk, advanced one node visit at a time (walk_step).
+struct BinWalk {
+    V3 o, d, inv;
+    float t_min, closest;
+    float t, u, v;
+    uint32_t tri;
+    uint32_t cur;   // packed child visited next
+    uint32_t iter;  // visits so far (bounded by the node count)
+    bool found;
+};
+
+// Starts a walk at the root's children; `root` is the root node's packed
+// (first, count).
+__device__ inline void walk_begin(BinWalk& w, uint32_t root, V3 o, V3 d, float t_min, float t_max) {
+    w.o = o;
+    w.d = d;
+    w.inv = safe_inv_dir(d);
+    w.t_min = t_min;
+    w.closest = t_max;
+    w.t = w.u = w.v = 0.0f;
+    w.tri = 0;
+    w.iter = 0;
+    w.found = false;
+    w.cur = root;
+}
+
+__device__ inline uint32_t packed_root(const SceneView& sc, uint32_t root_index) {
+    const GpuBvhNode root = sc.nodes[root_index];
+    return pack_child(root.left_first, root.count);
+}
+
+// Visits w.cur; returns true when the walk is over (stack back at `base`,
+// or, for any-hit, the first hit). A tree visits each node at most once per
+// ray; the cap only guarantees termination on a malformed (cyclic) array.
+template <bool ANY_HIT>
+__device__ __forceinline__ bool walk_step(const SceneView& sc, RayStack& st, BinWalk& w, int base, uint32_t* visits,
+                                          uint32_t* tests, uint32_t* overflow) {
+    *visits += 1;
+    const uint32_t first = w.cur >> COUNT_BITS, count = w.cur & COUNT_MASK;
+    if (count > 0) {
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t ti = first + k;
+            *tests += 1;
+            float t, u, v;
+            if (tri_test(sc.edges, ti, w.o, w.d, w.t_min, w.closest, &t, &u, &v)) {
+                w.found = true;
+                if (ANY_HIT) return true;
+                w.closest = t;
+                w.t = t; w.u = u; w.v = v; w.tri = ti;
+            }
+        }
+    } else {
+        const float4* cn = reinterpret_cast<const float4*>(sc.nodes + first);
+        const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
+        float ea, eb;
+        const bool ha = slab(a0, a1, w.o, w.inv, w.t_min, w.closest, &ea);
+        const bool hb = slab(b0, b1, w.o, w.inv, w.t_min, w.closest, &eb);
+        const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
+        const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
+        if (ha && hb) {
+            const bool a_near = ea <= eb;
+            if (!st.push(a_near ? cb : ca)) *overflow += 1;
+            w.cur = a_near ? ca : cb;
+            return ++w.iter >= 2u * sc.num_nodes + 8u;
+        } else if (ha || hb) {
+            w.cur = ha ? ca : cb;
+            return ++w.iter >= 2u * sc.num_nodes + 8u;
+        }
+    }
+    if (st.sp == base) return true;
+    w.cur = st.pop();
+    return ++w.iter >= 2u * sc.num_nodes + 8u;
+}
+
+// Closest hit (any_hit = false) or occlusion (any_hit = true) against the
+// triangles of the tree rooted at `root` (intersect_bvh_from, bvh.wgsl:137:
+// the root's own box is not tested). Returns the hit flag; closest-hit fills
+// t/u/v/tri. Entries the walk pushes sit above the stack's current top.
 template <bool ANY_HIT>
 __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root_index, V3 o, V3 d, float t_min,
                                 float t_max, float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
                                 uint32_t* visits, uint32_t* tests, uint32_t* overflow) {
-    bool found = false;
-    float closest = t_max;
     if (sc.num_nodes == 0) {
+        bool found = false;
+        float closest = t_max;
         for (uint32_t i = 0; i < sc.num_tris; ++i) {
             float t, u, v;
             if (tri_test(sc.edges, i, o, d, t_min, closest, &t, &u, &v)) {
@@ -347,55 +439,17 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
         }
         return found;
     }
-    const V3 inv = safe_inv_dir(d);
-    const GpuBvhNode root = sc.nodes[root_index];
-    uint32_t cur = pack_child(root.left_first, root.count);
+    BinWalk w;
+    walk_begin(w, packed_root(sc, root_index), o, d, t_min, t_max);
     const int base = st.sp;
-    // A tree visits each node at most once per ray; the cap only guarantees
-    // termination on a malformed (cyclic) node array.
-    const uint32_t cap = 2u * sc.num_nodes + 8u;
-    for (uint32_t iter = 0; iter < cap; ++iter) {
-        *visits += 1;
-        const uint32_t first = cur >> COUNT_BITS, count = cur & COUNT_MASK;
-        if (count > 0) {
-            for (uint32_t k = 0; k < count; ++k) {
-                const uint32_t ti = first + k;
-                *tests += 1;
-                float t, u, v;
-                if (tri_test(sc.edges, ti, o, d, t_min, closest, &t, &u, &v)) {
-                    if (ANY_HIT) return true;
-                    closest = t;
-                    *t_out = t; *u_out = u; *v_out = v; *tri_out = ti;
-                    found = true;
-                }
-            }
-        } else {
-            const float4* cn = reinterpret_cast<const float4*>(sc.nodes + first);
-            const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
-            float ea, eb;
-            const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
-            const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
-            const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
-            const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
-            if (ha && hb) {
-                const bool a_near = ea <= eb;
-                if (!st.push(a_near ? cb : ca)) *overflow += 1;
-                cur = a_near ? ca : cb;
-                continue;
-            } else if (ha) {
-                cur = ca;
-                continue;
-            } else if (hb) {
-                cur = cb;
-                continue;
-            }
-        }
-        if (st.sp == base) break;
-        cur = st.pop();
+    while (!walk_step<ANY_HIT>(sc, st, w, base, visits, tests, overflow)) {
     }
     st.sp = base;  // an any-hit return leaves entries behind
     if (st.lo > base) st.lo = base;
-    return found;
+    if (!ANY_HIT && w.found) {
+        *t_out = w.t; *u_out = w.u; *v_out = w.v; *tri_out = w.tri;
+    }
+    return w.found;
 }
 
 // The 4-wide walk (widebvh.hpp) over the same leaves: a visit tests the four
@@ -1051,7 +1105,11 @@ __global__ void k_fold_counters(uint32_t* counters, uint32_t depth) {
     }
     tot[0] += ext;
     tot[1] += sh;
-    for (int k = 0; k < 5; ++k) tot[2 + k] += batch[k];
+    for (int k = 0; k < 7; ++k) {
+        uint64_t v = 0;
+        for (uint32_t c = 0; c < STATS_STRIPES; ++c) v += batch[8 * c + k];
+        tot[2 + k] += v;
+    }
 }
 
 // Clears the per-bounce counters with a kernel (not a memset) so the batch can
@@ -1144,7 +1202,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
         LUCIDA_HIP(ctx->wf_spill.ensure((P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
         ctx->wf_capacity = P;
     }
-    LUCIDA_HIP(ctx->wf_counters.ensure(4096), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->wf_counters.ensure(COUNTER_BYTES), LUCIDA_OOM);
     LUCIDA_HIP(ctx->wf_uniforms.ensure(sizeof(GpuUniforms) * 2), LUCIDA_OOM);
     float4* pl = ctx->wf_planes.as<float4>();
     const size_t C = ctx->wf_capacity;
@@ -1198,6 +1256,7 @@ static void launch_connect(int mode, dim3 grid, hipStream_t s, const SceneView& 
     else
         hipLaunchKernelGGL((k_connect<BATCH, TRAV_BINARY>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, stats_on);
 }
+
 
 // The kernels' view of a resident scene.
 static SceneView make_view(lucida_ctx* ctx, const lucida_scene* scene) {
@@ -1302,7 +1361,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     };
 
     LUCIDA_HIP(hipEventRecord(ctx->ev0, s), LUCIDA_VALIDATION);
-    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT + 16u);
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT + 24u);
     for (uint32_t done = 0; done < samples; done += spb) {
         const uint32_t this_spb = std::min(spb, samples - done);
         GpuUniforms u = *uniforms;
@@ -1340,7 +1399,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     LUCIDA_HIP(hipEventSynchronize(ctx->ev1), LUCIDA_VALIDATION);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
     if (stats) {
-        uint64_t totals[8];
+        uint64_t totals[10];
         LUCIDA_HIP(hipMemcpy(totals, p.counters + TOTALS_AT, sizeof totals, hipMemcpyDeviceToHost), LUCIDA_READBACK);
         float ms = 0.0f;
         LUCIDA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), LUCIDA_VALIDATION);
@@ -1351,6 +1410,8 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         stats->node_visits = totals[2] + totals[4];
         stats->triangle_tests = totals[3] + totals[5];
         stats->stack_overflows = totals[6];
+        stats->extend_lane_slots = totals[7];
+        stats->connect_lane_slots = totals[8];
         stats->elapsed_ms = ms;
         for (size_t k = 0; k < ev_kind.size(); ++k) {
             float t = 0.0f;
@@ -1535,7 +1596,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     const SceneView none{};
 
     auto enqueue_all = [&]() {
-        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT + 16u);
+        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT + 24u);
         hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, 0u);
         for (size_t ri = 0; ri < rounds.size(); ++ri) {
             const Round& r = rounds[ri];
@@ -1614,7 +1675,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     if (st == LUCIDA_OK) LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
     if (st == LUCIDA_OK && (stats || job_done_ms)) {
         std::vector<unsigned long long> h_stamps(rounds.size() + 1);
-        uint64_t totals[8];
+        uint64_t totals[10];
         LUCIDA_HIP(hipMemcpy(h_stamps.data(), stamps, h_stamps.size() * 8, hipMemcpyDeviceToHost), LUCIDA_READBACK);
         LUCIDA_HIP(hipMemcpy(totals, p.counters + TOTALS_AT, sizeof totals, hipMemcpyDeviceToHost), LUCIDA_READBACK);
         int rate_khz = 0;
@@ -1634,6 +1695,8 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
             stats->node_visits = totals[2] + totals[4];
             stats->triangle_tests = totals[3] + totals[5];
             stats->stack_overflows = totals[6];
+            stats->extend_lane_slots = totals[7];
+            stats->connect_lane_slots = totals[8];
             stats->elapsed_ms = ms;
             stats->kernel_launches[0] = (uint32_t)rounds.size();
         }
