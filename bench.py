@@ -348,10 +348,10 @@ def main() -> int:
 
     # Roofline of the dominant kernel. EXTEND's algorithmic bytes per ray:
     # queue index 4 + origin/direction planes 32 + hit write 16, plus 64 per
-    # node visit (the two 32-byte children of a binary node; 128 for a 4-wide
-    # node's four boxes and child words) and 64 per triangle test (16-byte
-    # triangle + three 16-byte positions).
-    node_bytes = 128 if a.bvh_width == 4 else 64
+    # node visit (the two 32-byte children of a binary node, or one 64-byte
+    # compressed 4-wide node) and 64 per triangle test (16-byte triangle +
+    # three 16-byte positions).
+    node_bytes = 64
     ext_bytes = ext_rays * (4 + 32 + 16) + node_bytes * ext_visits + 64 * ext_tests
     ext_ms = agg["extend"]
     dominant = max(agg, key=agg.get)

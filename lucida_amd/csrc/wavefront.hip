@@ -29,6 +29,7 @@
 
 #include "context.hpp"
 #include "shading.hpp"
+#include "raystack.hpp"
 #include "widebvh.hpp"
 
 namespace lucida {
@@ -302,41 +303,12 @@ __device__ inline bool tri_test(const float4* __restrict__ edges, uint32_t ti, V
 constexpr uint32_t COUNT_BITS = 5, COUNT_MASK = 31u;
 __device__ inline uint32_t pack_child(uint32_t first, uint32_t count) { return (first << COUNT_BITS) | count; }
 
-// A lane's traversal stack of STACK entries (bvh.wgsl:126-133). The newest
-// LDS_STACK entries live in an LDS ring (one column per lane of the block),
-// older ones in a per-lane column of device memory, so the LDS held per wave
-// is halved and twice the waves fit on a CU; deep stacks are rare, so the
-// spill column is rarely touched. A push onto a full stack is dropped (and
-// counted), as in the reference.
-struct RayStack {
-    uint32_t* ring;        // LDS, entry k at ring[(k % LDS_STACK) * TRACE_WG]
-    uint32_t* spill;       // device memory, entry k at spill[k * spill_stride]
-    uint32_t spill_stride;
-    int sp, lo;            // entries [0, lo) spilled, [lo, sp) in the ring
-    int cap;               // STACK for the binary walks, WIDE_STACK for the 4-wide one
-    __device__ inline bool push(uint32_t v) {
-        if (sp >= cap) return false;
-        if (sp - lo == LDS_STACK) {
-            spill[(size_t)lo * spill_stride] = ring[(lo % LDS_STACK) * TRACE_WG];
-            ++lo;
-        }
-        ring[(sp % LDS_STACK) * TRACE_WG] = v;
-        ++sp;
-        return true;
-    }
-    __device__ inline uint32_t pop() {
-        --sp;
-        if (sp < lo) {
-            lo = sp;
-            return spill[(size_t)sp * spill_stride];
-        }
-        return ring[(sp % LDS_STACK) * TRACE_WG];
-    }
-};
+using RayStack = RayStackT<LDS_STACK, TRACE_WG>;
 
 __device__ inline RayStack make_stack(uint32_t* lds_base, uint32_t* spill_base, int cap) {
     const uint32_t lanes = gridDim.x * TRACE_WG;
-    return RayStack{lds_base + threadIdx.x, spill_base + blockIdx.x * TRACE_WG + threadIdx.x, lanes, 0, 0, cap};
+    return RayStack{lds_base + threadIdx.x, spill_base + blockIdx.x * TRACE_WG + threadIdx.x, lanes, 0, 0, cap, 0u,
+                    false};
 }
 
 // Closest hit (any_hit = false) or occlusion (any_hit = true) against the
@@ -444,8 +416,7 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
     const int base = st.sp;
     while (!walk_step<ANY_HIT>(sc, st, w, base, visits, tests, overflow)) {
     }
-    st.sp = base;  // an any-hit return leaves entries behind
-    if (st.lo > base) st.lo = base;
+    st.truncate(base);  // an any-hit return leaves entries behind
     if (!ANY_HIT && w.found) {
         *t_out = w.t; *u_out = w.u; *v_out = w.v; *tri_out = w.tri;
     }
@@ -453,10 +424,12 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
 }
 
 // The 4-wide walk (widebvh.hpp) over the same leaves: a visit tests the four
-// child boxes of one 128-byte node; hit leaves are intersected at once (so
-// `closest` tightens before the inner children are tested), hit inner
+// child boxes of one 64-byte node (8-bit planes, rounded outward, so every
+// box the binary walk enters is entered); hit leaves are intersected at once
+// (so `closest` tightens before the inner children are tested), hit inner
 // children are visited nearest first, the others pushed. Hits equal the
-// binary walk's except between triangles at exactly equal t.
+// binary walk's except between triangles at exactly equal t and rays that
+// graze a box within the binary slab test's rounding.
 __device__ inline void cswap(float& ea, uint32_t& ra, float& eb, uint32_t& rb) {
     if (eb < ea) {
         const float e = ea;
@@ -466,6 +439,27 @@ __device__ inline void cswap(float& ea, uint32_t& ra, float& eb, uint32_t& rb) {
         ra = rb;
         rb = r;
     }
+}
+
+// Child k's slab on a compressed wide node: plane = origin + q * 2^e, so
+// t = (origin - o) * inv + q * (2^e * inv), one fma per plane with the
+// per-node terms b (base) and s (scale). Returns the entry (at least t_min)
+// and the exit before clipping to the current closest hit; the box is hit
+// when entry <= min(exit, closest), exactly the clipped slab test.
+__device__ inline float qbyte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
+
+__device__ inline void qslab(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t hx, uint32_t hy, uint32_t hz, int k,
+                             V3 b, V3 s, float t_min, float* entry, float* exit) {
+    const float t0x = fmaf(qbyte(lx, k), s.x, b.x), t1x = fmaf(qbyte(hx, k), s.x, b.x);
+    const float t0y = fmaf(qbyte(ly, k), s.y, b.y), t1y = fmaf(qbyte(hy, k), s.y, b.y);
+    const float t0z = fmaf(qbyte(lz, k), s.z, b.z), t1z = fmaf(qbyte(hz, k), s.z, b.z);
+    *entry = fmaxf(fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z)), t_min);
+    *exit = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+}
+
+__device__ inline float exp2_int(uint32_t word, int axis) {
+    const int e = (int)(int8_t)((word >> (8 * axis)) & 0xFFu);
+    return __int_as_float((e + 127) << 23);
 }
 
 template <bool ANY_HIT>
@@ -479,21 +473,21 @@ __device__ inline bool traverse_wide(const SceneView& sc, RayStack& st, V3 o, V3
     const uint32_t cap = 2u * sc.num_wide + 8u;
     for (uint32_t iter = 0; iter < cap; ++iter) {
         *visits += 1;
-        const float4* q = reinterpret_cast<const float4*>(sc.wide + cur);
-        const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
-        const uint4 ch = *reinterpret_cast<const uint4*>(q + 6);
-        const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, LY[4] = {ly.x, ly.y, ly.z, ly.w}, LZ[4] = {lz.x, lz.y, lz.z, lz.w};
-        const float HX[4] = {hx.x, hx.y, hx.z, hx.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+        const uint4* q = reinterpret_cast<const uint4*>(sc.wide + cur);
+        const uint4 a = q[0], ch = q[1], lo = q[2], hi = q[3];
+        // lo = (qlo_x, qlo_y, qlo_z, qhi_x) words, hi = (qhi_y, qhi_z, pad, pad)
+        const V3 s{exp2_int(a.w, 0) * inv.x, exp2_int(a.w, 1) * inv.y, exp2_int(a.w, 2) * inv.z};
+        const V3 b{(__uint_as_float(a.x) - o.x) * inv.x, (__uint_as_float(a.y) - o.y) * inv.y,
+                   (__uint_as_float(a.z) - o.z) * inv.z};
         const uint32_t CH[4] = {ch.x, ch.y, ch.z, ch.w};
+        float EN[4], EX[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qslab(lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, k, b, s, t_min, &EN[k], &EX[k]);
         // leaves first
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t c = CH[k];
-            if (c == WIDE_EMPTY || !(c & WIDE_LEAF)) continue;
-            float en;
-            if (!slab(make_float4(LX[k], LY[k], LZ[k], 0.0f), make_float4(HX[k], HY[k], HZ[k], 0.0f), o, inv, t_min,
-                      closest, &en))
-                continue;
+            if (c == WIDE_EMPTY || !(c & WIDE_LEAF) || !(EN[k] <= fminf(EX[k], closest))) continue;
             const uint32_t first = (c & ~WIDE_LEAF) >> COUNT_BITS, count = c & COUNT_MASK;
             for (uint32_t m = 0; m < count; ++m) {
                 const uint32_t ti = first + m;
@@ -513,14 +507,8 @@ __device__ inline bool traverse_wide(const SceneView& sc, RayStack& st, V3 o, V3
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t c = CH[k];
-            float en = INFINITY;
-            if (c != WIDE_EMPTY && !(c & WIDE_LEAF)) {
-                float e;
-                if (slab(make_float4(LX[k], LY[k], LZ[k], 0.0f), make_float4(HX[k], HY[k], HZ[k], 0.0f), o, inv,
-                         t_min, closest, &e))
-                    en = e;
-            }
-            E[k] = en;
+            const bool inner = c != WIDE_EMPTY && !(c & WIDE_LEAF);
+            E[k] = inner && EN[k] <= fminf(EX[k], closest) ? EN[k] : INFINITY;
             R[k] = c;
         }
         cswap(E[0], R[0], E[1], R[1]);

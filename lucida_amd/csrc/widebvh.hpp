@@ -1,22 +1,30 @@
-// The 4-wide BVH the tracer walks by default: collapsed on the device from the
-// reference-layout binary BVH (GpuBvhNode), one wide node per binary internal
-// node at even depth, whose children are its binary grandchildren (or its
-// leaf children). 128 bytes per node: child boxes as structure-of-arrays, so a
-// visit is seven 16-byte loads from one 128-byte line.
+// The optional 4-wide BVH: collapsed on the device from the reference-layout
+// binary BVH (GpuBvhNode), one wide node per binary internal node at even
+// depth, whose children are its binary grandchildren (or its leaf children).
+// 64 bytes per node: the node's box minimum and per-axis power-of-two scales,
+// the four child words, and each child's box as 8-bit offsets from that
+// minimum in units of the scale, rounded outward (the decoded box contains the
+// child's exact box), so a visit is four 16-byte loads.
 #pragma once
 #include <cstdint>
 
 namespace lucida {
 
-struct alignas(128) WideNode {
-    float lo_x[4], lo_y[4], lo_z[4];
-    float hi_x[4], hi_y[4], hi_z[4];
+struct alignas(64) WideNode {
+    float origin[3];    // the node's box minimum
+    int8_t exp[3];      // child box planes are origin + q * 2^exp (per axis)
+    uint8_t _pad0;
     uint32_t child[4];  // WIDE_EMPTY, WIDE_LEAF | (first << 5) | count, or a wide node index
-    uint32_t _pad[4];
+    uint8_t qlo_x[4], qlo_y[4], qlo_z[4];
+    uint8_t qhi_x[4], qhi_y[4], qhi_z[4];
+    uint32_t _pad1[2];
 };
-static_assert(sizeof(WideNode) == 128, "WideNode is one 128-byte line");
+static_assert(sizeof(WideNode) == 64, "WideNode is 64 bytes");
 
 constexpr uint32_t WIDE_LEAF = 0x80000000u;
 constexpr uint32_t WIDE_EMPTY = 0xFFFFFFFFu;
+// Largest scale exponent accepted: keeps 2^exp times a clamped reciprocal
+// direction (at most 1e16) finite.
+constexpr int WIDE_MAX_EXP = 64;
 
 }  // namespace lucida
