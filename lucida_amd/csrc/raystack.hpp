@@ -21,7 +21,8 @@ namespace lucida {
 
 template <int LDS, int RING_STRIDE>
 struct RayStackT {
-    uint32_t* ring;        // entry k at ring[(k % LDS) * RING_STRIDE]
+    static_assert(LDS > 0 && (LDS & (LDS - 1)) == 0, "the ring size is a power of two");
+    uint32_t* ring;        // entry k at ring[slot(k)] = ring[(k % LDS) * RING_STRIDE]
     uint32_t* spill;       // entry k at spill[k * spill_stride]
     uint32_t spill_stride;
     int sp, lo;            // entries [0, lo) spilled, [lo, sp - held) in the ring
@@ -29,13 +30,15 @@ struct RayStackT {
     uint32_t top;          // entry sp - 1 when held
     bool held;
 
+    // Ring word of entry k (k >= 0; LDS a power of two, so this is a mask).
+    LUCIDA_HD static inline uint32_t slot(int k) { return ((uint32_t)k % (uint32_t)LDS) * (uint32_t)RING_STRIDE; }
     // Stores entry k into the ring, spilling the ring's oldest entry if full.
     LUCIDA_HD inline void sink(int k, uint32_t v) {
         if (k - lo == LDS) {
-            spill[(size_t)lo * spill_stride] = ring[(lo % LDS) * RING_STRIDE];
+            spill[(size_t)lo * spill_stride] = ring[slot(lo)];
             ++lo;
         }
-        ring[(k % LDS) * RING_STRIDE] = v;
+        ring[slot(k)] = v;
     }
     LUCIDA_HD inline bool push(uint32_t v) {
         if (sp >= cap) return false;
@@ -56,7 +59,7 @@ struct RayStackT {
             lo = sp;
             return spill[(size_t)sp * spill_stride];
         }
-        return ring[(sp % LDS) * RING_STRIDE];
+        return ring[slot(sp)];
     }
     // Drops the entries above `base` (a walk that returned early). The entry
     // below them, if any, was stored by the first push above it.

@@ -315,91 +315,27 @@ __device__ inline RayStack make_stack(uint32_t* lds_base, uint32_t* spill_base, 
 // triangles of the tree rooted at `root` (intersect_bvh_from, bvh.wgsl:137:
 // the root's own box is not tested). Returns the hit flag; closest-hit fills
 // t/u/v/tri. Entries the walk pushes sit above the stack's current top.
-// One ray's walk, advanced one node visit at a time (walk_step).
-struct BinWalk {
-    V3 o, d, inv;
-    float t_min, closest;
-    float t, u, v;
-    uint32_t tri;
-    uint32_t cur;   // packed child visited next
-    uint32_t iter;  // visits so far (bounded by the node count)
-    bool found;
-};
-
-// Starts a walk at the root's children; `root` is the root node's packed
-// (first, count).
-__device__ inline void walk_begin(BinWalk& w, uint32_t root, V3 o, V3 d, float t_min, float t_max) {
-    w.o = o;
-    w.d = d;
-    w.inv = safe_inv_dir(d);
-    w.t_min = t_min;
-    w.closest = t_max;
-    w.t = w.u = w.v = 0.0f;
-    w.tri = 0;
-    w.iter = 0;
-    w.found = false;
-    w.cur = root;
-}
-
-__device__ inline uint32_t packed_root(const SceneView& sc, uint32_t root_index) {
-    const GpuBvhNode root = sc.nodes[root_index];
-    return pack_child(root.left_first, root.count);
-}
-
-// Visits w.cur; returns true when the walk is over (stack back at `base`,
-// or, for any-hit, the first hit). A tree visits each node at most once per
-// ray; the cap only guarantees termination on a malformed (cyclic) array.
-template <bool ANY_HIT>
-__device__ __forceinline__ bool walk_step(const SceneView& sc, RayStack& st, BinWalk& w, int base, uint32_t* visits,
-                                          uint32_t* tests, uint32_t* overflow) {
-    *visits += 1;
-    const uint32_t first = w.cur >> COUNT_BITS, count = w.cur & COUNT_MASK;
-    if (count > 0) {
-        for (uint32_t k = 0; k < count; ++k) {
-            const uint32_t ti = first + k;
-            *tests += 1;
-            float t, u, v;
-            if (tri_test(sc.edges, ti, w.o, w.d, w.t_min, w.closest, &t, &u, &v)) {
-                w.found = true;
-                if (ANY_HIT) return true;
-                w.closest = t;
-                w.t = t; w.u = u; w.v = v; w.tri = ti;
-            }
-        }
-    } else {
-        const float4* cn = reinterpret_cast<const float4*>(sc.nodes + first);
-        const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
-        float ea, eb;
-        const bool ha = slab(a0, a1, w.o, w.inv, w.t_min, w.closest, &ea);
-        const bool hb = slab(b0, b1, w.o, w.inv, w.t_min, w.closest, &eb);
-        const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
-        const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
-        if (ha && hb) {
-            const bool a_near = ea <= eb;
-            if (!st.push(a_near ? cb : ca)) *overflow += 1;
-            w.cur = a_near ? ca : cb;
-            return ++w.iter >= 2u * sc.num_nodes + 8u;
-        } else if (ha || hb) {
-            w.cur = ha ? ca : cb;
-            return ++w.iter >= 2u * sc.num_nodes + 8u;
-        }
-    }
-    if (st.sp == base) return true;
-    w.cur = st.pop();
-    return ++w.iter >= 2u * sc.num_nodes + 8u;
-}
-
 // Closest hit (any_hit = false) or occlusion (any_hit = true) against the
 // triangles of the tree rooted at `root` (intersect_bvh_from, bvh.wgsl:137:
 // the root's own box is not tested). Returns the hit flag; closest-hit fills
 // t/u/v/tri. Entries the walk pushes sit above the stack's current top.
+//
+// The walk visits nodes in the reference's order. For closest hit it runs as
+// two loops: inner nodes until the lane stands on a leaf, then the leaf. A
+// wave executes an iteration of a loop while any lane needs it, so
+// separating the two kinds of visit keeps a wave from paying for both in
+// every step (the kernels are bound by vector-instruction issue, not
+// memory). Occlusion rays mostly end at their first hit leaf, where one loop
+// that takes either kind of visit per step measured faster. A tree visits
+// each node at most once per ray; the cap only guarantees termination on a
+// malformed (cyclic) node array.
 template <bool ANY_HIT>
 __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root_index, V3 o, V3 d, float t_min,
                                 float t_max, float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
                                 uint32_t* visits, uint32_t* tests, uint32_t* overflow) {
+    bool found = false;
+    float closest = t_max;
     if (sc.num_nodes == 0) {
-        bool found = false;
-        float closest = t_max;
         for (uint32_t i = 0; i < sc.num_tris; ++i) {
             float t, u, v;
             if (tri_test(sc.edges, i, o, d, t_min, closest, &t, &u, &v)) {
@@ -411,16 +347,98 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
         }
         return found;
     }
-    BinWalk w;
-    walk_begin(w, packed_root(sc, root_index), o, d, t_min, t_max);
+    const V3 inv = safe_inv_dir(d);
+    const GpuBvhNode root = sc.nodes[root_index];
+    uint32_t cur = pack_child(root.left_first, root.count);
     const int base = st.sp;
-    while (!walk_step<ANY_HIT>(sc, st, w, base, visits, tests, overflow)) {
+    const uint32_t cap = 2u * sc.num_nodes + 8u;
+    uint32_t iter = 0;
+    if (ANY_HIT) {
+        // one visit, of either kind, per step
+        for (;;) {
+            *visits += 1;
+            const uint32_t first = cur >> COUNT_BITS, count = cur & COUNT_MASK;
+            if (count > 0) {
+                for (uint32_t k = 0; k < count; ++k) {
+                    *tests += 1;
+                    float t, u, v;
+                    if (tri_test(sc.edges, first + k, o, d, t_min, closest, &t, &u, &v)) {
+                        st.truncate(base);
+                        return true;
+                    }
+                }
+            } else {
+                const float4* cn = reinterpret_cast<const float4*>(sc.nodes + first);
+                const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
+                float ea, eb;
+                const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
+                const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
+                const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
+                const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
+                if (ha && hb) {
+                    const bool a_near = ea <= eb;
+                    if (!st.push(a_near ? cb : ca)) *overflow += 1;
+                    cur = a_near ? ca : cb;
+                    if (++iter >= cap) break;
+                    continue;
+                } else if (ha || hb) {
+                    cur = ha ? ca : cb;
+                    if (++iter >= cap) break;
+                    continue;
+                }
+            }
+            if (st.sp == base) break;
+            cur = st.pop();
+            if (++iter >= cap) break;
+        }
+        st.truncate(base);
+        return false;
     }
-    st.truncate(base);  // an any-hit return leaves entries behind
-    if (!ANY_HIT && w.found) {
-        *t_out = w.t; *u_out = w.u; *v_out = w.v; *tri_out = w.tri;
+    bool more = true;
+    while (more) {
+        // inner nodes, until a leaf
+        while (more && (cur & COUNT_MASK) == 0) {
+            *visits += 1;
+            const float4* cn = reinterpret_cast<const float4*>(sc.nodes + (cur >> COUNT_BITS));
+            const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
+            float ea, eb;
+            const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
+            const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
+            const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
+            const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
+            if (ha && hb) {
+                const bool a_near = ea <= eb;
+                if (!st.push(a_near ? cb : ca)) *overflow += 1;
+                cur = a_near ? ca : cb;
+            } else if (ha || hb) {
+                cur = ha ? ca : cb;
+            } else if (st.sp == base) {
+                more = false;
+            } else {
+                cur = st.pop();
+            }
+            more = more && ++iter < cap;
+        }
+        if (!more) break;
+        // the leaf
+        *visits += 1;
+        const uint32_t first = cur >> COUNT_BITS, count = cur & COUNT_MASK;
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t ti = first + k;
+            *tests += 1;
+            float t, u, v;
+            if (tri_test(sc.edges, ti, o, d, t_min, closest, &t, &u, &v)) {
+                found = true;
+                closest = t;
+                *t_out = t; *u_out = u; *v_out = v; *tri_out = ti;
+            }
+        }
+        if (st.sp == base) break;
+        cur = st.pop();
+        more = ++iter < cap;
     }
-    return w.found;
+    st.truncate(base);
+    return found;
 }
 
 // The 4-wide walk (widebvh.hpp) over the same leaves: a visit tests the four
