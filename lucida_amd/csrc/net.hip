@@ -66,65 +66,77 @@ __global__ void k_wprep(const float* __restrict__ W, int cin, int cout, int tran
 
 // ---------------------------------------------------------------------------
 // Convolution (forward, or input gradient with prepared weights).
-// in:  NHWC bf16 [n][h][w][cin]; out: [n][h][w][MT*16]. One wave: 16
-// consecutive pixels of a row (w % 16 == 0). bias (optional, fp32) and ReLU
-// fused in the epilogue.
+// in:  NHWC bf16 [n][h][w][cin]; out: [n][h][w][MT*16]. One wave: NT tiles
+// of 16 consecutive pixels of a row (w % (16 * NT) == 0); each K chunk's
+// weight fragments are loaded once and used for all NT tiles, and every
+// output keeps the same sequence of MFMAs whatever NT is. bias (optional,
+// fp32) and ReLU fused in the epilogue.
 // ---------------------------------------------------------------------------
-template <int MT>
+template <int MT, int NT>
 __global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ in, int cin, const bf16x8* __restrict__ frag,
                                               const float* __restrict__ bias, int relu, bf16* __restrict__ out,
                                               int n, int h, int w) {
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * (blockDim.x >> 6)) + (threadIdx.x >> 6);
-    const int tiles = n * h * (w >> 4);
+    const int row_tiles = w / (16 * NT);
+    const int tiles = n * h * row_tiles;
     if (wave >= tiles) return;
-    const int xt = wave % (w >> 4), rowi = wave / (w >> 4);
+    const int xt = wave % row_tiles, rowi = wave / row_tiles;
     const int y = rowi % h, img = rowi / h;
-    const int x0 = xt * 16;
+    const int x0 = xt * 16 * NT;
     const int g = lane >> 4, r = lane & 15;
     const int cbs = cin / 16;
     const int kbs = 9 * cbs;
     const int chunks = (kbs + 1) / 2;
-    const int px = x0 + r;
-    f32x4 acc[MT];
+    f32x4 acc[NT][MT];
 #pragma unroll
-    for (int m = 0; m < MT; ++m) acc[m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[t][m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int c = 0; c < chunks; ++c) {
         const int kb = 2 * c + (g >> 1);
-        bf16x8 bv;
-        bool ok = kb < kbs;
+        const bool kok = kb < kbs;
         int tap = 0, cb = 0;
-        if (ok) {
+        if (kok) {
             tap = kb / cbs;
             cb = kb - tap * cbs;
         }
-        const int sy = y + tap / 3 - 1, sx = px + tap % 3 - 1;
-        ok = ok && sy >= 0 && sy < h && sx >= 0 && sx < w;
-        if (ok) {
-            bv = *reinterpret_cast<const bf16x8*>(in + (((size_t)img * h + sy) * w + sx) * cin + cb * 16 + 8 * (g & 1));
-        } else {
+        const int sy = y + tap / 3 - 1;
+        const bool yok = kok && sy >= 0 && sy < h;
+        bf16x8 av[MT];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) bv[i] = f2b(0.0f);
-        }
+        for (int m = 0; m < MT; ++m) av[m] = frag[(c * MT + m) * 64 + lane];
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const bf16x8 av = frag[(c * MT + m) * 64 + lane];
-            acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[m], 0, 0, 0);
+        for (int t = 0; t < NT; ++t) {
+            const int sx = x0 + 16 * t + r + tap % 3 - 1;
+            bf16x8 bv;
+            if (yok && sx >= 0 && sx < w) {
+                bv = *reinterpret_cast<const bf16x8*>(in + (((size_t)img * h + sy) * w + sx) * cin + cb * 16 +
+                                                      8 * (g & 1));
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) bv[i] = f2b(0.0f);
+            }
+#pragma unroll
+            for (int m = 0; m < MT; ++m) acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[t][m], 0, 0, 0);
         }
     }
     const int cout = MT * 16;
-    bf16* o = out + (((size_t)img * h + y) * w + px) * cout;
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-        bf16x4 v;
+    for (int t = 0; t < NT; ++t) {
+        bf16* o = out + (((size_t)img * h + y) * w + x0 + 16 * t + r) * cout;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int co = m * 16 + 4 * g + j;
-            float s = acc[m][j] + (bias ? bias[co] : 0.0f);
-            if (relu) s = fmaxf(s, 0.0f);
-            v[j] = f2b(s);
+        for (int m = 0; m < MT; ++m) {
+            bf16x4 v;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int co = m * 16 + 4 * g + j;
+                float s = acc[t][m][j] + (bias ? bias[co] : 0.0f);
+                if (relu) s = fmaxf(s, 0.0f);
+                v[j] = f2b(s);
+            }
+            *reinterpret_cast<bf16x4*>(o + m * 16 + 4 * g) = v;
         }
-        *reinterpret_cast<bf16x4*>(o + m * 16 + 4 * g) = v;
     }
 }
 
@@ -234,18 +246,26 @@ __global__ void k_relu_bwd(bf16* __restrict__ d, const bf16* __restrict__ y, siz
 }
 
 // 2x2 average pool: out[n][h/2][w/2][c].
+// 2x2 average pool and nearest-upsample + channel concat, eight channels
+// (16 bytes) per thread; channel counts are multiples of 16. The pool sums
+// the four taps in the same order for every channel.
 __global__ void k_pool(const bf16* __restrict__ in, int n, int h, int w, int c, bf16* __restrict__ out) {
-    const size_t total = (size_t)n * (h / 2) * (w / 2) * c;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int ch = e % c;
-        size_t p = e / c;
-        const int x = p % (w / 2), y = (p / (w / 2)) % (h / 2), img = p / ((size_t)(w / 2) * (h / 2));
-        const bf16* b = in + (((size_t)img * h + 2 * y) * w + 2 * x) * c + ch;
-        const float s = b2f(b[0]) + b2f(b[c]) + b2f(b[(size_t)w * c]) + b2f(b[(size_t)w * c + c]);
-        out[e] = f2b(0.25f * s);
+    const uint32_t c8 = (uint32_t)c / 8, w2 = (uint32_t)w / 2, h2 = (uint32_t)h / 2;
+    const uint32_t total = (uint32_t)n * h2 * w2 * c8;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const uint32_t g = e % c8, p = e / c8;
+        const uint32_t x = p % w2, q = p / w2, y = q % h2, img = q / h2;
+        const bf16* b = in + (((size_t)img * h + 2 * y) * w + 2 * x) * c + 8 * g;
+        const bf16x8 v00 = *reinterpret_cast<const bf16x8*>(b);
+        const bf16x8 v01 = *reinterpret_cast<const bf16x8*>(b + c);
+        const bf16x8 v10 = *reinterpret_cast<const bf16x8*>(b + (size_t)w * c);
+        const bf16x8 v11 = *reinterpret_cast<const bf16x8*>(b + (size_t)w * c + c);
+        bf16x8 r;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i] = f2b(0.25f * (b2f(v00[i]) + b2f(v01[i]) + b2f(v10[i]) + b2f(v11[i])));
+        *reinterpret_cast<bf16x8*>(out + (size_t)p * c + 8 * g) = r;
     }
 }
-
 // Gradient through the pool plus the skip branch: out = up(dpool)/4 + dskip.
 __global__ void k_pool_bwd_add(const bf16* __restrict__ dpool, const bf16* __restrict__ dskip, int n, int h, int w,
                                int c, bf16* __restrict__ out) {
@@ -262,17 +282,16 @@ __global__ void k_pool_bwd_add(const bf16* __restrict__ dpool, const bf16* __res
 // Nearest 2x upsample of `low` (c1 channels) concatenated with `skip` (c2).
 __global__ void k_upcat(const bf16* __restrict__ low, int c1, const bf16* __restrict__ skip, int c2, int n, int h,
                         int w, bf16* __restrict__ out) {
-    const int c = c1 + c2;
-    const size_t total = (size_t)n * h * w * c;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int ch = e % c;
-        size_t p = e / c;
-        const int x = p % w, y = (p / w) % h, img = p / ((size_t)w * h);
-        out[e] = ch < c1 ? low[(((size_t)img * (h / 2) + y / 2) * (w / 2) + x / 2) * c1 + ch]
-                         : skip[p * c2 + (ch - c1)];
+    const uint32_t c8 = (uint32_t)(c1 + c2) / 8, l8 = (uint32_t)c1 / 8;
+    const uint32_t total = (uint32_t)n * h * w * c8;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const uint32_t g = e % c8, p = e / c8;
+        const uint32_t x = p % (uint32_t)w, q = p / (uint32_t)w, y = q % (uint32_t)h, img = q / (uint32_t)h;
+        const bf16* src = g < l8 ? low + (((size_t)img * (h / 2) + y / 2) * (w / 2) + x / 2) * c1 + 8 * g
+                                 : skip + (size_t)p * c2 + 8 * (g - l8);
+        *reinterpret_cast<bf16x8*>(out + (size_t)e * 8) = *reinterpret_cast<const bf16x8*>(src);
     }
 }
-
 // Backward of k_upcat: dlow sums its 2x2 block, dskip is a slice.
 __global__ void k_upcat_bwd(const bf16* __restrict__ dcat, int c1, int c2, int n, int h, int w, bf16* __restrict__ dlow,
                             bf16* __restrict__ dskip) {
@@ -449,20 +468,40 @@ size_t net_param_count() {
     return c;
 }
 
+template <int MT, int NT>
+static void launch_conv_t(hipStream_t s, const bf16* in, int cin, const bf16x8* f, const float* bias, int relu,
+                          bf16* out, int n, int h, int w) {
+    const int tiles = n * h * (w / (16 * NT));
+    hipLaunchKernelGGL((k_conv<MT, NT>), dim3((tiles + 3) / 4), dim3(256), 0, s, in, cin, f, bias, relu, out, n, h, w);
+}
+
+// Tiles per wave: 4 for 16 output channels, 2 above (more tiles cost
+// occupancy the weight reuse does not repay), fewer when the row width does
+// not divide.
+template <int MT>
+static void launch_conv_m(hipStream_t s, const bf16* in, int cin, const bf16x8* f, const float* bias, int relu,
+                          bf16* out, int n, int h, int w) {
+    constexpr int NT_MAX = MT == 1 ? 4 : 2;
+    if (NT_MAX == 4 && w % 64 == 0)
+        launch_conv_t<MT, NT_MAX>(s, in, cin, f, bias, relu, out, n, h, w);
+    else if (w % 32 == 0)
+        launch_conv_t<MT, 2>(s, in, cin, f, bias, relu, out, n, h, w);
+    else
+        launch_conv_t<MT, 1>(s, in, cin, f, bias, relu, out, n, h, w);
+}
+
 static lucida_status launch_conv(hipStream_t s, const bf16* in, int cin, int cout, const bf16* frag, const float* bias,
                                  int relu, bf16* out, int n, int h, int w) {
-    const int tiles = n * h * (w / 16);
-    const dim3 grid((tiles + 3) / 4), block(256);
     const bf16x8* f = reinterpret_cast<const bf16x8*>(frag);
     switch (cout / 16) {
-        case 1: hipLaunchKernelGGL(k_conv<1>, grid, block, 0, s, in, cin, f, bias, relu, out, n, h, w); break;
-        case 2: hipLaunchKernelGGL(k_conv<2>, grid, block, 0, s, in, cin, f, bias, relu, out, n, h, w); break;
-        case 3: hipLaunchKernelGGL(k_conv<3>, grid, block, 0, s, in, cin, f, bias, relu, out, n, h, w); break;
-        case 4: hipLaunchKernelGGL(k_conv<4>, grid, block, 0, s, in, cin, f, bias, relu, out, n, h, w); break;
-        case 6: hipLaunchKernelGGL(k_conv<6>, grid, block, 0, s, in, cin, f, bias, relu, out, n, h, w); break;
-        case 7: hipLaunchKernelGGL(k_conv<7>, grid, block, 0, s, in, cin, f, bias, relu, out, n, h, w); break;
-        case 10: hipLaunchKernelGGL(k_conv<10>, grid, block, 0, s, in, cin, f, bias, relu, out, n, h, w); break;
-        case 5: hipLaunchKernelGGL(k_conv<5>, grid, block, 0, s, in, cin, f, bias, relu, out, n, h, w); break;
+        case 1: launch_conv_m<1>(s, in, cin, f, bias, relu, out, n, h, w); break;
+        case 2: launch_conv_m<2>(s, in, cin, f, bias, relu, out, n, h, w); break;
+        case 3: launch_conv_m<3>(s, in, cin, f, bias, relu, out, n, h, w); break;
+        case 4: launch_conv_m<4>(s, in, cin, f, bias, relu, out, n, h, w); break;
+        case 5: launch_conv_m<5>(s, in, cin, f, bias, relu, out, n, h, w); break;
+        case 6: launch_conv_m<6>(s, in, cin, f, bias, relu, out, n, h, w); break;
+        case 7: launch_conv_m<7>(s, in, cin, f, bias, relu, out, n, h, w); break;
+        case 10: launch_conv_m<10>(s, in, cin, f, bias, relu, out, n, h, w); break;
         default: return set_error(LUCIDA_INVALID_ARG, "conv: unsupported output channel count");
     }
     return LUCIDA_OK;
@@ -581,12 +620,12 @@ static lucida_status net_forward(lucida_net* net) {
                            Wd[lvl]);
     };
     auto pool = [&](size_t in_off, int lvl, int c, size_t out_off) {
-        size_t total = (size_t)n * H[lvl + 1] * Wd[lvl + 1] * c;
+        size_t total = (size_t)n * H[lvl + 1] * Wd[lvl + 1] * c / 8;
         hipLaunchKernelGGL(k_pool, dim3(grid1(total)), dim3(256), 0, s, A + in_off, n, H[lvl], Wd[lvl], c,
                            A + out_off);
     };
     auto upcat = [&](size_t low, int c1, size_t skip, int c2, int lvl, size_t out_off) {
-        size_t total = (size_t)n * H[lvl] * Wd[lvl] * (c1 + c2);
+        size_t total = (size_t)n * H[lvl] * Wd[lvl] * (c1 + c2) / 8;
         hipLaunchKernelGGL(k_upcat, dim3(grid1(total)), dim3(256), 0, s, A + low, c1, A + skip, c2, n, H[lvl],
                            Wd[lvl], A + out_off);
     };
