@@ -50,8 +50,7 @@ __device__ inline float wave_max(float v) {
 __global__ __launch_bounds__(PREP_BLOCK) void k_prepare(const GpuTriangle* __restrict__ tris,
                                                         const float4* __restrict__ pos, uint32_t n,
                                                         LucidaAabb* __restrict__ tri_bounds,
-                                                        float* __restrict__ partial, uint32_t* __restrict__ ticket,
-                                                        LucidaAabb* __restrict__ scene_box) {
+                                                        float* __restrict__ partial) {
     const uint32_t i = blockIdx.x * PREP_BLOCK + threadIdx.x;
     float c[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     if (i < n) {
@@ -83,7 +82,6 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prepare(const GpuTriangle* __res
         }
     }
     __shared__ float red[PREP_BLOCK / 64][6];
-    __shared__ bool last;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -102,19 +100,21 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prepare(const GpuTriangle* __res
                 r[3 + a] = fmaxf(r[3 + a], red[w][3 + a]);
             }
         for (int a = 0; a < 6; ++a) partial[blockIdx.x * 6 + a] = r[a];
-        __threadfence();
-        uint32_t done = atomicAdd(ticket, 1u);
-        last = (done == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // The last block folds every partial.
+}
+
+// Folds the per-block centroid bounds into the scene's (one block). A launch
+// of its own: a last-block-done ticket would cost every block of k_prepare a
+// device-scope fence and an atomic on one address.
+__global__ __launch_bounds__(PREP_BLOCK) void k_fold_bounds(const float* __restrict__ partial, uint32_t blocks,
+                                                            LucidaAabb* __restrict__ scene_box) {
+    __shared__ float red[PREP_BLOCK / 64][6];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += PREP_BLOCK) {
+    for (uint32_t b = threadIdx.x; b < blocks; b += PREP_BLOCK) {
         for (int a = 0; a < 3; ++a) {
-            r[a] = fminf(r[a], __builtin_nontemporal_load(&partial[b * 6 + a]));
-            r[3 + a] = fmaxf(r[3 + a], __builtin_nontemporal_load(&partial[b * 6 + 3 + a]));
+            r[a] = fminf(r[a], partial[b * 6 + a]);
+            r[3 + a] = fmaxf(r[3 + a], partial[b * 6 + 3 + a]);
         }
     }
     for (int a = 0; a < 3; ++a) {
@@ -130,11 +130,10 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prepare(const GpuTriangle* __res
                 red[0][a] = fminf(red[0][a], red[w][a]);
                 red[0][3 + a] = fmaxf(red[0][3 + a], red[w][3 + a]);
             }
-        LucidaAabb s;
-        s.lo[0] = red[0][0]; s.lo[1] = red[0][1]; s.lo[2] = red[0][2]; s._p0 = 0.0f;
-        s.hi[0] = red[0][3]; s.hi[1] = red[0][4]; s.hi[2] = red[0][5]; s._p1 = 0.0f;
-        *scene_box = s;
-        *ticket = 0u;  // ready for the next build
+        LucidaAabb sb;
+        sb.lo[0] = red[0][0]; sb.lo[1] = red[0][1]; sb.lo[2] = red[0][2]; sb._p0 = 0.0f;
+        sb.hi[0] = red[0][3]; sb.hi[1] = red[0][4]; sb.hi[2] = red[0][5]; sb._p1 = 0.0f;
+        *scene_box = sb;
     }
 }
 
@@ -744,7 +743,7 @@ static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size
     LUCIDA_HIP(ctx->lb_subtree.ensure(internal * 4 * 3), LUCIDA_OOM);  // subtree, inner, rank
     LUCIDA_HIP(ctx->lb_flags.ensure(internal * 4), LUCIDA_OOM);
 
-    uint32_t* misc = ctx->lb_misc.as<uint32_t>();  // [0] ticket, [8..16) scene box
+    uint32_t* misc = ctx->lb_misc.as<uint32_t>();  // [8..16) scene box
     LucidaAabb* scene_box = reinterpret_cast<LucidaAabb*>(misc + 8);
     hipEvent_t ev[6];
     const bool timed = stage_us != nullptr;
@@ -752,9 +751,10 @@ static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size
         for (auto& e : ev) LUCIDA_HIP(hipEventCreate(&e), LUCIDA_VALIDATION);
     if (timed) LUCIDA_HIP(hipEventRecord(ev[0], s), LUCIDA_VALIDATION);
 
-    LUCIDA_HIP(hipMemsetAsync(misc, 0, 4, s), LUCIDA_VALIDATION);
     hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(PREP_BLOCK), 0, s, tris, reinterpret_cast<const float4*>(positions),
-                       (uint32_t)n, ctx->lb_bounds.as<LucidaAabb>(), ctx->lb_partial.as<float>(), misc, scene_box);
+                       (uint32_t)n, ctx->lb_bounds.as<LucidaAabb>(), ctx->lb_partial.as<float>());
+    hipLaunchKernelGGL(k_fold_bounds, dim3(1), dim3(PREP_BLOCK), 0, s, ctx->lb_partial.as<float>(), (uint32_t)nb,
+                       scene_box);
     hipLaunchKernelGGL(k_morton, dim3(nb), dim3(PREP_BLOCK), 0, s, ctx->lb_bounds.as<LucidaAabb>(), (uint32_t)n,
                        scene_box, ctx->lb_codes.as<uint32_t>(), ctx->lb_vals.as<uint32_t>());
     if (keep_unsorted_codes)
