@@ -109,6 +109,40 @@ def cpu_baseline(blob, u, target_s: float) -> dict:
                       f"{u.max_depth} bounces ({paths} paths, {dt:.1f} s)"}
 
 
+def other_configs(L, gpu) -> dict:
+    """Throughput of the other two render workloads at their full sizes: the
+    Cornell box (256^2, 64 spp, 8 bounces, independent PCG samples) and the
+    two-level instanced scene (4,096 instances of a 9,680-triangle mesh, a
+    2048x1024 environment map, 1024^2, 32 spp, 12 bounces, Sobol). Scenes are
+    resident before timing; one warm-up render each."""
+    out = {}
+    for key, name, w, h, spp, depth, sampler in (
+            ("box_c1", "box-c1", 256, 256, 64, 8, L.SAMPLER_INDEPENDENT),
+            ("instances_c3", "instances-c3", 1024, 1024, 32, 12, L.SAMPLER_SOBOL)):
+        blob = L.Blob(name)
+        t0 = time.perf_counter()
+        scene = L.Scene(gpu, blob)
+        gpu.check_errors()
+        build_ms = (time.perf_counter() - t0) * 1e3
+        film = L.Film(gpu, w, h)
+        u = blob.uniforms(w, h, samples=spp, max_depth=depth, sampling=L.SAMPLING_MIS, sampler=sampler)
+        L.render_wavefront(gpu, scene, u, spp, film)
+        gpu.check_errors()
+        film.clear()
+        t0 = time.perf_counter()
+        st = L.render_wavefront(gpu, scene, u, spp, film)
+        gpu.check_errors()
+        sec = time.perf_counter() - t0
+        counts = scene.counts()
+        out[key] = {"mpaths_per_s": round(w * h * spp / sec / 1e6, 2), "ms": round(sec * 1e3, 2),
+                    "device_ms": round(st.elapsed_ms, 2), "width": w, "height": h, "spp": spp, "max_depth": depth,
+                    "extension_rays": st.extension_rays, "shadow_rays": st.shadow_rays,
+                    "scene_upload_and_build_ms": round(build_ms, 1), "triangles": counts.get("triangles"),
+                    "instances": counts.get("instances")}
+        del film, scene
+    return out
+
+
 def relmse(x: np.ndarray, ref: np.ndarray) -> float:
     """Relative MSE, mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2).
     The reference defines no relMSE; this is the definition used throughout."""
@@ -277,6 +311,8 @@ def main() -> int:
     ap.add_argument("--net-ref-spp", type=int, default=1024, help="target samples per pixel of the training pairs")
     ap.add_argument("--net-steps", type=int, default=3000, help="Adam steps (0: no learned denoiser)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched request stream")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the Cornell-box and instanced-scene throughput figures")
     ap.add_argument("--denoiser-timing", action="store_true",
                     help="only time the two denoisers on a 1-spp render of the benchmark scene (untrained network "
                          "weights: inference cost does not depend on them) and exit")
@@ -427,6 +463,8 @@ def main() -> int:
             "atrous_ms": round(atrous_ms, 3),
         },
     }
+    if not a.no_configs:
+        out["extra"]["other_configs"] = other_configs(L, gpu)
     if not a.no_relmse:
         net = None
         if a.net_steps > 0:
