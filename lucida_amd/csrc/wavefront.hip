@@ -43,7 +43,8 @@ constexpr int WG = 64;          // one wave per workgroup for the per-path kerne
 constexpr int TRACE_WG = 128;   // traversal workgroups (two waves)
 constexpr int STACK = 32;       // traversal stack entries per lane (shaders/common/bvh.wgsl:133)
 constexpr int LDS_STACK = 16;   // the newest entries, in LDS; older ones spill to memory
-constexpr int WIDE_STACK = 48;  // the 4-wide walk pushes up to three entries per visit
+constexpr int WIDE_STACK = 48;  // the 4-wide walk pushes up to three entries per visit; the two-level
+                                // walk adds a RETURN marker and up to three waiting instances
 // Traversal variants a kernel is compiled for.
 constexpr int TRAV_BINARY = 0, TRAV_WIDE = 1, TRAV_INSTANCED = 2;
 constexpr int MAX_BOUNCES = 64;
@@ -576,38 +577,52 @@ __device__ inline V3 xf_normal(const InstanceXf& I, V3 n) {
 // Two-level traversal (intersect_instances, bvh.wgsl:305): the TLAS rooted at
 // sc.tlas_root in world space; at each instance leaf the ray goes to object
 // space and walks that instance's BLAS with `closest` carried across levels.
-// The BLAS walk uses the stack entries above the TLAS's own. With ANY_HIT it
-// is the two-level occlusion query (the reference's shadow rays walk only the
-// tree at node 0, see connect.wgsl:192-212; here they see every instance).
+// With ANY_HIT it is the two-level occlusion query (the reference's shadow
+// rays walk only the tree at node 0, see connect.wgsl:192-212; here they see
+// every instance).
+//
+// One loop for both levels: a lane is either in the TLAS (world ray) or in a
+// BLAS (object-space ray). Entering an instance pushes a RETURN marker below
+// the BLAS's entries and the leaf's remaining instances as tagged entries
+// below that, so popping the marker restores the world ray and popping a
+// tagged entry enters the next instance. Nodes, instances and triangles are
+// visited in the order of a BLAS walk nested in the TLAS walk (so the hits
+// are the same), but a lane never waits in the TLAS loop for another lane's
+// whole BLAS walk. Each level's visits are capped as a nested walk's would be.
 template <bool ANY_HIT>
-__device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 o, V3 d, float t_min, float t_max,
-                                          float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
+__device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 o_w, V3 d_w, float t_min,
+                                          float t_max, float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
                                           uint32_t* inst_out, uint32_t* visits, uint32_t* tests,
                                           uint32_t* overflow) {
+    constexpr uint32_t RETURN = 0xFFFFFFFFu;  // never a packed child: node indices are below 2^26
+    constexpr uint32_t INST = 0x80000000u;    // tag of a waiting instance (primitive index below 2^26)
     bool found = false;
     float closest = t_max;
-    const V3 inv = safe_inv_dir(d);
+    const V3 inv_w = safe_inv_dir(d_w);
+    V3 o = o_w, d = d_w, inv = inv_w;
     const GpuBvhNode root = sc.nodes[sc.tlas_root];
     uint32_t cur = pack_child(root.left_first, root.count);
+    bool in_blas = false;
+    uint32_t inst = 0, tlas_iter = 0, blas_iter = 0;
     const uint32_t cap = 2u * sc.num_nodes + 8u;
-    for (uint32_t iter = 0; iter < cap; ++iter) {
+    const int base = st.sp;
+    auto enter = [&](uint32_t ii) {
+        if (!st.push(RETURN)) *overflow += 1;
+        const InstanceXf I = load_instance(sc.insts, ii);
+        o = xf_point(I, o_w);
+        d = xf_vector(I, d_w);
+        inv = safe_inv_dir(d);
+        const GpuBvhNode br = sc.nodes[I.blas_root];
+        cur = pack_child(br.left_first, br.count);
+        in_blas = true;
+        inst = ii;
+        blas_iter = 0;
+    };
+    for (;;) {
         *visits += 1;
         const uint32_t first = cur >> COUNT_BITS, count = cur & COUNT_MASK;
-        if (count > 0) {
-            for (uint32_t k = 0; k < count; ++k) {
-                const uint32_t ii = first + k;
-                const InstanceXf I = load_instance(sc.insts, ii);
-                float t, u, v;
-                uint32_t tri;
-                if (traverse<ANY_HIT>(sc, st, I.blas_root, xf_point(I, o), xf_vector(I, d), t_min, closest, &t, &u,
-                                      &v, &tri, visits, tests, overflow)) {
-                    if (ANY_HIT) return true;
-                    closest = t;
-                    *t_out = t; *u_out = u; *v_out = v; *tri_out = tri; *inst_out = ii;
-                    found = true;
-                }
-            }
-        } else {
+        bool next = false;  // cur holds the next node to visit
+        if (count == 0) {
             const float4* cn = reinterpret_cast<const float4*>(sc.nodes + first);
             const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
             float ea, eb;
@@ -619,18 +634,62 @@ __device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 
                 const bool a_near = ea <= eb;
                 if (!st.push(a_near ? cb : ca)) *overflow += 1;
                 cur = a_near ? ca : cb;
-                continue;
-            } else if (ha) {
-                cur = ca;
-                continue;
-            } else if (hb) {
-                cur = cb;
+                next = true;
+            } else if (ha || hb) {
+                cur = ha ? ca : cb;
+                next = true;
+            }
+        } else if (in_blas) {
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t ti = first + k;
+                *tests += 1;
+                float t, u, v;
+                if (tri_test(sc.edges, ti, o, d, t_min, closest, &t, &u, &v)) {
+                    if (ANY_HIT) {
+                        st.truncate(base);
+                        return true;
+                    }
+                    closest = t;
+                    *t_out = t; *u_out = u; *v_out = v; *tri_out = ti; *inst_out = inst;
+                    found = true;
+                }
+            }
+        } else {
+            // instances first .. first + count - 1, in order: the later ones wait
+            for (uint32_t k = count - 1; k >= 1; --k)
+                if (!st.push(INST | (first + k))) *overflow += 1;
+            enter(first);
+            next = true;
+        }
+        if (next) {
+            if (++(in_blas ? blas_iter : tlas_iter) >= cap) break;
+            continue;
+        }
+        // pop the next node, crossing RETURN markers and entering waiting instances
+        bool done = false;
+        for (;;) {
+            if (st.sp == base) {
+                done = true;
+                break;
+            }
+            const uint32_t e = st.pop();
+            if (e == RETURN) {
+                in_blas = false;
+                o = o_w;
+                d = d_w;
+                inv = inv_w;
                 continue;
             }
+            if (e & INST) {
+                enter(e & ~INST);
+            } else {
+                cur = e;
+            }
+            break;
         }
-        if (st.sp == 0) break;
-        cur = st.pop();
+        if (done || ++(in_blas ? blas_iter : tlas_iter) >= cap) break;
     }
+    st.truncate(base);
     return found;
 }
 
@@ -650,7 +709,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJ
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (i < count) {
-      RayStack st = make_stack(stack_mem, p.spill, MODE == TRAV_WIDE ? WIDE_STACK : STACK);
+      RayStack st = make_stack(stack_mem, p.spill, MODE == TRAV_BINARY ? STACK : WIDE_STACK);
       const uint32_t pid = queue[i];
       const uint32_t jid = BATCH ? p.job[pid] : 0u;
       for (;;) {
@@ -949,7 +1008,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (i < count) {
-      RayStack st = make_stack(stack_mem, p.spill, MODE == TRAV_WIDE ? WIDE_STACK : STACK);
+      RayStack st = make_stack(stack_mem, p.spill, MODE == TRAV_BINARY ? STACK : WIDE_STACK);
       const float4 a = p.sro[i], b = p.srd[i];
       const uint32_t jid = BATCH ? p.job[__float_as_uint(b.w)] : 0u;
       for (;;) {
@@ -1005,7 +1064,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* _
     const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
     if (i >= n) return;
     const bool use_wide = wide && sc.wide && !sc.num_instances;
-    RayStack st = make_stack(stack_mem, spill, use_wide ? WIDE_STACK : STACK);
+    RayStack st = make_stack(stack_mem, spill, (use_wide || sc.num_instances) ? WIDE_STACK : STACK);
     const float4 a = rays[2 * i], b = rays[2 * i + 1];
     const V3 o = f4xyz(a), d = f4xyz(b);
     float best_t = b.w;
