@@ -618,60 +618,11 @@ __device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 
         inst = ii;
         blas_iter = 0;
     };
-    for (;;) {
-        *visits += 1;
-        const uint32_t first = cur >> COUNT_BITS, count = cur & COUNT_MASK;
-        bool next = false;  // cur holds the next node to visit
-        if (count == 0) {
-            const float4* cn = reinterpret_cast<const float4*>(sc.nodes + first);
-            const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
-            float ea, eb;
-            const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
-            const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
-            const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
-            const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
-            if (ha && hb) {
-                const bool a_near = ea <= eb;
-                if (!st.push(a_near ? cb : ca)) *overflow += 1;
-                cur = a_near ? ca : cb;
-                next = true;
-            } else if (ha || hb) {
-                cur = ha ? ca : cb;
-                next = true;
-            }
-        } else if (in_blas) {
-            for (uint32_t k = 0; k < count; ++k) {
-                const uint32_t ti = first + k;
-                *tests += 1;
-                float t, u, v;
-                if (tri_test(sc.edges, ti, o, d, t_min, closest, &t, &u, &v)) {
-                    if (ANY_HIT) {
-                        st.truncate(base);
-                        return true;
-                    }
-                    closest = t;
-                    *t_out = t; *u_out = u; *v_out = v; *tri_out = ti; *inst_out = inst;
-                    found = true;
-                }
-            }
-        } else {
-            // instances first .. first + count - 1, in order: the later ones wait
-            for (uint32_t k = count - 1; k >= 1; --k)
-                if (!st.push(INST | (first + k))) *overflow += 1;
-            enter(first);
-            next = true;
-        }
-        if (next) {
-            if (++(in_blas ? blas_iter : tlas_iter) >= cap) break;
-            continue;
-        }
-        // pop the next node, crossing RETURN markers and entering waiting instances
-        bool done = false;
+    // Pops the next node into cur, crossing RETURN markers and entering
+    // waiting instances; false when the walk is over.
+    auto pop_next = [&]() -> bool {
         for (;;) {
-            if (st.sp == base) {
-                done = true;
-                break;
-            }
+            if (st.sp == base) return false;
             const uint32_t e = st.pop();
             if (e == RETURN) {
                 in_blas = false;
@@ -680,14 +631,86 @@ __device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 
                 inv = inv_w;
                 continue;
             }
-            if (e & INST) {
-                enter(e & ~INST);
-            } else {
-                cur = e;
-            }
-            break;
+            if (e & INST) enter(e & ~INST);
+            else cur = e;
+            return ++(in_blas ? blas_iter : tlas_iter) < cap;
         }
-        if (done || ++(in_blas ? blas_iter : tlas_iter) >= cap) break;
+    };
+    // Visits the inner node cur; false when neither child is hit.
+    auto inner = [&]() -> bool {
+        const float4* cn = reinterpret_cast<const float4*>(sc.nodes + (cur >> COUNT_BITS));
+        const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
+        float ea, eb;
+        const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
+        const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
+        const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
+        const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
+        if (ha && hb) {
+            const bool a_near = ea <= eb;
+            if (!st.push(a_near ? cb : ca)) *overflow += 1;
+            cur = a_near ? ca : cb;
+            return true;
+        }
+        if (ha || hb) {
+            cur = ha ? ca : cb;
+            return true;
+        }
+        return false;
+    };
+    // The leaf cur: a BLAS leaf's triangles, or a TLAS leaf's instances
+    // (the first entered, the later ones waiting). True for an any-hit.
+    auto leaf = [&]() -> bool {
+        const uint32_t first = cur >> COUNT_BITS, count = cur & COUNT_MASK;
+        if (in_blas) {
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t ti = first + k;
+                *tests += 1;
+                float t, u, v;
+                if (tri_test(sc.edges, ti, o, d, t_min, closest, &t, &u, &v)) {
+                    if (ANY_HIT) return true;
+                    closest = t;
+                    *t_out = t; *u_out = u; *v_out = v; *tri_out = ti; *inst_out = inst;
+                    found = true;
+                }
+            }
+            return false;
+        }
+        for (uint32_t k = count - 1; k >= 1; --k)
+            if (!st.push(INST | (first + k))) *overflow += 1;
+        enter(first);
+        return false;
+    };
+    if (ANY_HIT) {
+        // one visit, of either kind, per step
+        for (;;) {
+            *visits += 1;
+            bool more;
+            if ((cur & COUNT_MASK) == 0) {
+                more = inner() ? ++(in_blas ? blas_iter : tlas_iter) < cap : pop_next();
+            } else {
+                const bool was_blas = in_blas;
+                if (leaf()) {
+                    st.truncate(base);
+                    return true;
+                }
+                more = was_blas ? pop_next() : ++blas_iter < cap;
+            }
+            if (!more) break;
+        }
+    } else {
+        // closest hit: inner nodes of either level until a leaf, then the leaf
+        bool more = true;
+        while (more) {
+            while (more && (cur & COUNT_MASK) == 0) {
+                *visits += 1;
+                more = inner() ? ++(in_blas ? blas_iter : tlas_iter) < cap : pop_next();
+            }
+            if (!more) break;
+            *visits += 1;
+            const bool was_blas = in_blas;
+            leaf();
+            more = was_blas ? pop_next() : ++blas_iter < cap;
+        }
     }
     st.truncate(base);
     return found;
