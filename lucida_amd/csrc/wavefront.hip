@@ -40,6 +40,7 @@ using namespace dev;
 constexpr uint32_t MISS = 0xFFFFFFFFu;
 constexpr uint32_t ANALYTIC = 0x80000000u;
 constexpr int WG = 64;          // one wave per workgroup for the per-path kernels
+constexpr int SHADE_WG = 256;   // SHADE: four waves, one queue atomic per block
 constexpr int TRACE_WG = 128;   // traversal workgroups (two waves)
 constexpr int STACK = 32;       // traversal stack entries per lane (shaders/common/bvh.wgsl:133)
 constexpr int LDS_STACK = 16;   // the newest entries, in LDS; older ones spill to memory
@@ -132,6 +133,31 @@ __device__ inline uint32_t wave_append(bool want, uint32_t* counter) {
     if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(counter, total);
     base = __shfl(base, leader, 64);
     return base + (uint32_t)__popcll(m & lanemask_lt());
+}
+
+// The same for a whole block of NW waves: one atomic per block (the queue
+// counters are single addresses that every block of a launch hits). Slots
+// are handed out in wave order, then lane order. Every thread of the block
+// must call it; `lds` holds NW + 1 words.
+template <int NW>
+__device__ inline uint32_t block_append(bool want, uint32_t* counter, uint32_t* lds) {
+    const uint64_t m = __ballot(want);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) lds[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = lds[w];
+            lds[w] = total;
+            total += c;
+        }
+        lds[NW] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = lds[NW] + lds[wave] + (uint32_t)__popcll(m & lanemask_lt());
+    __syncthreads();  // lds is reused by the next call
+    return slot;
 }
 
 __device__ inline uint64_t wave_sum_u32(uint32_t v) {
@@ -782,12 +808,12 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJ
 // SHADE (shade.wgsl:120-348 = integrator.rs radiance_with_guide, one bounce).
 // ---------------------------------------------------------------------------
 template <bool BATCH>
-__global__ __launch_bounds__(WG) void k_shade(SceneView sc0, const GpuUniforms* __restrict__ U,
+__global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUniforms* __restrict__ U,
                                               const BatchJob* __restrict__ jobs, Planes p,
                                               const uint32_t* __restrict__ queue_in,
                                               const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ queue_out,
                                               uint32_t* __restrict__ next_count, uint32_t* __restrict__ shadow_count) {
-    const uint32_t i = blockIdx.x * WG + threadIdx.x;
+    const uint32_t i = blockIdx.x * SHADE_WG + threadIdx.x;
     const uint32_t count = *count_ptr;
     bool alive = false, shadow = false;
     uint32_t pid = 0;
@@ -1007,14 +1033,15 @@ __global__ __launch_bounds__(WG) void k_shade(SceneView sc0, const GpuUniforms* 
         break;
       }
     }
-    // Compaction: one atomic per wave for each queue.
-    const uint32_t sslot = wave_append(shadow, shadow_count);
+    // Compaction: one atomic per block for each queue.
+    __shared__ uint32_t append_lds[SHADE_WG / 64 + 1];
+    const uint32_t sslot = block_append<SHADE_WG / 64>(shadow, shadow_count, append_lds);
     if (shadow) {
         p.sro[sslot] = make_float4(sr_o.x, sr_o.y, sr_o.z, sr_t);
         p.srd[sslot] = make_float4(sr_d.x, sr_d.y, sr_d.z, __uint_as_float(pid));
         p.src[sslot] = make_float4(sr_c.x, sr_c.y, sr_c.z, 0.0f);
     }
-    const uint32_t qslot = wave_append(alive, next_count);
+    const uint32_t qslot = block_append<SHADE_WG / 64>(alive, next_count, append_lds);
     if (alive) queue_out[qslot] = pid;
 }
 
@@ -1469,7 +1496,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
                                  stats_on);
             mark(1, false);
             mark(2, true);
-            hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, WG)), dim3(WG), 0, s, sc, dU, nullptr, p, qin,
+            hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, dU, nullptr, p, qin,
                                p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
             mark(2, false);
             mark(3, true);
@@ -1703,7 +1730,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                     uint32_t* qout = (b & 1) ? p.q0 : p.q1;
                     launch_extend<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p, qin,
                                          p.counters + b, stats_on);
-                    hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, WG)), dim3(WG), 0, s, sc, uj, nullptr, p, qin,
+                    hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, uj, nullptr, p, qin,
                                        p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
                     launch_connect<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p,
                                           p.counters + SHADOW_AT + b, stats_on);
@@ -1719,7 +1746,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 uint32_t* qin = (b & 1) ? p.q1 : p.q0;
                 uint32_t* qout = (b & 1) ? p.q0 : p.q1;
                 launch_extend<true>(mode, dim3(grid_for(P, TRACE_WG)), s, none, jr, p, qin, p.counters + b, stats_on);
-                hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, WG)), dim3(WG), 0, s, none, nullptr, jr, p, qin,
+                hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, none, nullptr, jr, p, qin,
                                    p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
                 launch_connect<true>(mode, dim3(grid_for(P, TRACE_WG)), s, none, jr, p, p.counters + SHADOW_AT + b,
                                      stats_on);
