@@ -122,23 +122,12 @@ __device__ inline uint64_t lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Wave-level append: every lane of the wave must call this; lanes with
-// `want` get consecutive slots from one atomic per wave.
-__device__ inline uint32_t wave_append(bool want, uint32_t* counter) {
-    const uint64_t m = __ballot(want);
-    if (m == 0) return 0;
-    const uint32_t total = (uint32_t)__popcll(m);
-    const int leader = __builtin_ctzll(m);
-    uint32_t base = 0;
-    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(counter, total);
-    base = __shfl(base, leader, 64);
-    return base + (uint32_t)__popcll(m & lanemask_lt());
-}
 
-// The same for a whole block of NW waves: one atomic per block (the queue
-// counters are single addresses that every block of a launch hits). Slots
-// are handed out in wave order, then lane order. Every thread of the block
-// must call it; `lds` holds NW + 1 words.
+// Queue append for a block of NW waves: each thread that wants a slot gets
+// one, with one atomic per block on the queue's counter (a single address
+// that every block of a launch hits; one atomic per wave made SHADE twice as
+// slow). Slots are handed out in wave order, then lane order. Every thread
+// of the block must call it; `lds` holds NW + 1 words.
 template <int NW>
 __device__ inline uint32_t block_append(bool want, uint32_t* counter, uint32_t* lds) {
     const uint64_t m = __ballot(want);
