@@ -94,9 +94,12 @@ lucida_status lucida_ctx_create(int device, lucida_ctx** out) {
     ctx->cu_count = prop.multiProcessorCount;
     ctx->name = prop.gcnArchName;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_shaded, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_connected, hipEventDisableTiming) != hipSuccess) {
         delete ctx;
-        return set_error(LUCIDA_NO_DEVICE, "could not create the context's stream or events");
+        return set_error(LUCIDA_NO_DEVICE, "could not create the context's streams or events");
     }
     *out = ctx;
     return LUCIDA_OK;
@@ -106,6 +109,7 @@ void lucida_ctx_destroy(lucida_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream2);
     DevBuf* bufs[] = {&ctx->lb_bounds, &ctx->lb_partial, &ctx->lb_codes, &ctx->lb_vals,   &ctx->lb_tmpk,
                       &ctx->lb_tmpv,   &ctx->lb_hist,    &ctx->lb_karras, &ctx->lb_parent, &ctx->lb_nodebox,
                       &ctx->lb_subtree, &ctx->lb_flags,  &ctx->lb_misc,  &ctx->lb_rank,   &ctx->lb_nodes,
@@ -115,12 +119,16 @@ void lucida_ctx_destroy(lucida_ctx* ctx) {
     for (DevBuf* b : bufs) b->release();
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    (void)hipEventDestroy(ctx->ev_shaded);
+    (void)hipEventDestroy(ctx->ev_connected);
+    (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
 lucida_status lucida_ctx_synchronize(lucida_ctx* ctx) {
     if (!ctx) return set_error(LUCIDA_INVALID_ARG, "lucida_ctx_synchronize: null ctx");
+    LUCIDA_HIP(hipStreamSynchronize(ctx->stream2), LUCIDA_VALIDATION);
     LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
     return LUCIDA_OK;

@@ -54,6 +54,10 @@ struct lucida_ctx {
     std::string name;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // A second stream (default priority, like the first) on which a render's
+    // CONNECT runs beside the next bounce's EXTEND, ordered by these events.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_shaded = nullptr, ev_connected = nullptr;
     // LBVH scratch, reused between builds.
     lucida::DevBuf lb_bounds, lb_partial, lb_codes, lb_vals, lb_tmpk, lb_tmpv, lb_hist, lb_karras, lb_parent,
         lb_nodebox, lb_subtree, lb_flags, lb_misc, lb_rank, lb_nodes, lb_tris, lb_pos, lb_prims, lb_level;

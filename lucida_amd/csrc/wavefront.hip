@@ -80,7 +80,8 @@ struct Planes {
     uint32_t* q1;
     uint32_t* counters;  // see SHADOW_AT / STATS_AT / TOTALS_AT
     uint32_t* job;       // job of each path (batched renders only)
-    uint32_t* spill;     // traversal stack spill columns (STACK - LDS_STACK per lane)
+    uint32_t* spill;     // traversal stack spill columns of EXTEND (WIDE_STACK - LDS_STACK per lane)
+    uint32_t* spill2;    // the same for CONNECT, which may run beside the next EXTEND
 };
 
 struct SceneView {
@@ -1047,7 +1048,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (i < count) {
-      RayStack st = make_stack(stack_mem, p.spill, MODE == TRAV_BINARY ? STACK : WIDE_STACK);
+      RayStack st = make_stack(stack_mem, p.spill2, MODE == TRAV_BINARY ? STACK : WIDE_STACK);
       const float4 a = p.sro[i], b = p.srd[i];
       const uint32_t jid = BATCH ? p.job[__float_as_uint(b.w)] : 0u;
       for (;;) {
@@ -1303,7 +1304,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
         LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * 3), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_queues.ensure(P * 4 * 3), LUCIDA_OOM);
         // spill columns for every lane of a traversal grid over P items
-        LUCIDA_HIP(ctx->wf_spill.ensure((P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_spill.ensure(2 * (P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
         ctx->wf_capacity = P;
     }
     LUCIDA_HIP(ctx->wf_counters.ensure(COUNTER_BYTES), LUCIDA_OOM);
@@ -1327,6 +1328,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     out->q1 = ctx->wf_queues.as<uint32_t>() + C;
     out->job = ctx->wf_queues.as<uint32_t>() + 2 * C;
     out->spill = ctx->wf_spill.as<uint32_t>();
+    out->spill2 = out->spill + (C + TRACE_WG) * (WIDE_STACK - LDS_STACK);
     out->counters = ctx->wf_counters.as<uint32_t>();
     *dU = ctx->wf_uniforms.as<GpuUniforms>();
     return LUCIDA_OK;
@@ -1443,6 +1445,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
                                                            : TRAV_BINARY;
 
     hipStream_t s = ctx->stream;
+    hipStream_t s2 = ctx->stream2;
     const uint32_t depth = uniforms->max_depth;
     const uint32_t stats_on = (stats && cfg.count_traversal) ? 1u : 0u;
     const bool profile = stats && cfg.profile_kernels;
@@ -1458,10 +1461,10 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         ev_kind.reserve(launches);
     }
     size_t ev_next = 0;
-    auto mark = [&](int kind, bool start) {
+    auto mark = [&](int kind, bool start, hipStream_t on) {
         if (!profile) return;
         if (start) ev_kind.push_back(kind);
-        (void)hipEventRecord(evs[ev_next++], s);
+        (void)hipEventRecord(evs[ev_next++], on);
     };
 
     LUCIDA_HIP(hipEventRecord(ctx->ev0, s), LUCIDA_VALIDATION);
@@ -1474,28 +1477,37 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         const size_t Pb = pixels * this_spb;
         LUCIDA_HIP(hipMemcpyAsync(dU, &u, sizeof u, hipMemcpyHostToDevice, s), LUCIDA_VALIDATION);
         hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT);
-        mark(0, true);
+        mark(0, true, s);
         hipLaunchKernelGGL(k_generate, dim3(grid_for(Pb, 256)), dim3(256), 0, s, dU, p);
-        mark(0, false);
+        mark(0, false, s);
+        // CONNECT of bounce b runs on the second stream beside EXTEND of
+        // bounce b + 1 (shadow planes and radiance against queues, rays and
+        // hits); SHADE of bounce b + 1 and RESOLVE wait for it (radiance, and
+        // the shadow planes SHADE rewrites).
         for (uint32_t b = 0; b < depth; ++b) {
             uint32_t* qin = (b & 1) ? p.q1 : p.q0;
             uint32_t* qout = (b & 1) ? p.q0 : p.q1;
-            mark(1, true);
+            mark(1, true, s);
             launch_extend<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s, sc, nullptr, p, qin, p.counters + b,
                                  stats_on);
-            mark(1, false);
-            mark(2, true);
-            hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, dU, nullptr, p, qin,
-                               p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
-            mark(2, false);
-            mark(3, true);
-            launch_connect<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s, sc, nullptr, p, p.counters + SHADOW_AT + b,
+            mark(1, false, s);
+            if (b > 0) LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_connected, 0), LUCIDA_VALIDATION);
+            mark(2, true, s);
+            hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, dU, nullptr, p,
+                               qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
+            mark(2, false, s);
+            LUCIDA_HIP(hipEventRecord(ctx->ev_shaded, s), LUCIDA_VALIDATION);
+            LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_shaded, 0), LUCIDA_VALIDATION);
+            mark(3, true, s2);
+            launch_connect<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s2, sc, nullptr, p, p.counters + SHADOW_AT + b,
                                   stats_on);
-            mark(3, false);
+            mark(3, false, s2);
+            LUCIDA_HIP(hipEventRecord(ctx->ev_connected, s2), LUCIDA_VALIDATION);
         }
-        mark(4, true);
+        if (depth > 0) LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_connected, 0), LUCIDA_VALIDATION);
+        mark(4, true, s);
         hipLaunchKernelGGL(k_resolve, dim3(grid_for(pixels, 256)), dim3(256), 0, s, dU, p, film->accum);
-        mark(4, false);
+        mark(4, false, s);
         hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, depth);
         LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
     }
