@@ -1702,6 +1702,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     LUCIDA_HIP(d_table.ensure(table.size() * sizeof(BatchJob)), LUCIDA_OOM);
     LUCIDA_HIP(d_stamps.ensure((rounds.size() + 1) * 8), LUCIDA_OOM);
     hipStream_t s = ctx->stream;
+    hipStream_t s2 = ctx->stream2;  // CONNECT, beside the next EXTEND (as in a single render)
     st = LUCIDA_OK;
     if (hipMemcpyAsync(d_table.ptr, table.data(), table.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s) !=
         hipSuccess)
@@ -1731,11 +1732,17 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                     uint32_t* qout = (b & 1) ? p.q0 : p.q1;
                     launch_extend<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p, qin,
                                          p.counters + b, stats_on);
-                    hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, uj, nullptr, p, qin,
-                                       p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
-                    launch_connect<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p,
+                    if (b > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
+                    hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, uj,
+                                       nullptr, p, qin, p.counters + b, qout, p.counters + b + 1,
+                                       p.counters + SHADOW_AT + b);
+                    (void)hipEventRecord(ctx->ev_shaded, s);
+                    (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
+                    launch_connect<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s2, sc, nullptr, p,
                                           p.counters + SHADOW_AT + b, stats_on);
+                    (void)hipEventRecord(ctx->ev_connected, s2);
                 }
+                if (r.depth > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
                 hipLaunchKernelGGL(k_resolve, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, uj, p,
                                    table[r.first].film);
                 hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
@@ -1747,11 +1754,16 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 uint32_t* qin = (b & 1) ? p.q1 : p.q0;
                 uint32_t* qout = (b & 1) ? p.q0 : p.q1;
                 launch_extend<true>(mode, dim3(grid_for(P, TRACE_WG)), s, none, jr, p, qin, p.counters + b, stats_on);
-                hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, none, nullptr, jr, p, qin,
-                                   p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
-                launch_connect<true>(mode, dim3(grid_for(P, TRACE_WG)), s, none, jr, p, p.counters + SHADOW_AT + b,
+                if (b > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
+                hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, none, nullptr, jr,
+                                   p, qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
+                (void)hipEventRecord(ctx->ev_shaded, s);
+                (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
+                launch_connect<true>(mode, dim3(grid_for(P, TRACE_WG)), s2, none, jr, p, p.counters + SHADOW_AT + b,
                                      stats_on);
+                (void)hipEventRecord(ctx->ev_connected, s2);
             }
+            if (r.depth > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
             hipLaunchKernelGGL(k_resolve_batch, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, jr, nj,
                                (uint32_t)r.pixels, p);
             hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
