@@ -420,6 +420,13 @@ def main() -> int:
     ext_ms = agg["extend"]
     dominant = max(agg, key=agg.get)
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    # CONNECT runs beside the next EXTEND on a second stream, so each launch's
+    # span includes time shared with the other walk. The two walks together:
+    # EXTEND's bytes plus CONNECT's (shadow record 48 + radiance 24 per ray,
+    # 64 per node visit and per triangle test) over the time the device is
+    # not in GENERATE, SHADE or RESOLVE (which run alone).
+    con_bytes = shadow_rays * (48 + 24) + node_bytes * (visits - ext_visits) + 64 * (tests - ext_tests)
+    walks_ms = device_ms - agg["generate"] - agg["shade"] - agg["resolve"]
 
     # a-trous on the final film (1024^2)
     den = L.Film(gpu, a.width, a.height)
@@ -457,6 +464,11 @@ def main() -> int:
             "node_visits_per_ray": round(visits / max(ext_rays + shadow_rays, 1), 2),
             "triangle_tests_per_ray": round(tests / max(ext_rays + shadow_rays, 1), 2),
             "kernel_ms_per_step": {k: round(v / a.steps, 3) for k, v in agg.items()},
+            "walks_combined": {"bytes_per_step": int((ext_bytes + con_bytes) / a.steps),
+                               "ms_per_step": round(walks_ms / a.steps, 3),
+                               "gbs": round((ext_bytes + con_bytes) / max(walks_ms * 1e-3, 1e-9) / 1e9, 1),
+                               "frac_of_8tbs": round((ext_bytes + con_bytes) / max(walks_ms * 1e-3, 1e-9) / 1e9
+                                                     / HBM_PEAK_GBS, 4)},
             "dominant_kernel": dominant,
             "device_ms_per_step": round(device_ms / a.steps, 3),
             "lbvh": lbvh,
