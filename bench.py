@@ -302,6 +302,8 @@ def main() -> int:
     ap.add_argument("--samples-per-batch", type=int, default=0)
     ap.add_argument("--bvh-width", type=int, default=0, choices=(0, 2, 4),
                     help="0/2: the binary walk (default), 4: the 4-wide walk")
+    ap.add_argument("--ray-order", type=int, default=0, choices=(0, 1, 2),
+                    help="extension-queue order: 0 automatic (sorted for this scene), 1 kept, 2 always sorted")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--relmse-ref-spp", type=int, default=4096)
@@ -375,7 +377,7 @@ def main() -> int:
     # Timed renders run as a caller would (no traversal counters; per-kernel
     # HIP events only); node visits and triangle tests come from one more,
     # untimed, counted render of the same (deterministic) step.
-    cfg = L.WavefrontConfig(a.samples_per_batch, 0, a.bvh_width, 0, 0, 1)
+    cfg = L.WavefrontConfig(a.samples_per_batch, 0, a.bvh_width, 0, 0, 1, a.ray_order)
 
     for _ in range(a.warmup):
         film.clear()
@@ -385,6 +387,7 @@ def main() -> int:
     agg = {k: 0.0 for k in L.RenderStats.KERNELS}
     launches = {k: 0 for k in L.RenderStats.KERNELS}
     device_ms = 0.0
+    sort_ms, sorts = 0.0, 0
     t0 = time.perf_counter()
     for _ in range(a.steps):
         film.clear()
@@ -393,6 +396,8 @@ def main() -> int:
             agg[k] += st.kernel_ms[i]
             launches[k] += st.kernel_launches[i]
         device_ms += st.elapsed_ms
+        sort_ms += st.queue_sort_ms
+        sorts += st.queue_sorts
         if st.stack_overflows:
             print(f"warning: {st.stack_overflows} traversal stack overflows", file=sys.stderr)
     gpu.check_errors()
@@ -400,7 +405,7 @@ def main() -> int:
     sec = (t1 - t0) / a.steps
     film.clear()
     cst = L.render_wavefront(gpu, scene, u, a.spp, film,
-                             L.WavefrontConfig(a.samples_per_batch, 0, a.bvh_width, 0, 1, 0))
+                             L.WavefrontConfig(a.samples_per_batch, 0, a.bvh_width, 0, 1, 0, a.ray_order))
     gpu.check_errors()
     n = a.steps  # every step traces the same rays
     ext_rays, shadow_rays = cst.extension_rays * n, cst.shadow_rays * n
@@ -424,7 +429,8 @@ def main() -> int:
     # span includes time shared with the other walk. The two walks together:
     # EXTEND's bytes plus CONNECT's (shadow record 48 + radiance 24 per ray,
     # 64 per node visit and per triangle test) over the time the device is
-    # not in GENERATE, SHADE or RESOLVE (which run alone).
+    # not in GENERATE, SHADE or RESOLVE (which run alone). The extension-queue
+    # sorts run beside CONNECT and stay inside this span.
     con_bytes = shadow_rays * (48 + 24) + node_bytes * (visits - ext_visits) + 64 * (tests - ext_tests)
     walks_ms = device_ms - agg["generate"] - agg["shade"] - agg["resolve"]
 
@@ -463,7 +469,9 @@ def main() -> int:
                                      "connect": round((visits - ext_visits) / max(con_slots, 1), 3)},
             "node_visits_per_ray": round(visits / max(ext_rays + shadow_rays, 1), 2),
             "triangle_tests_per_ray": round(tests / max(ext_rays + shadow_rays, 1), 2),
-            "kernel_ms_per_step": {k: round(v / a.steps, 3) for k, v in agg.items()},
+            "kernel_ms_per_step": {**{k: round(v / a.steps, 3) for k, v in agg.items()},
+                                   "queue_sort": round(sort_ms / a.steps, 3)},
+            "queue_sorts_per_step": sorts // a.steps,
             "walks_combined": {"bytes_per_step": int((ext_bytes + con_bytes) / a.steps),
                                "ms_per_step": round(walks_ms / a.steps, 3),
                                "gbs": round((ext_bytes + con_bytes) / max(walks_ms * 1e-3, 1e-9) / 1e9, 1),

@@ -451,6 +451,10 @@ typedef struct lucida_render_stats {
      * lanes a walk kept busy (its SIMD efficiency). */
     uint64_t extend_lane_slots;
     uint64_t connect_lane_slots;
+    /* With profile_kernels: device time and count of the extension-queue
+     * sorts (see lucida_wavefront_config.ray_order). */
+    float queue_sort_ms;
+    uint32_t queue_sorts;
 } lucida_render_stats;
 
 /* Tunables of the wavefront. Zero fields take the defaults. */
@@ -462,6 +466,12 @@ typedef struct lucida_wavefront_config {
     uint32_t use_graph;         /* batched renders: capture every round's launches in one hipGraph */
     uint32_t count_traversal;   /* keep node/triangle counters (costs a little)                   */
     uint32_t profile_kernels;   /* time every kernel with HIP events (single renders)             */
+    uint32_t ray_order;         /* 0: sort each bounce's extension queue by ray direction and      *
+                                 * origin when the scene's BVH has at least 4096 nodes and a batch *
+                                 * holds at least 2^16 paths; 1: never; 2: whenever there is a BVH.*
+                                 * The order changes which lanes trace which rays together, never  *
+                                 * a path's result: images are bit-identical either way.           *
+                                 * Rounds of a batched render that mix jobs keep queue order.      */
 } lucida_wavefront_config;
 
 /* wavefront::render_native_on (crates/gpu/src/wavefront.rs:134): renders

@@ -85,14 +85,18 @@ class RenderStats(C.Structure):
                 ("triangle_tests", C.c_uint64), ("extend_node_visits", C.c_uint64),
                 ("extend_triangle_tests", C.c_uint64), ("stack_overflows", C.c_uint64), ("elapsed_ms", C.c_float),
                 ("kernel_ms", C.c_float * 5), ("kernel_launches", C.c_uint32 * 5),
-                ("extend_lane_slots", C.c_uint64), ("connect_lane_slots", C.c_uint64)]
+                ("extend_lane_slots", C.c_uint64), ("connect_lane_slots", C.c_uint64),
+                ("queue_sort_ms", C.c_float), ("queue_sorts", C.c_uint32)]
 
     KERNELS = ("generate", "extend", "shade", "connect", "resolve")
 
 
 class WavefrontConfig(C.Structure):
     _fields_ = [("samples_per_batch", C.c_uint32), ("max_paths", C.c_uint32), ("bvh_width", C.c_uint32),
-                ("use_graph", C.c_uint32), ("count_traversal", C.c_uint32), ("profile_kernels", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("count_traversal", C.c_uint32), ("profile_kernels", C.c_uint32),
+                ("ray_order", C.c_uint32)]
+
+    RAY_ORDER_AUTO, RAY_ORDER_NEVER, RAY_ORDER_ALWAYS = 0, 1, 2
 
 
 class Job(C.Structure):

@@ -188,17 +188,22 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_morton(const LucidaAabb* __restr
 // input order, so equal keys keep their order.
 // ---------------------------------------------------------------------------
 
+// `count`, when given, is a device word that caps n (a queue whose length
+// only the device knows; the grid covers n).
 __global__ __launch_bounds__(SORT_BLOCK) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                           uint32_t shift, uint32_t* __restrict__ hist,
-                                                           uint32_t tiles) {
+                                                           const uint32_t* __restrict__ count, uint32_t shift,
+                                                           uint32_t* __restrict__ hist, uint32_t tiles) {
     __shared__ uint32_t h[RADIX];
     h[threadIdx.x] = 0;
+    if (count) n = min(n, *count);
     __syncthreads();
     const uint32_t base = blockIdx.x * SORT_TILE;
+    if (base < n) {
 #pragma unroll
-    for (int r = 0; r < SORT_ITEMS; ++r) {
-        uint32_t idx = base + r * SORT_BLOCK + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (RADIX - 1)], 1u);
+        for (int r = 0; r < SORT_ITEMS; ++r) {
+            uint32_t idx = base + r * SORT_BLOCK + threadIdx.x;
+            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (RADIX - 1)], 1u);
+        }
     }
     __syncthreads();
     hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
@@ -300,15 +305,18 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(uint32_t* __restrict__
 
 __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __restrict__ keys,
                                                               const uint32_t* __restrict__ vals, uint32_t n,
-                                                              uint32_t shift, const uint32_t* __restrict__ offsets,
-                                                              uint32_t tiles, uint32_t* __restrict__ keys_out,
+                                                              const uint32_t* __restrict__ count, uint32_t shift,
+                                                              const uint32_t* __restrict__ offsets, uint32_t tiles,
+                                                              uint32_t* __restrict__ keys_out,
                                                               uint32_t* __restrict__ vals_out) {
     __shared__ uint32_t wave_cnt[SORT_WAVES][RADIX];
     __shared__ uint32_t running[RADIX];
+    if (count) n = min(n, *count);
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    if (base >= n) return;  // the whole block: nothing of this tile is queued
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     running[threadIdx.x] = offsets[threadIdx.x * tiles + blockIdx.x];
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t base = blockIdx.x * SORT_TILE;
     for (int r = 0; r < SORT_ITEMS; ++r) {
 #pragma unroll
         for (int w = 0; w < SORT_WAVES; ++w) wave_cnt[w][threadIdx.x] = 0;
@@ -661,19 +669,55 @@ static lucida_status sort_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals
     const uint32_t passes = (bits + 7) / 8;
     for (uint32_t pass = 0; pass < passes; ++pass) {
         const uint32_t shift = pass * 8;
-        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_BLOCK), 0, ctx->stream, src_k, (uint32_t)n, shift,
-                           hist, tiles);
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_BLOCK), 0, ctx->stream, src_k, (uint32_t)n,
+                           nullptr, shift, hist, tiles);
         hipLaunchKernelGGL(k_scan_reduce, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, hist, hist_words, partials);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, partials, segs);
         hipLaunchKernelGGL(k_scan_down, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, hist, hist_words, partials);
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_BLOCK), 0, ctx->stream, src_k, src_v,
-                           (uint32_t)n, shift, hist, tiles, dst_k, dst_v);
+                           (uint32_t)n, nullptr, shift, hist, tiles, dst_k, dst_v);
         std::swap(src_k, dst_k);
         std::swap(src_v, dst_v);
     }
     if (src_k != keys) {  // odd number of passes: copy back
         LUCIDA_HIP(hipMemcpyAsync(keys, src_k, n * 4, hipMemcpyDeviceToDevice, ctx->stream), LUCIDA_VALIDATION);
         LUCIDA_HIP(hipMemcpyAsync(vals, src_v, n * 4, hipMemcpyDeviceToDevice, ctx->stream), LUCIDA_VALIDATION);
+    }
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    return LUCIDA_OK;
+}
+
+size_t sort_counted_scratch_words(size_t capacity) {
+    const size_t hist_words = blocks_for(capacity, SORT_TILE) * (size_t)RADIX;
+    return hist_words + blocks_for(hist_words, SCAN_SEG);
+}
+
+// Stable sort of (keys, vals)[0, *count) by key bits [0, bits) on `stream`,
+// bits a multiple of 16 (an even number of 8-bit passes, so the result ends
+// in keys/vals). The count stays on the device: the grid covers `capacity`,
+// and tiles past the count write empty histograms and return. Every launch
+// is a kernel (no memset or copy), so the sort may be captured in a graph.
+lucida_status sort_pairs_counted(hipStream_t stream, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
+                                 size_t capacity, const uint32_t* count, uint32_t bits, uint32_t* scratch) {
+    if (capacity == 0) return LUCIDA_OK;
+    if (bits == 0 || bits % 16 != 0 || bits > 32) return set_error(LUCIDA_INVALID_ARG, "sort: bits must be 16 or 32");
+    if (capacity > 0xFFFFFFFFull / 2) return set_error(LUCIDA_INVALID_ARG, "sort: more than 2^31 keys");
+    const uint32_t tiles = blocks_for(capacity, SORT_TILE);
+    const uint32_t hist_words = tiles * RADIX;
+    const uint32_t segs = blocks_for(hist_words, SCAN_SEG);
+    uint32_t* hist = scratch;
+    uint32_t* partials = scratch + hist_words;
+    uint32_t *src_k = keys, *src_v = vals, *dst_k = tk, *dst_v = tv;
+    for (uint32_t shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_BLOCK), 0, stream, src_k, (uint32_t)capacity, count,
+                           shift, hist, tiles);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(segs), dim3(SCAN_BLOCK), 0, stream, hist, hist_words, partials);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, stream, partials, segs);
+        hipLaunchKernelGGL(k_scan_down, dim3(segs), dim3(SCAN_BLOCK), 0, stream, hist, hist_words, partials);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_BLOCK), 0, stream, src_k, src_v,
+                           (uint32_t)capacity, count, shift, hist, tiles, dst_k, dst_v);
+        std::swap(src_k, dst_k);
+        std::swap(src_v, dst_v);
     }
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
     return LUCIDA_OK;

@@ -62,15 +62,25 @@ constexpr uint32_t TOTALS_AT = STATS_AT + STATS_STRIPES * 16;
 constexpr size_t COUNTER_BYTES = 4096;
 static_assert((TOTALS_AT + 24) * 4 <= COUNTER_BYTES, "counter words exceed their buffer");
 
-// Per-render device state. Planes are float4 arrays of P entries.
+// A path's state is one 128-byte record (seven float4 fields and a pad), so
+// a path read through a sorted queue, in no particular order, costs one
+// cache line instead of one line per field. PathField indexes one field of
+// every record: field[i] is path i's.
+constexpr size_t PATH_RECORD = 8;  // float4s per path
+struct PathField {
+    float4* base;
+    __device__ float4& operator[](size_t i) const { return base[i * PATH_RECORD]; }
+};
+
+// Per-render device state: path records and queue/shadow planes of P entries.
 struct Planes {
-    float4* org;   // origin.xyz, depth (uint bits)
-    float4* dir;   // direction.xyz, rng_state (uint bits)
-    float4* thr;   // throughput.xyz, prev_bsdf_pdf
-    float4* prv;   // prev_position.xyz, sampler_dim (uint bits)
-    float4* rad;   // radiance.xyz, unused
-    float4* gal;   // guide albedo.xyz, guide depth
-    float4* gnr;   // guide normal.xyz, unused
+    PathField org;  // origin.xyz, depth (uint bits)
+    PathField dir;  // direction.xyz, rng_state (uint bits)
+    PathField thr;  // throughput.xyz, prev_bsdf_pdf
+    PathField prv;  // prev_position.xyz, sampler_dim (uint bits)
+    PathField rad;  // radiance.xyz, unused
+    PathField gal;  // guide albedo.xyz, guide depth
+    PathField gnr;  // guide normal.xyz, unused
     float4* hit;   // t, u, v, primitive id (uint bits)
     uint32_t* hin; // instance of a triangle hit (instanced scenes only)
     float4* sro;   // shadow: origin.xyz, t_max
@@ -82,6 +92,10 @@ struct Planes {
     uint32_t* job;       // job of each path (batched renders only)
     uint32_t* spill;     // traversal stack spill columns of EXTEND (WIDE_STACK - LDS_STACK per lane)
     uint32_t* spill2;    // the same for CONNECT, which may run beside the next EXTEND
+    uint32_t* qkey;      // sort key of each extension-queue entry SHADE appends (ray_order_key)
+    uint32_t* qtmp;      // the queue sort's temporaries: entries and keys
+    uint32_t* ktmp;
+    uint32_t* sort_scratch;  // the queue sort's digit histograms and scan partials
 };
 
 struct SceneView {
@@ -794,19 +808,47 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJ
     flush_traversal_stats(p.counters, visits, tests, overflow, stats_on, 0);
 }
 
+// Sort key of an extension ray. Between bounces the queue is sorted by it, so
+// a wave walks rays that leave nearby points in similar directions: they
+// visit the same nodes in the same order, which keeps more of the wave's
+// lanes busy (the walks are bound by instruction issue over diverged lanes)
+// and shares node fetches. Three bits per direction component over [-1, 1]
+// and per origin coordinate over the scene box, interleaved from the most
+// significant bit in the order d.x d.y d.z o.x o.y o.z; the top 16 of the 18
+// bits are kept (two 8-bit radix passes).
+__device__ inline uint32_t ray_order_key(V3 o, V3 d, const GpuBvhNode& box) {
+    auto cell = [](float x) { return (uint32_t)fminf(fmaxf(x, 0.0f), 7.0f); };
+    auto ocell = [&](float x, float lo, float hi) { return cell((x - lo) * (8.0f / fmaxf(hi - lo, 1.0e-30f))); };
+    const uint32_t c[6] = {cell((d.x + 1.0f) * 4.0f),
+                           cell((d.y + 1.0f) * 4.0f),
+                           cell((d.z + 1.0f) * 4.0f),
+                           ocell(o.x, box.bounds_min[0], box.bounds_max[0]),
+                           ocell(o.y, box.bounds_min[1], box.bounds_max[1]),
+                           ocell(o.z, box.bounds_min[2], box.bounds_max[2])};
+    uint32_t k = 0;
+#pragma unroll
+    for (int b = 2; b >= 0; --b)
+#pragma unroll
+        for (int a = 0; a < 6; ++a) k = (k << 1) | ((c[a] >> b) & 1u);
+    return k >> 2;
+}
+
 // ---------------------------------------------------------------------------
 // SHADE (shade.wgsl:120-348 = integrator.rs radiance_with_guide, one bounce).
+// With `key_out`, each appended extension-queue entry gets its ray's sort key
+// at the same slot (the scene must have a BVH: its root box scales origins).
 // ---------------------------------------------------------------------------
 template <bool BATCH>
 __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUniforms* __restrict__ U,
                                               const BatchJob* __restrict__ jobs, Planes p,
                                               const uint32_t* __restrict__ queue_in,
                                               const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ queue_out,
-                                              uint32_t* __restrict__ next_count, uint32_t* __restrict__ shadow_count) {
+                                              uint32_t* __restrict__ next_count, uint32_t* __restrict__ shadow_count,
+                                              uint32_t* __restrict__ key_out) {
     const uint32_t i = blockIdx.x * SHADE_WG + threadIdx.x;
     const uint32_t count = *count_ptr;
     bool alive = false, shadow = false;
-    uint32_t pid = 0;
+    uint32_t pid = 0, key = 0;
     V3 sr_o{0, 0, 0}, sr_d{0, 0, 0}, sr_c{0, 0, 0};
     float sr_t = 0.0f;
     if (i < count) {
@@ -1015,6 +1057,7 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
                         p.dir[pid] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(s.state));
                         p.thr[pid] = make_float4(nt.x, nt.y, nt.z, bpdf);
                         p.prv[pid] = make_float4(position.x, position.y, position.z, __uint_as_float(s.dim));
+                        if (key_out) key = ray_order_key(no, nd, sc.nodes[sc.num_instances ? sc.tlas_root : 0]);
                     }
                 }
             }
@@ -1032,7 +1075,10 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
         p.src[sslot] = make_float4(sr_c.x, sr_c.y, sr_c.z, 0.0f);
     }
     const uint32_t qslot = block_append<SHADE_WG / 64>(alive, next_count, append_lds);
-    if (alive) queue_out[qslot] = pid;
+    if (alive) {
+        queue_out[qslot] = pid;
+        if (key_out) key_out[qslot] = key;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1299,10 +1345,11 @@ lucida_status ensure_ggx_table(lucida_ctx* ctx) {
 static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, GpuUniforms** dU) {
     if (P > ctx->wf_capacity) {
         LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
-        LUCIDA_HIP(ctx->wf_planes.ensure(P * 16 * 7), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_planes.ensure(P * 16 * PATH_RECORD), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_hits.ensure(P * 20), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * 3), LUCIDA_OOM);
-        LUCIDA_HIP(ctx->wf_queues.ensure(P * 4 * 3), LUCIDA_OOM);
+        // q0, q1, job, then the sort's keys and temporaries and its scratch
+        LUCIDA_HIP(ctx->wf_queues.ensure((P * 6 + sort_counted_scratch_words(P)) * 4), LUCIDA_OOM);
         // spill columns for every lane of a traversal grid over P items
         LUCIDA_HIP(ctx->wf_spill.ensure(2 * (P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
         ctx->wf_capacity = P;
@@ -1311,13 +1358,13 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     LUCIDA_HIP(ctx->wf_uniforms.ensure(sizeof(GpuUniforms) * 2), LUCIDA_OOM);
     float4* pl = ctx->wf_planes.as<float4>();
     const size_t C = ctx->wf_capacity;
-    out->org = pl;
-    out->dir = pl + C;
-    out->thr = pl + 2 * C;
-    out->prv = pl + 3 * C;
-    out->rad = pl + 4 * C;
-    out->gal = pl + 5 * C;
-    out->gnr = pl + 6 * C;
+    out->org = PathField{pl};
+    out->dir = PathField{pl + 1};
+    out->thr = PathField{pl + 2};
+    out->prv = PathField{pl + 3};
+    out->rad = PathField{pl + 4};
+    out->gal = PathField{pl + 5};
+    out->gnr = PathField{pl + 6};
     out->hit = ctx->wf_hits.as<float4>();
     out->hin = reinterpret_cast<uint32_t*>(ctx->wf_hits.as<float4>() + C);
     float4* sh = ctx->wf_shadow.as<float4>();
@@ -1327,6 +1374,10 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     out->q0 = ctx->wf_queues.as<uint32_t>();
     out->q1 = ctx->wf_queues.as<uint32_t>() + C;
     out->job = ctx->wf_queues.as<uint32_t>() + 2 * C;
+    out->qkey = ctx->wf_queues.as<uint32_t>() + 3 * C;
+    out->qtmp = ctx->wf_queues.as<uint32_t>() + 4 * C;
+    out->ktmp = ctx->wf_queues.as<uint32_t>() + 5 * C;
+    out->sort_scratch = ctx->wf_queues.as<uint32_t>() + 6 * C;
     out->spill = ctx->wf_spill.as<uint32_t>();
     out->spill2 = out->spill + (C + TRACE_WG) * (WIDE_STACK - LDS_STACK);
     out->counters = ctx->wf_counters.as<uint32_t>();
@@ -1365,6 +1416,16 @@ static void launch_connect(int mode, dim3 grid, hipStream_t s, const SceneView& 
 
 
 // The kernels' view of a resident scene.
+// Whether a render of `paths` paths per batch sorts its extension queues
+// (lucida_wavefront_config.ray_order): the sort costs two radix passes per
+// bounce, which pays off when the walks are long (a large tree) and the
+// queue is long enough to group similar rays.
+static bool want_ray_order(const lucida_wavefront_config& cfg, const SceneView& sc, size_t paths) {
+    if (sc.num_nodes == 0 || cfg.ray_order == 1) return false;
+    if (cfg.ray_order == 2) return true;
+    return sc.num_nodes >= 4096 && paths >= ((size_t)1 << 16);
+}
+
 static SceneView make_view(lucida_ctx* ctx, const lucida_scene* scene) {
     SceneView sc;
     sc.materials = scene->materials;
@@ -1447,6 +1508,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     hipStream_t s = ctx->stream;
     hipStream_t s2 = ctx->stream2;
     const uint32_t depth = uniforms->max_depth;
+    const bool sort_rays = want_ray_order(cfg, sc, pixels * spb);
     const uint32_t stats_on = (stats && cfg.count_traversal) ? 1u : 0u;
     const bool profile = stats && cfg.profile_kernels;
     const uint32_t batches = (samples + spb - 1) / spb;
@@ -1455,7 +1517,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     std::vector<hipEvent_t> evs;
     std::vector<int> ev_kind;
     if (profile) {
-        const size_t launches = (size_t)batches * (2 + 3 * depth);
+        const size_t launches = (size_t)batches * (2 + 4 * depth);
         evs.resize(2 * launches, nullptr);
         for (auto& e : evs) LUCIDA_HIP(hipEventCreate(&e), LUCIDA_VALIDATION);
         ev_kind.reserve(launches);
@@ -1492,10 +1554,18 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
                                  stats_on);
             mark(1, false, s);
             if (b > 0) LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_connected, 0), LUCIDA_VALIDATION);
+            const bool sort = sort_rays && b + 1 < depth;
             mark(2, true, s);
             hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, dU, nullptr, p,
-                               qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
+                               qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
+                               sort ? p.qkey : nullptr);
             mark(2, false, s);
+            if (sort) {  // before CONNECT starts: its long waves would hold the CUs the sort's short kernels need
+                mark(5, true, s);
+                st = sort_pairs_counted(s, p.qkey, qout, p.ktmp, p.qtmp, Pb, p.counters + b + 1, 16, p.sort_scratch);
+                if (st != LUCIDA_OK) return st;
+                mark(5, false, s);
+            }
             LUCIDA_HIP(hipEventRecord(ctx->ev_shaded, s), LUCIDA_VALIDATION);
             LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_shaded, 0), LUCIDA_VALIDATION);
             mark(3, true, s2);
@@ -1532,8 +1602,13 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         for (size_t k = 0; k < ev_kind.size(); ++k) {
             float t = 0.0f;
             LUCIDA_HIP(hipEventElapsedTime(&t, evs[2 * k], evs[2 * k + 1]), LUCIDA_VALIDATION);
-            stats->kernel_ms[ev_kind[k]] += t;
-            stats->kernel_launches[ev_kind[k]] += 1;
+            if (ev_kind[k] == 5) {
+                stats->queue_sort_ms += t;
+                stats->queue_sorts += 1;
+            } else {
+                stats->kernel_ms[ev_kind[k]] += t;
+                stats->kernel_launches[ev_kind[k]] += 1;
+            }
         }
     }
     for (auto& e : evs) (void)hipEventDestroy(e);
@@ -1619,6 +1694,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
         size_t first, count, paths, pixels;
         uint32_t depth;
         int solo_mode;  // -1: mixed round (batch kernels); else the traversal mode of its one chunk
+        bool sort;      // a solo round that sorts its extension queues (ray_order)
     };
     std::vector<Round> rounds;
     std::vector<BatchJob> table;
@@ -1664,7 +1740,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
         if (next[j] == chunks[j]) last_round[j] = (uint32_t)rounds.size();
     };
     for (;;) {
-        Round r{table.size(), 0, 0, 0, 0, -1};
+        Round r{table.size(), 0, 0, 0, 0, -1, false};
         for (size_t j = 0; j < n; ++j) {
             if (solo[j] || next[j] >= chunks[j]) continue;
             const uint32_t this_spb = std::min(spb[j], jobs[j].samples - next[j] * spb[j]);
@@ -1684,8 +1760,9 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     });
     for (size_t j : order) {
         while (next[j] < chunks[j]) {
-            Round r{table.size(), 0, 0, 0, 0, solo_mode[j]};
+            Round r{table.size(), 0, 0, 0, 0, solo_mode[j], false};
             add_chunk(j, r);
+            r.sort = want_ray_order(cfg, views[j], r.paths);
             if (r.paths >= 0x7FFFFFFFull / 4) return set_error(LUCIDA_INVALID_ARG, "a job's chunk exceeds 2^29 paths");
             rounds.push_back(r);
         }
@@ -1733,10 +1810,14 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                     launch_extend<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p, qin,
                                          p.counters + b, stats_on);
                     if (b > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
+                    const bool sort = r.sort && b + 1 < r.depth;
                     hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, uj,
                                        nullptr, p, qin, p.counters + b, qout, p.counters + b + 1,
-                                       p.counters + SHADOW_AT + b);
+                                       p.counters + SHADOW_AT + b, sort ? p.qkey : nullptr);
                     (void)hipEventRecord(ctx->ev_shaded, s);
+                    if (sort)
+                        (void)sort_pairs_counted(s, p.qkey, qout, p.ktmp, p.qtmp, P, p.counters + b + 1, 16,
+                                                 p.sort_scratch);
                     (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
                     launch_connect<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s2, sc, nullptr, p,
                                           p.counters + SHADOW_AT + b, stats_on);
@@ -1756,7 +1837,8 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 launch_extend<true>(mode, dim3(grid_for(P, TRACE_WG)), s, none, jr, p, qin, p.counters + b, stats_on);
                 if (b > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
                 hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, none, nullptr, jr,
-                                   p, qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b);
+                                   p, qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
+                                   nullptr);
                 (void)hipEventRecord(ctx->ev_shaded, s);
                 (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
                 launch_connect<true>(mode, dim3(grid_for(P, TRACE_WG)), s2, none, jr, p, p.counters + SHADOW_AT + b,
