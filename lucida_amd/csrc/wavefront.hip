@@ -62,15 +62,20 @@ constexpr uint32_t TOTALS_AT = STATS_AT + STATS_STRIPES * 16;
 constexpr size_t COUNTER_BYTES = 4096;
 static_assert((TOTALS_AT + 24) * 4 <= COUNTER_BYTES, "counter words exceed their buffer");
 
-// A path's state is one 128-byte record (seven float4 fields and a pad), so
-// a path read through a sorted queue, in no particular order, costs one
-// cache line instead of one line per field. PathField indexes one field of
-// every record: field[i] is path i's.
-constexpr size_t PATH_RECORD = 8;  // float4s per path
-struct PathField {
+// A path's state is one 128-byte record (seven float4 fields and a pad) and
+// a shadow ray one 64-byte record (three fields and a pad), so a record read
+// through a sorted queue, in no particular order, costs one cache line
+// instead of one line per field. A Field indexes one field of every record:
+// field[i] is record i's.
+constexpr size_t PATH_RECORD = 8;    // float4s per path
+constexpr size_t SHADOW_RECORD = 4;  // float4s per shadow ray
+template <size_t N>
+struct Field {
     float4* base;
-    __device__ float4& operator[](size_t i) const { return base[i * PATH_RECORD]; }
+    __device__ float4& operator[](size_t i) const { return base[i * N]; }
 };
+using PathField = Field<PATH_RECORD>;
+using ShadowField = Field<SHADOW_RECORD>;
 
 // Per-render device state: path records and queue/shadow planes of P entries.
 struct Planes {
@@ -83,9 +88,9 @@ struct Planes {
     PathField gnr;  // guide normal.xyz, unused
     float4* hit;   // t, u, v, primitive id (uint bits)
     uint32_t* hin; // instance of a triangle hit (instanced scenes only)
-    float4* sro;   // shadow: origin.xyz, t_max
-    float4* srd;   // shadow: direction.xyz, path (uint bits)
-    float4* src;   // shadow: contribution.xyz, unused
+    ShadowField sro;  // shadow: origin.xyz, t_max
+    ShadowField srd;  // shadow: direction.xyz, path (uint bits)
+    ShadowField src;  // shadow: contribution.xyz, unused
     uint32_t* q0;
     uint32_t* q1;
     uint32_t* counters;  // see SHADOW_AT / STATS_AT / TOTALS_AT
@@ -93,7 +98,9 @@ struct Planes {
     uint32_t* spill;     // traversal stack spill columns of EXTEND (WIDE_STACK - LDS_STACK per lane)
     uint32_t* spill2;    // the same for CONNECT, which may run beside the next EXTEND
     uint32_t* qkey;      // sort key of each extension-queue entry SHADE appends (ray_order_key)
-    uint32_t* qtmp;      // the queue sort's temporaries: entries and keys
+    uint32_t* skey;      // sort key of each shadow ray SHADE appends, and
+    uint32_t* sidx;      // its slot: after the sort, CONNECT's order over the shadow records
+    uint32_t* qtmp;      // the sorts' temporaries: entries and keys
     uint32_t* ktmp;
     uint32_t* sort_scratch;  // the queue sort's digit histograms and scan partials
 };
@@ -835,8 +842,10 @@ __device__ inline uint32_t ray_order_key(V3 o, V3 d, const GpuBvhNode& box) {
 
 // ---------------------------------------------------------------------------
 // SHADE (shade.wgsl:120-348 = integrator.rs radiance_with_guide, one bounce).
-// With `key_out`, each appended extension-queue entry gets its ray's sort key
-// at the same slot (the scene must have a BVH: its root box scales origins).
+// With `key_out` (single-scene launches only), each appended extension-queue
+// entry gets its ray's sort key at the same slot, and each shadow ray its key
+// and slot in skey / sidx (the scene must have a BVH: its root box scales
+// origins).
 // ---------------------------------------------------------------------------
 template <bool BATCH>
 __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUniforms* __restrict__ U,
@@ -1073,6 +1082,10 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
         p.sro[sslot] = make_float4(sr_o.x, sr_o.y, sr_o.z, sr_t);
         p.srd[sslot] = make_float4(sr_d.x, sr_d.y, sr_d.z, __uint_as_float(pid));
         p.src[sslot] = make_float4(sr_c.x, sr_c.y, sr_c.z, 0.0f);
+        if (!BATCH && key_out) {
+            p.skey[sslot] = ray_order_key(sr_o, sr_d, sc0.nodes[sc0.num_instances ? sc0.tlas_root : 0]);
+            p.sidx[sslot] = sslot;
+        }
     }
     const uint32_t qslot = block_append<SHADE_WG / 64>(alive, next_count, append_lds);
     if (alive) {
@@ -1088,12 +1101,14 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
 // ---------------------------------------------------------------------------
 template <bool BATCH, int MODE>
 __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const BatchJob* __restrict__ jobs, Planes p,
-                                                      const uint32_t* __restrict__ count_ptr, uint32_t stats_on) {
+                                                      const uint32_t* __restrict__ count_ptr,
+                                                      const uint32_t* __restrict__ order, uint32_t stats_on) {
     __shared__ uint32_t stack_mem[LDS_STACK * TRACE_WG];
-    const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
+    const uint32_t q = blockIdx.x * TRACE_WG + threadIdx.x;
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
-    if (i < count) {
+    if (q < count) {
+      const uint32_t i = order ? order[q] : q;  // the shadow record this lane traces
       RayStack st = make_stack(stack_mem, p.spill2, MODE == TRAV_BINARY ? STACK : WIDE_STACK);
       const float4 a = p.sro[i], b = p.srd[i];
       const uint32_t jid = BATCH ? p.job[__float_as_uint(b.w)] : 0u;
@@ -1347,9 +1362,9 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
         LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
         LUCIDA_HIP(ctx->wf_planes.ensure(P * 16 * PATH_RECORD), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_hits.ensure(P * 20), LUCIDA_OOM);
-        LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * 3), LUCIDA_OOM);
-        // q0, q1, job, then the sort's keys and temporaries and its scratch
-        LUCIDA_HIP(ctx->wf_queues.ensure((P * 6 + sort_counted_scratch_words(P)) * 4), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * SHADOW_RECORD), LUCIDA_OOM);
+        // q0, q1, job, the sorts' keys, orders and temporaries, then their scratch
+        LUCIDA_HIP(ctx->wf_queues.ensure((P * 8 + sort_counted_scratch_words(P)) * 4), LUCIDA_OOM);
         // spill columns for every lane of a traversal grid over P items
         LUCIDA_HIP(ctx->wf_spill.ensure(2 * (P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
         ctx->wf_capacity = P;
@@ -1368,16 +1383,18 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     out->hit = ctx->wf_hits.as<float4>();
     out->hin = reinterpret_cast<uint32_t*>(ctx->wf_hits.as<float4>() + C);
     float4* sh = ctx->wf_shadow.as<float4>();
-    out->sro = sh;
-    out->srd = sh + C;
-    out->src = sh + 2 * C;
+    out->sro = ShadowField{sh};
+    out->srd = ShadowField{sh + 1};
+    out->src = ShadowField{sh + 2};
     out->q0 = ctx->wf_queues.as<uint32_t>();
     out->q1 = ctx->wf_queues.as<uint32_t>() + C;
     out->job = ctx->wf_queues.as<uint32_t>() + 2 * C;
     out->qkey = ctx->wf_queues.as<uint32_t>() + 3 * C;
     out->qtmp = ctx->wf_queues.as<uint32_t>() + 4 * C;
     out->ktmp = ctx->wf_queues.as<uint32_t>() + 5 * C;
-    out->sort_scratch = ctx->wf_queues.as<uint32_t>() + 6 * C;
+    out->skey = ctx->wf_queues.as<uint32_t>() + 6 * C;
+    out->sidx = ctx->wf_queues.as<uint32_t>() + 7 * C;
+    out->sort_scratch = ctx->wf_queues.as<uint32_t>() + 8 * C;
     out->spill = ctx->wf_spill.as<uint32_t>();
     out->spill2 = out->spill + (C + TRACE_WG) * (WIDE_STACK - LDS_STACK);
     out->counters = ctx->wf_counters.as<uint32_t>();
@@ -1405,17 +1422,19 @@ static void launch_extend(int mode, dim3 grid, hipStream_t s, const SceneView& s
 
 template <bool BATCH>
 static void launch_connect(int mode, dim3 grid, hipStream_t s, const SceneView& sc, const BatchJob* jobs,
-                           const Planes& p, const uint32_t* count, uint32_t stats_on) {
+                           const Planes& p, const uint32_t* count, const uint32_t* order, uint32_t stats_on) {
     if (mode == TRAV_WIDE)
-        hipLaunchKernelGGL((k_connect<BATCH, TRAV_WIDE>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, stats_on);
+        hipLaunchKernelGGL((k_connect<BATCH, TRAV_WIDE>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, order,
+                           stats_on);
     else if (mode == TRAV_INSTANCED)
-        hipLaunchKernelGGL((k_connect<BATCH, TRAV_INSTANCED>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, stats_on);
+        hipLaunchKernelGGL((k_connect<BATCH, TRAV_INSTANCED>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, order,
+                           stats_on);
     else
-        hipLaunchKernelGGL((k_connect<BATCH, TRAV_BINARY>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, stats_on);
+        hipLaunchKernelGGL((k_connect<BATCH, TRAV_BINARY>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, order,
+                           stats_on);
 }
 
 
-// The kernels' view of a resident scene.
 // Whether a render of `paths` paths per batch sorts its extension queues
 // (lucida_wavefront_config.ray_order): the sort costs two radix passes per
 // bounce, which pays off when the walks are long (a large tree) and the
@@ -1426,6 +1445,20 @@ static bool want_ray_order(const lucida_wavefront_config& cfg, const SceneView& 
     return sc.num_nodes >= 4096 && paths >= ((size_t)1 << 16);
 }
 
+// The sorts after SHADE of bounce b over a batch of P paths: the shadow rays
+// (CONNECT reads them through sidx) and, unless this was the last bounce,
+// the next extension queue. Both run on `s`, one after the other, sharing
+// the temporaries.
+static lucida_status sort_queues(hipStream_t s, const Planes& p, uint32_t* qout, size_t P, uint32_t b, bool extend) {
+    if (extend) {
+        const lucida_status st =
+            sort_pairs_counted(s, p.qkey, qout, p.ktmp, p.qtmp, P, p.counters + b + 1, 16, p.sort_scratch);
+        if (st != LUCIDA_OK) return st;
+    }
+    return sort_pairs_counted(s, p.skey, p.sidx, p.ktmp, p.qtmp, P, p.counters + SHADOW_AT + b, 16, p.sort_scratch);
+}
+
+// The kernels' view of a resident scene.
 static SceneView make_view(lucida_ctx* ctx, const lucida_scene* scene) {
     SceneView sc;
     sc.materials = scene->materials;
@@ -1554,15 +1587,14 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
                                  stats_on);
             mark(1, false, s);
             if (b > 0) LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_connected, 0), LUCIDA_VALIDATION);
-            const bool sort = sort_rays && b + 1 < depth;
             mark(2, true, s);
             hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, dU, nullptr, p,
                                qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
-                               sort ? p.qkey : nullptr);
+                               sort_rays ? p.qkey : nullptr);
             mark(2, false, s);
-            if (sort) {  // before CONNECT starts: its long waves would hold the CUs the sort's short kernels need
+            if (sort_rays) {  // before CONNECT starts: its long waves would hold the CUs the sort's short kernels need
                 mark(5, true, s);
-                st = sort_pairs_counted(s, p.qkey, qout, p.ktmp, p.qtmp, Pb, p.counters + b + 1, 16, p.sort_scratch);
+                st = sort_queues(s, p, qout, Pb, b, b + 1 < depth);
                 if (st != LUCIDA_OK) return st;
                 mark(5, false, s);
             }
@@ -1570,7 +1602,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
             LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_shaded, 0), LUCIDA_VALIDATION);
             mark(3, true, s2);
             launch_connect<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s2, sc, nullptr, p, p.counters + SHADOW_AT + b,
-                                  stats_on);
+                                  sort_rays ? p.sidx : nullptr, stats_on);
             mark(3, false, s2);
             LUCIDA_HIP(hipEventRecord(ctx->ev_connected, s2), LUCIDA_VALIDATION);
         }
@@ -1810,17 +1842,14 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                     launch_extend<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p, qin,
                                          p.counters + b, stats_on);
                     if (b > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
-                    const bool sort = r.sort && b + 1 < r.depth;
                     hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, uj,
                                        nullptr, p, qin, p.counters + b, qout, p.counters + b + 1,
-                                       p.counters + SHADOW_AT + b, sort ? p.qkey : nullptr);
+                                       p.counters + SHADOW_AT + b, r.sort ? p.qkey : nullptr);
+                    if (r.sort) (void)sort_queues(s, p, qout, P, b, b + 1 < r.depth);
                     (void)hipEventRecord(ctx->ev_shaded, s);
-                    if (sort)
-                        (void)sort_pairs_counted(s, p.qkey, qout, p.ktmp, p.qtmp, P, p.counters + b + 1, 16,
-                                                 p.sort_scratch);
                     (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
                     launch_connect<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s2, sc, nullptr, p,
-                                          p.counters + SHADOW_AT + b, stats_on);
+                                          p.counters + SHADOW_AT + b, r.sort ? p.sidx : nullptr, stats_on);
                     (void)hipEventRecord(ctx->ev_connected, s2);
                 }
                 if (r.depth > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
@@ -1842,7 +1871,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 (void)hipEventRecord(ctx->ev_shaded, s);
                 (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
                 launch_connect<true>(mode, dim3(grid_for(P, TRACE_WG)), s2, none, jr, p, p.counters + SHADOW_AT + b,
-                                     stats_on);
+                                     nullptr, stats_on);
                 (void)hipEventRecord(ctx->ev_connected, s2);
             }
             if (r.depth > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);

@@ -189,9 +189,10 @@ def test_ray_order_does_not_change_the_image(gpu, oracle, name, width):
     np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
     for f in ("extension_rays", "shadow_rays", "node_visits", "triangle_tests", "stack_overflows"):
         assert getattr(sa, f) == getattr(sb, f), f
-    assert sa.queue_sorts == 0 and sb.queue_sorts == 7  # one batch of 6 spp: a sort after each SHADE but the last
-    say(f"closest-hit SIMD efficiency: kept {sa.extend_node_visits / sa.extend_lane_slots:.3f}, "
-        f"sorted {sb.extend_node_visits / sb.extend_lane_slots:.3f}")
+    assert sa.queue_sorts == 0 and sb.queue_sorts == 8  # one batch of 6 spp: the queues are sorted after every SHADE
+    eff = lambda st: (st.extend_node_visits / st.extend_lane_slots,
+                      (st.node_visits - st.extend_node_visits) / st.connect_lane_slots)
+    say("SIMD efficiency (closest hit, occlusion): kept %.3f %.3f, sorted %.3f %.3f" % (*eff(sa), *eff(sb)))
 
 
 def test_bvh_equals_brute_force(gpu, oracle):
