@@ -303,33 +303,47 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(uint32_t* __restrict__
     }
 }
 
+// The scatter stages its tile in LDS: items are ranked within the tile by
+// digit (ballots within a wave, per-digit prefixes over the block's waves and
+// rounds), placed in LDS in digit order, and then written out so that
+// consecutive threads write consecutive addresses of one digit's run (an
+// item at tile position i of digit d goes to the digit's global offset plus
+// i minus the digit's first tile position). Without the staging, the 256
+// lanes of a round write to up to 256 different runs.
 __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __restrict__ keys,
                                                               const uint32_t* __restrict__ vals, uint32_t n,
                                                               const uint32_t* __restrict__ count, uint32_t shift,
                                                               const uint32_t* __restrict__ offsets, uint32_t tiles,
                                                               uint32_t* __restrict__ keys_out,
                                                               uint32_t* __restrict__ vals_out) {
+    __shared__ uint32_t s_key[SORT_TILE];
+    __shared__ uint32_t s_val[SORT_TILE];
     __shared__ uint32_t wave_cnt[SORT_WAVES][RADIX];
-    __shared__ uint32_t running[RADIX];
+    __shared__ uint32_t running[RADIX];   // per digit: items of the tile ranked so far
+    __shared__ uint32_t out_base[RADIX];  // per digit: global offset minus first tile position
+    __shared__ uint32_t wsum[SORT_BLOCK / 64];
     if (count) n = min(n, *count);
     const uint32_t base = blockIdx.x * SORT_TILE;
     if (base >= n) return;  // the whole block: nothing of this tile is queued
+    const uint32_t in_tile = min((uint32_t)SORT_TILE, n - base);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    running[threadIdx.x] = offsets[threadIdx.x * tiles + blockIdx.x];
+    running[threadIdx.x] = 0;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t key[SORT_ITEMS], val[SORT_ITEMS], pos[SORT_ITEMS];
+#pragma unroll
     for (int r = 0; r < SORT_ITEMS; ++r) {
 #pragma unroll
         for (int w = 0; w < SORT_WAVES; ++w) wave_cnt[w][threadIdx.x] = 0;
         const uint32_t idx = base + r * SORT_BLOCK + threadIdx.x;
         const bool valid = idx < n;
-        uint32_t key = valid ? keys[idx] : 0u;
-        uint32_t val = valid ? vals[idx] : 0u;
-        uint32_t digit = (key >> shift) & (RADIX - 1);
+        key[r] = valid ? keys[idx] : 0u;
+        val[r] = valid ? vals[idx] : 0u;
+        const uint32_t digit = (key[r] >> shift) & (RADIX - 1);
         uint64_t same = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            bool bit = (digit >> b) & 1u;
-            uint64_t m = __ballot(bit);
+            const bool bit = (digit >> b) & 1u;
+            const uint64_t m = __ballot(bit);
             same &= bit ? m : ~m;
         }
         const uint32_t rank = __popcll(same & lt_mask);
@@ -340,19 +354,37 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __
             uint32_t acc = running[threadIdx.x];
 #pragma unroll
             for (int w = 0; w < SORT_WAVES; ++w) {
-                uint32_t t = wave_cnt[w][threadIdx.x];
+                const uint32_t t = wave_cnt[w][threadIdx.x];
                 wave_cnt[w][threadIdx.x] = acc;
                 acc += t;
             }
             running[threadIdx.x] = acc;
         }
         __syncthreads();
-        if (valid) {
-            uint32_t dst = wave_cnt[wave][digit] + rank;
-            keys_out[dst] = key;
-            vals_out[dst] = val;
-        }
+        pos[r] = valid ? wave_cnt[wave][digit] + rank : 0xFFFFFFFFu;  // rank among the tile's items of its digit
         __syncthreads();  // wave_cnt is read above before the next round clears it
+    }
+    // first tile position of each digit, and where its run goes
+    const uint32_t cnt = running[threadIdx.x];
+    uint32_t total;
+    const uint32_t first = block_inclusive_scan(cnt, wsum, &total) - cnt;
+    running[threadIdx.x] = first;
+    out_base[threadIdx.x] = offsets[threadIdx.x * tiles + blockIdx.x] - first;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        if (pos[r] != 0xFFFFFFFFu) {
+            const uint32_t at = running[(key[r] >> shift) & (RADIX - 1)] + pos[r];
+            s_key[at] = key[r];
+            s_val[at] = val[r];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < in_tile; i += SORT_BLOCK) {
+        const uint32_t k = s_key[i];
+        const uint32_t dst = out_base[(k >> shift) & (RADIX - 1)] + i;
+        if (keys_out) keys_out[dst] = k;
+        vals_out[dst] = s_val[i];
     }
 }
 
