@@ -470,8 +470,9 @@ typedef struct lucida_wavefront_config {
                                  * origin when the scene's BVH has at least 4096 nodes and a batch *
                                  * holds at least 2^16 paths; 1: never; 2: whenever there is a BVH.*
                                  * The order changes which lanes trace which rays together, never  *
-                                 * a path's result: images are bit-identical either way.           *
-                                 * Rounds of a batched render that mix jobs keep queue order.      */
+                                 * a path's result: images are bit-identical either way. Rounds of *
+                                 * a batched render that mix jobs sort by (job, ray), up to 256    *
+                                 * jobs per round.                                                 */
 } lucida_wavefront_config;
 
 /* wavefront::render_native_on (crates/gpu/src/wavefront.rs:134): renders

@@ -110,13 +110,15 @@ lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t
 // Exclusive prefix sum of device words in place (lbvh.hip); `partials` holds
 // ceil(count / 2048) words.
 lucida_status scan_u32_device(lucida_ctx* ctx, uint32_t* data, uint32_t count, uint32_t* partials);
-// Stable radix sort of (keys, vals)[0, *count) by the low `bits` (16 or 32)
-// key bits on `stream` (lbvh.hip); the count is a device word at most
-// `capacity`; tk/tv are temporaries of `capacity` words and `scratch` holds
-// sort_counted_scratch_words(capacity) words.
+// Stable radix sort of (keys, vals)[0, *count) by the low `bits` (a multiple
+// of 8) key bits on `stream` (lbvh.hip); the count is a device word at most
+// `capacity`; tk/tv are temporaries of `capacity` words, `scratch` holds
+// sort_counted_scratch_words(capacity) words, and *sorted_vals receives vals
+// or tv, whichever holds the result (tv after an odd number of 8-bit passes).
 size_t sort_counted_scratch_words(size_t capacity);
 lucida_status sort_pairs_counted(hipStream_t stream, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
-                                 size_t capacity, const uint32_t* count, uint32_t bits, uint32_t* scratch);
+                                 size_t capacity, const uint32_t* count, uint32_t bits, uint32_t* scratch,
+                                 uint32_t** sorted_vals);
 // Builds scene->wide (the 4-wide BVH traversed by default) from its binary
 // BVH; a no-op for instanced scenes and scenes without a BVH.
 lucida_status build_wide_bvh(lucida_ctx* ctx, lucida_scene* scene);

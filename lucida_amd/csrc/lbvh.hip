@@ -725,14 +725,19 @@ size_t sort_counted_scratch_words(size_t capacity) {
 }
 
 // Stable sort of (keys, vals)[0, *count) by key bits [0, bits) on `stream`,
-// bits a multiple of 16 (an even number of 8-bit passes, so the result ends
-// in keys/vals). The count stays on the device: the grid covers `capacity`,
-// and tiles past the count write empty histograms and return. Every launch
-// is a kernel (no memset or copy), so the sort may be captured in a graph.
+// bits a multiple of 8 (one pass per 8 bits). The count stays on the device:
+// the grid covers `capacity`, and tiles past the count write empty
+// histograms and return. Passes alternate between (keys, vals) and (tk, tv):
+// after an even number the result is in keys/vals, after an odd number in
+// tk/tv; *sorted_vals (when given) receives the array holding the sorted
+// values. Every launch is a kernel (no memset or copy), so the sort may be
+// captured in a graph.
 lucida_status sort_pairs_counted(hipStream_t stream, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
-                                 size_t capacity, const uint32_t* count, uint32_t bits, uint32_t* scratch) {
+                                 size_t capacity, const uint32_t* count, uint32_t bits, uint32_t* scratch,
+                                 uint32_t** sorted_vals) {
+    if (sorted_vals) *sorted_vals = vals;
     if (capacity == 0) return LUCIDA_OK;
-    if (bits == 0 || bits % 16 != 0 || bits > 32) return set_error(LUCIDA_INVALID_ARG, "sort: bits must be 16 or 32");
+    if (bits == 0 || bits % 8 != 0 || bits > 32) return set_error(LUCIDA_INVALID_ARG, "sort: bits must be 8, 16, 24 or 32");
     if (capacity > 0xFFFFFFFFull / 2) return set_error(LUCIDA_INVALID_ARG, "sort: more than 2^31 keys");
     const uint32_t tiles = blocks_for(capacity, SORT_TILE);
     const uint32_t hist_words = tiles * RADIX;
@@ -746,11 +751,13 @@ lucida_status sort_pairs_counted(hipStream_t stream, uint32_t* keys, uint32_t* v
         hipLaunchKernelGGL(k_scan_reduce, dim3(segs), dim3(SCAN_BLOCK), 0, stream, hist, hist_words, partials);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, stream, partials, segs);
         hipLaunchKernelGGL(k_scan_down, dim3(segs), dim3(SCAN_BLOCK), 0, stream, hist, hist_words, partials);
+        // the last pass needs only the values
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_BLOCK), 0, stream, src_k, src_v,
-                           (uint32_t)capacity, count, shift, hist, tiles, dst_k, dst_v);
+                           (uint32_t)capacity, count, shift, hist, tiles, shift + 8 < bits ? dst_k : nullptr, dst_v);
         std::swap(src_k, dst_k);
         std::swap(src_v, dst_v);
     }
+    if (sorted_vals) *sorted_vals = src_v;
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
     return LUCIDA_OK;
 }

@@ -100,8 +100,10 @@ struct Planes {
     uint32_t* qkey;      // sort key of each extension-queue entry SHADE appends (ray_order_key)
     uint32_t* skey;      // sort key of each shadow ray SHADE appends, and
     uint32_t* sidx;      // its slot: after the sort, CONNECT's order over the shadow records
-    uint32_t* qtmp;      // the sorts' temporaries: entries and keys
-    uint32_t* ktmp;
+    uint32_t* qtmp;      // the extension sort's temporaries: entries and keys (after an
+    uint32_t* ktmp;      // odd number of 8-bit passes, qtmp holds the sorted queue)
+    uint32_t* qtmp2;     // the same for the shadow sort
+    uint32_t* ktmp2;
     uint32_t* sort_scratch;  // the queue sort's digit histograms and scan partials
 };
 
@@ -840,12 +842,19 @@ __device__ inline uint32_t ray_order_key(V3 o, V3 d, const GpuBvhNode& box) {
     return k >> 2;
 }
 
+// Queue sort key of a ray of job `job` of a round (0 outside batched
+// renders): the job above the ray's key, so a sorted round keeps each job's
+// rays together (a wave of one job walks one tree).
+__device__ inline uint32_t queue_key(const SceneView& sc, uint32_t job, V3 o, V3 d) {
+    const uint32_t k = sc.num_nodes ? ray_order_key(o, d, sc.nodes[sc.num_instances ? sc.tlas_root : 0]) : 0u;
+    return (job << 16) | k;
+}
+
 // ---------------------------------------------------------------------------
 // SHADE (shade.wgsl:120-348 = integrator.rs radiance_with_guide, one bounce).
-// With `key_out` (single-scene launches only), each appended extension-queue
-// entry gets its ray's sort key at the same slot, and each shadow ray its key
-// and slot in skey / sidx (the scene must have a BVH: its root box scales
-// origins).
+// With `key_out`, each appended extension-queue entry gets its ray's sort key
+// (queue_key) at the same slot, and each shadow ray its key and slot in
+// skey / sidx.
 // ---------------------------------------------------------------------------
 template <bool BATCH>
 __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUniforms* __restrict__ U,
@@ -857,7 +866,7 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
     const uint32_t i = blockIdx.x * SHADE_WG + threadIdx.x;
     const uint32_t count = *count_ptr;
     bool alive = false, shadow = false;
-    uint32_t pid = 0, key = 0;
+    uint32_t pid = 0, key = 0, skey = 0;
     V3 sr_o{0, 0, 0}, sr_d{0, 0, 0}, sr_c{0, 0, 0};
     float sr_t = 0.0f;
     if (i < count) {
@@ -1046,6 +1055,7 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
                                     sr_t = distance * (1.0f - 1.0e-3f);
                                     sr_c = throughput * (f * fabsf(wi.z) * rin * (mis / pdf));
                                     shadow = true;
+                                    if (key_out) skey = queue_key(sc, j, sr_o, sr_d);
                                 }
                             }
                         }
@@ -1066,7 +1076,7 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
                         p.dir[pid] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(s.state));
                         p.thr[pid] = make_float4(nt.x, nt.y, nt.z, bpdf);
                         p.prv[pid] = make_float4(position.x, position.y, position.z, __uint_as_float(s.dim));
-                        if (key_out) key = ray_order_key(no, nd, sc.nodes[sc.num_instances ? sc.tlas_root : 0]);
+                        if (key_out) key = queue_key(sc, j, no, nd);
                     }
                 }
             }
@@ -1082,8 +1092,8 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
         p.sro[sslot] = make_float4(sr_o.x, sr_o.y, sr_o.z, sr_t);
         p.srd[sslot] = make_float4(sr_d.x, sr_d.y, sr_d.z, __uint_as_float(pid));
         p.src[sslot] = make_float4(sr_c.x, sr_c.y, sr_c.z, 0.0f);
-        if (!BATCH && key_out) {
-            p.skey[sslot] = ray_order_key(sr_o, sr_d, sc0.nodes[sc0.num_instances ? sc0.tlas_root : 0]);
+        if (key_out) {
+            p.skey[sslot] = skey;
             p.sidx[sslot] = sslot;
         }
     }
@@ -1364,7 +1374,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
         LUCIDA_HIP(ctx->wf_hits.ensure(P * 20), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * SHADOW_RECORD), LUCIDA_OOM);
         // q0, q1, job, the sorts' keys, orders and temporaries, then their scratch
-        LUCIDA_HIP(ctx->wf_queues.ensure((P * 8 + sort_counted_scratch_words(P)) * 4), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_queues.ensure((P * 10 + sort_counted_scratch_words(P)) * 4), LUCIDA_OOM);
         // spill columns for every lane of a traversal grid over P items
         LUCIDA_HIP(ctx->wf_spill.ensure(2 * (P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
         ctx->wf_capacity = P;
@@ -1394,7 +1404,9 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     out->ktmp = ctx->wf_queues.as<uint32_t>() + 5 * C;
     out->skey = ctx->wf_queues.as<uint32_t>() + 6 * C;
     out->sidx = ctx->wf_queues.as<uint32_t>() + 7 * C;
-    out->sort_scratch = ctx->wf_queues.as<uint32_t>() + 8 * C;
+    out->qtmp2 = ctx->wf_queues.as<uint32_t>() + 8 * C;
+    out->ktmp2 = ctx->wf_queues.as<uint32_t>() + 9 * C;
+    out->sort_scratch = ctx->wf_queues.as<uint32_t>() + 10 * C;
     out->spill = ctx->wf_spill.as<uint32_t>();
     out->spill2 = out->spill + (C + TRACE_WG) * (WIDE_STACK - LDS_STACK);
     out->counters = ctx->wf_counters.as<uint32_t>();
@@ -1445,17 +1457,21 @@ static bool want_ray_order(const lucida_wavefront_config& cfg, const SceneView& 
     return sc.num_nodes >= 4096 && paths >= ((size_t)1 << 16);
 }
 
-// The sorts after SHADE of bounce b over a batch of P paths: the shadow rays
-// (CONNECT reads them through sidx) and, unless this was the last bounce,
-// the next extension queue. Both run on `s`, one after the other, sharing
-// the temporaries.
-static lucida_status sort_queues(hipStream_t s, const Planes& p, uint32_t* qout, size_t P, uint32_t b, bool extend) {
+// The sorts after SHADE of bounce b over a batch of P paths, by `bits` key
+// bits: the shadow rays (*shadow_order receives the array of record slots in
+// CONNECT's order) and, unless this was the last bounce, the extension queue
+// SHADE wrote to `qraw` (*next_queue receives the sorted queue). Both run on
+// `s`, one after the other; they share the histogram scratch.
+static lucida_status sort_queues(hipStream_t s, const Planes& p, uint32_t* qraw, size_t P, uint32_t b, bool extend,
+                                 uint32_t bits, uint32_t** next_queue, uint32_t** shadow_order) {
+    *next_queue = qraw;
     if (extend) {
-        const lucida_status st =
-            sort_pairs_counted(s, p.qkey, qout, p.ktmp, p.qtmp, P, p.counters + b + 1, 16, p.sort_scratch);
+        const lucida_status st = sort_pairs_counted(s, p.qkey, qraw, p.ktmp, p.qtmp, P, p.counters + b + 1, bits,
+                                                    p.sort_scratch, next_queue);
         if (st != LUCIDA_OK) return st;
     }
-    return sort_pairs_counted(s, p.skey, p.sidx, p.ktmp, p.qtmp, P, p.counters + SHADOW_AT + b, 16, p.sort_scratch);
+    return sort_pairs_counted(s, p.skey, p.sidx, p.ktmp2, p.qtmp2, P, p.counters + SHADOW_AT + b, bits,
+                              p.sort_scratch, shadow_order);
 }
 
 // The kernels' view of a resident scene.
@@ -1579,9 +1595,11 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         // bounce b + 1 (shadow planes and radiance against queues, rays and
         // hits); SHADE of bounce b + 1 and RESOLVE wait for it (radiance, and
         // the shadow planes SHADE rewrites).
+        uint32_t* qin = p.q0;  // GENERATE's queue
         for (uint32_t b = 0; b < depth; ++b) {
-            uint32_t* qin = (b & 1) ? p.q1 : p.q0;
-            uint32_t* qout = (b & 1) ? p.q0 : p.q1;
+            uint32_t* qout = qin == p.q1 ? p.q0 : p.q1;  // SHADE's appends: any queue but the one it reads
+            uint32_t* qnext = qout;
+            uint32_t* order = nullptr;
             mark(1, true, s);
             launch_extend<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s, sc, nullptr, p, qin, p.counters + b,
                                  stats_on);
@@ -1594,7 +1612,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
             mark(2, false, s);
             if (sort_rays) {  // before CONNECT starts: its long waves would hold the CUs the sort's short kernels need
                 mark(5, true, s);
-                st = sort_queues(s, p, qout, Pb, b, b + 1 < depth);
+                st = sort_queues(s, p, qout, Pb, b, b + 1 < depth, 16, &qnext, &order);
                 if (st != LUCIDA_OK) return st;
                 mark(5, false, s);
             }
@@ -1602,9 +1620,10 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
             LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_shaded, 0), LUCIDA_VALIDATION);
             mark(3, true, s2);
             launch_connect<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s2, sc, nullptr, p, p.counters + SHADOW_AT + b,
-                                  sort_rays ? p.sidx : nullptr, stats_on);
+                                  order, stats_on);
             mark(3, false, s2);
             LUCIDA_HIP(hipEventRecord(ctx->ev_connected, s2), LUCIDA_VALIDATION);
+            qin = qnext;
         }
         if (depth > 0) LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_connected, 0), LUCIDA_VALIDATION);
         mark(4, true, s);
@@ -1781,6 +1800,9 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
         }
         if (r.count == 0) break;
         if (r.paths >= 0x7FFFFFFFull / 4) return set_error(LUCIDA_INVALID_ARG, "a job's chunk exceeds 2^29 paths");
+        // sorted when some job's tree qualifies and the job index fits the key's 8 bits
+        for (size_t k = r.first; k < r.first + r.count && r.count <= 256; ++k)
+            r.sort = r.sort || want_ray_order(cfg, table[k].sc, r.paths);
         rounds.push_back(r);
     }
     std::vector<size_t> order;
@@ -1836,21 +1858,24 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                     reinterpret_cast<const GpuUniforms*>(reinterpret_cast<const char*>(jr) + offsetof(BatchJob, u));
                 const SceneView& sc = table[r.first].sc;
                 hipLaunchKernelGGL(k_generate, dim3(grid_for(P, 256)), dim3(256), 0, s, uj, p);
+                uint32_t* qin = p.q0;
                 for (uint32_t b = 0; b < r.depth; ++b) {
-                    uint32_t* qin = (b & 1) ? p.q1 : p.q0;
-                    uint32_t* qout = (b & 1) ? p.q0 : p.q1;
+                    uint32_t* qout = qin == p.q1 ? p.q0 : p.q1;
+                    uint32_t* qnext = qout;
+                    uint32_t* order = nullptr;
                     launch_extend<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p, qin,
                                          p.counters + b, stats_on);
                     if (b > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
                     hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, uj,
                                        nullptr, p, qin, p.counters + b, qout, p.counters + b + 1,
                                        p.counters + SHADOW_AT + b, r.sort ? p.qkey : nullptr);
-                    if (r.sort) (void)sort_queues(s, p, qout, P, b, b + 1 < r.depth);
+                    if (r.sort) (void)sort_queues(s, p, qout, P, b, b + 1 < r.depth, 16, &qnext, &order);
                     (void)hipEventRecord(ctx->ev_shaded, s);
                     (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
                     launch_connect<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s2, sc, nullptr, p,
-                                          p.counters + SHADOW_AT + b, r.sort ? p.sidx : nullptr, stats_on);
+                                          p.counters + SHADOW_AT + b, order, stats_on);
                     (void)hipEventRecord(ctx->ev_connected, s2);
+                    qin = qnext;
                 }
                 if (r.depth > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
                 hipLaunchKernelGGL(k_resolve, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, uj, p,
@@ -1860,19 +1885,24 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 continue;
             }
             hipLaunchKernelGGL(k_generate_batch, dim3(grid_for(P, 256)), dim3(256), 0, s, jr, nj, P, p);
+            // a sorted mixed round orders by (job, ray key): 8 + 16 bits
+            uint32_t* qin = p.q0;
             for (uint32_t b = 0; b < r.depth; ++b) {
-                uint32_t* qin = (b & 1) ? p.q1 : p.q0;
-                uint32_t* qout = (b & 1) ? p.q0 : p.q1;
+                uint32_t* qout = qin == p.q1 ? p.q0 : p.q1;
+                uint32_t* qnext = qout;
+                uint32_t* order = nullptr;
                 launch_extend<true>(mode, dim3(grid_for(P, TRACE_WG)), s, none, jr, p, qin, p.counters + b, stats_on);
                 if (b > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
                 hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, none, nullptr, jr,
                                    p, qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
-                                   nullptr);
+                                   r.sort ? p.qkey : nullptr);
+                if (r.sort) (void)sort_queues(s, p, qout, P, b, b + 1 < r.depth, 24, &qnext, &order);
                 (void)hipEventRecord(ctx->ev_shaded, s);
                 (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
                 launch_connect<true>(mode, dim3(grid_for(P, TRACE_WG)), s2, none, jr, p, p.counters + SHADOW_AT + b,
-                                     nullptr, stats_on);
+                                     order, stats_on);
                 (void)hipEventRecord(ctx->ev_connected, s2);
+                qin = qnext;
             }
             if (r.depth > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
             hipLaunchKernelGGL(k_resolve_batch, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, jr, nj,

@@ -83,6 +83,24 @@ def test_batch_equals_solo_renders(gpu, scenes, use_graph, max_paths):
     say(f"rounds {st.kernel_launches[0]}, batch {st.elapsed_ms:.2f} ms, done {np.round(done, 2)}")
 
 
+def test_batch_ray_order_does_not_change_films(gpu, scenes):
+    """Sorted queues (by job, then ray) only regroup rays into waves: the films
+    of a batch are bit-identical with and without the sorts."""
+    say(f"batch case: {len(JOBS)} jobs, queue order kept vs sorted")
+    jobs = make_jobs(gpu, scenes)
+    out = []
+    for order in (L.WavefrontConfig.RAY_ORDER_NEVER, L.WavefrontConfig.RAY_ORDER_ALWAYS):
+        for _, _, _, film in jobs:
+            film.clear()
+        st, _ = L.render_batch(gpu, jobs, L.WavefrontConfig(0, 0, 0, 0, 1, 0, order))
+        out.append(([film.download() for _, _, _, film in jobs], st))
+    (kept, sk), (sorted_, ss) = out
+    for k, (a, b) in enumerate(zip(kept, sorted_)):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=f"job {k} {JOBS[k]}")
+    for f in ("extension_rays", "shadow_rays", "node_visits", "triangle_tests"):
+        assert getattr(sk, f) == getattr(ss, f), f
+
+
 def test_batch_accumulates_like_progressive_renders(gpu, scenes):
     """Two batches into the same films equal one render of all the samples."""
     blob, scene = scenes["box-c1"]
