@@ -62,20 +62,22 @@ constexpr uint32_t TOTALS_AT = STATS_AT + STATS_STRIPES * 16;
 constexpr size_t COUNTER_BYTES = 4096;
 static_assert((TOTALS_AT + 24) * 4 <= COUNTER_BYTES, "counter words exceed their buffer");
 
-// A path's state is one 128-byte record (seven float4 fields and a pad) and
-// a shadow ray one 64-byte record (three fields and a pad), so a record read
-// through a sorted queue, in no particular order, costs one cache line
-// instead of one line per field. A Field indexes one field of every record:
-// field[i] is record i's.
-constexpr size_t PATH_RECORD = 8;    // float4s per path
-constexpr size_t SHADOW_RECORD = 4;  // float4s per shadow ray
-template <size_t N>
+// Path and shadow-ray state has two layouts. Read in queue order (the queue
+// a compacted, mostly ascending list of paths), one array per field keeps a
+// wave's loads of a field contiguous. Read through sorted queues, in no
+// particular order, records do better: a path's seven float4 fields and a
+// pad in one 128-byte record, a shadow ray's three fields and a pad in 64
+// bytes, so a lane's reads share one cache line. A Field addresses one field
+// of every path (or shadow ray) in either layout: field[i] = base[i * stride].
+constexpr uint32_t PATH_RECORD = 8;    // float4s per path record
+constexpr uint32_t SHADOW_RECORD = 4;  // float4s per shadow record
 struct Field {
     float4* base;
-    __device__ float4& operator[](size_t i) const { return base[i * N]; }
+    uint32_t stride;  // 1: arrays of fields; else the record size in float4s
+    __device__ float4& operator[](size_t i) const { return base[i * stride]; }
 };
-using PathField = Field<PATH_RECORD>;
-using ShadowField = Field<SHADOW_RECORD>;
+using PathField = Field;
+using ShadowField = Field;
 
 // Per-render device state: path records and queue/shadow planes of P entries.
 struct Planes {
@@ -1366,8 +1368,23 @@ lucida_status ensure_ggx_table(lucida_ctx* ctx) {
     return LUCIDA_OK;
 }
 
-// Device buffers of the wavefront, grown on demand and kept by the context.
-static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, GpuUniforms** dU) {
+// Points the path and shadow fields into the context's buffers (capacity C)
+// as records or as arrays of fields.
+static void set_layout(lucida_ctx* ctx, Planes* out, bool records) {
+    const size_t C = ctx->wf_capacity;
+    float4* pl = ctx->wf_planes.as<float4>();
+    PathField* path[7] = {&out->org, &out->dir, &out->thr, &out->prv, &out->rad, &out->gal, &out->gnr};
+    for (int k = 0; k < 7; ++k)
+        *path[k] = records ? PathField{pl + k, PATH_RECORD} : PathField{pl + k * C, 1u};
+    float4* sh = ctx->wf_shadow.as<float4>();
+    ShadowField* shadow[3] = {&out->sro, &out->srd, &out->src};
+    for (int k = 0; k < 3; ++k)
+        *shadow[k] = records ? ShadowField{sh + k, SHADOW_RECORD} : ShadowField{sh + k * C, 1u};
+}
+
+// Device buffers of the wavefront, grown on demand and kept by the context;
+// `records` picks the state layout (records for sorted queues).
+static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, GpuUniforms** dU, bool records) {
     if (P > ctx->wf_capacity) {
         LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
         LUCIDA_HIP(ctx->wf_planes.ensure(P * 16 * PATH_RECORD), LUCIDA_OOM);
@@ -1381,21 +1398,10 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     }
     LUCIDA_HIP(ctx->wf_counters.ensure(COUNTER_BYTES), LUCIDA_OOM);
     LUCIDA_HIP(ctx->wf_uniforms.ensure(sizeof(GpuUniforms) * 2), LUCIDA_OOM);
-    float4* pl = ctx->wf_planes.as<float4>();
     const size_t C = ctx->wf_capacity;
-    out->org = PathField{pl};
-    out->dir = PathField{pl + 1};
-    out->thr = PathField{pl + 2};
-    out->prv = PathField{pl + 3};
-    out->rad = PathField{pl + 4};
-    out->gal = PathField{pl + 5};
-    out->gnr = PathField{pl + 6};
+    set_layout(ctx, out, records);
     out->hit = ctx->wf_hits.as<float4>();
     out->hin = reinterpret_cast<uint32_t*>(ctx->wf_hits.as<float4>() + C);
-    float4* sh = ctx->wf_shadow.as<float4>();
-    out->sro = ShadowField{sh};
-    out->srd = ShadowField{sh + 1};
-    out->src = ShadowField{sh + 2};
     out->q0 = ctx->wf_queues.as<uint32_t>();
     out->q1 = ctx->wf_queues.as<uint32_t>() + C;
     out->job = ctx->wf_queues.as<uint32_t>() + 2 * C;
@@ -1546,20 +1552,20 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     const size_t P = pixels * spb;
     if (P >= 0x7FFFFFFFull / 4) return set_error(LUCIDA_INVALID_ARG, "too many paths in one batch");
 
-    Planes p;
-    GpuUniforms* dU = nullptr;
-    st = ensure_wavefront(ctx, P, &p, &dU);
-    if (st != LUCIDA_OK) return st;
-
     const SceneView sc = make_view(ctx, scene);
     const int mode = scene->num_instances ? TRAV_INSTANCED
                      : (scene->wide && cfg.bvh_width == 4) ? TRAV_WIDE
                                                            : TRAV_BINARY;
+    const bool sort_rays = want_ray_order(cfg, sc, pixels * spb);
+
+    Planes p;
+    GpuUniforms* dU = nullptr;
+    st = ensure_wavefront(ctx, P, &p, &dU, sort_rays);
+    if (st != LUCIDA_OK) return st;
 
     hipStream_t s = ctx->stream;
     hipStream_t s2 = ctx->stream2;
     const uint32_t depth = uniforms->max_depth;
-    const bool sort_rays = want_ray_order(cfg, sc, pixels * spb);
     const uint32_t stats_on = (stats && cfg.count_traversal) ? 1u : 0u;
     const bool profile = stats && cfg.profile_kernels;
     const uint32_t batches = (samples + spb - 1) / spb;
@@ -1827,9 +1833,11 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     size_t max_paths = 0;
     for (const Round& r : rounds) max_paths = std::max(max_paths, r.paths);
 
-    Planes p;
+    Planes p_fields, p_rec;  // state as arrays of fields, and as records (sorted rounds)
     GpuUniforms* dU = nullptr;
-    st = ensure_wavefront(ctx, max_paths, &p, &dU);
+    st = ensure_wavefront(ctx, max_paths, &p_fields, &dU, false);
+    p_rec = p_fields;
+    set_layout(ctx, &p_rec, true);
     if (st != LUCIDA_OK) return st;
     DevBuf d_table, d_stamps;
     LUCIDA_HIP(d_table.ensure(table.size() * sizeof(BatchJob)), LUCIDA_OOM);
@@ -1846,10 +1854,11 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     const SceneView none{};
 
     auto enqueue_all = [&]() {
-        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT + 24u);
+        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p_fields.counters, (uint32_t)TOTALS_AT + 24u);
         hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, 0u);
         for (size_t ri = 0; ri < rounds.size(); ++ri) {
             const Round& r = rounds[ri];
+            const Planes& p = r.sort ? p_rec : p_fields;  // every round rewrites all path state
             const BatchJob* jr = dt + r.first;
             const uint32_t nj = (uint32_t)r.count, P = (uint32_t)r.paths;
             hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT);
@@ -1948,7 +1957,8 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
         std::vector<unsigned long long> h_stamps(rounds.size() + 1);
         uint64_t totals[10];
         LUCIDA_HIP(hipMemcpy(h_stamps.data(), stamps, h_stamps.size() * 8, hipMemcpyDeviceToHost), LUCIDA_READBACK);
-        LUCIDA_HIP(hipMemcpy(totals, p.counters + TOTALS_AT, sizeof totals, hipMemcpyDeviceToHost), LUCIDA_READBACK);
+        LUCIDA_HIP(hipMemcpy(totals, p_fields.counters + TOTALS_AT, sizeof totals, hipMemcpyDeviceToHost),
+                   LUCIDA_READBACK);
         int rate_khz = 0;
         LUCIDA_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device), LUCIDA_VALIDATION);
         if (job_done_ms)
