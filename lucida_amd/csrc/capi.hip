@@ -97,7 +97,9 @@ lucida_status lucida_ctx_create(int device, lucida_ctx** out) {
         hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_shaded, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_connected, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->ev_connected, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_sorted, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_shadow_sorted, hipEventDisableTiming) != hipSuccess) {
         delete ctx;
         return set_error(LUCIDA_NO_DEVICE, "could not create the context's streams or events");
     }
@@ -121,6 +123,8 @@ void lucida_ctx_destroy(lucida_ctx* ctx) {
     (void)hipEventDestroy(ctx->ev1);
     (void)hipEventDestroy(ctx->ev_shaded);
     (void)hipEventDestroy(ctx->ev_connected);
+    (void)hipEventDestroy(ctx->ev_sorted);
+    (void)hipEventDestroy(ctx->ev_shadow_sorted);
     (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -58,6 +58,8 @@ struct lucida_ctx {
     // CONNECT runs beside the next bounce's EXTEND, ordered by these events.
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_shaded = nullptr, ev_connected = nullptr;
+    // the two queue sorts, one per stream, joined before the walks start
+    hipEvent_t ev_sorted = nullptr, ev_shadow_sorted = nullptr;
     // LBVH scratch, reused between builds.
     lucida::DevBuf lb_bounds, lb_partial, lb_codes, lb_vals, lb_tmpk, lb_tmpv, lb_hist, lb_karras, lb_parent,
         lb_nodebox, lb_subtree, lb_flags, lb_misc, lb_rank, lb_nodes, lb_tris, lb_pos, lb_prims, lb_level;

@@ -106,7 +106,8 @@ struct Planes {
     uint32_t* ktmp;      // odd number of 8-bit passes, qtmp holds the sorted queue)
     uint32_t* qtmp2;     // the same for the shadow sort
     uint32_t* ktmp2;
-    uint32_t* sort_scratch;  // the queue sort's digit histograms and scan partials
+    uint32_t* sort_scratch;  // the extension sort's digit histograms and scan partials
+    uint32_t* sort_scratch2; // the shadow sort's (the two sorts run at once)
 };
 
 struct SceneView {
@@ -1391,7 +1392,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
         LUCIDA_HIP(ctx->wf_hits.ensure(P * 20), LUCIDA_OOM);
         LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * SHADOW_RECORD), LUCIDA_OOM);
         // q0, q1, job, the sorts' keys, orders and temporaries, then their scratch
-        LUCIDA_HIP(ctx->wf_queues.ensure((P * 10 + sort_counted_scratch_words(P)) * 4), LUCIDA_OOM);
+        LUCIDA_HIP(ctx->wf_queues.ensure((P * 10 + 2 * sort_counted_scratch_words(P)) * 4), LUCIDA_OOM);
         // spill columns for every lane of a traversal grid over P items
         LUCIDA_HIP(ctx->wf_spill.ensure(2 * (P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
         ctx->wf_capacity = P;
@@ -1413,6 +1414,7 @@ static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, Gp
     out->qtmp2 = ctx->wf_queues.as<uint32_t>() + 8 * C;
     out->ktmp2 = ctx->wf_queues.as<uint32_t>() + 9 * C;
     out->sort_scratch = ctx->wf_queues.as<uint32_t>() + 10 * C;
+    out->sort_scratch2 = out->sort_scratch + sort_counted_scratch_words(C);
     out->spill = ctx->wf_spill.as<uint32_t>();
     out->spill2 = out->spill + (C + TRACE_WG) * (WIDE_STACK - LDS_STACK);
     out->counters = ctx->wf_counters.as<uint32_t>();
@@ -1466,20 +1468,32 @@ static bool want_ray_order(const lucida_wavefront_config& cfg, const SceneView& 
 }
 
 // The sorts after SHADE of bounce b over a batch of P paths, by `bits` key
-// bits: the shadow rays (*shadow_order receives the array of record slots in
-// CONNECT's order) and, unless this was the last bounce, the extension queue
-// SHADE wrote to `qraw` (*next_queue receives the sorted queue). Both run on
-// `s`, one after the other; they share the histogram scratch.
-static lucida_status sort_queues(hipStream_t s, const Planes& p, uint32_t* qraw, size_t P, uint32_t b, bool extend,
-                                 uint32_t bits, uint32_t** next_queue, uint32_t** shadow_order) {
+// bits: the extension queue SHADE wrote to `qraw` on `s` (unless this was the
+// last bounce; *next_queue receives the sorted queue) and, at the same time
+// on `s2`, the shadow rays (*shadow_order receives the record slots in
+// CONNECT's order). Each stream then waits for the other's sort, so EXTEND
+// and CONNECT start together, after both: a walk's long waves beside a sort
+// would hold the CUs its short kernels need. The caller has recorded
+// ctx->ev_shaded on `s` after SHADE. Every step is a launch or an event, so
+// the sequence may be captured in a graph.
+static lucida_status sort_queues(lucida_ctx* ctx, hipStream_t s, hipStream_t s2, const Planes& p, uint32_t* qraw,
+                                 size_t P, uint32_t b, bool extend, uint32_t bits, uint32_t** next_queue,
+                                 uint32_t** shadow_order) {
     *next_queue = qraw;
+    LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_shaded, 0), LUCIDA_VALIDATION);
+    lucida_status st = sort_pairs_counted(s2, p.skey, p.sidx, p.ktmp2, p.qtmp2, P, p.counters + SHADOW_AT + b, bits,
+                                          p.sort_scratch2, shadow_order);
+    if (st != LUCIDA_OK) return st;
     if (extend) {
-        const lucida_status st = sort_pairs_counted(s, p.qkey, qraw, p.ktmp, p.qtmp, P, p.counters + b + 1, bits,
-                                                    p.sort_scratch, next_queue);
+        st = sort_pairs_counted(s, p.qkey, qraw, p.ktmp, p.qtmp, P, p.counters + b + 1, bits, p.sort_scratch,
+                                next_queue);
         if (st != LUCIDA_OK) return st;
     }
-    return sort_pairs_counted(s, p.skey, p.sidx, p.ktmp2, p.qtmp2, P, p.counters + SHADOW_AT + b, bits,
-                              p.sort_scratch, shadow_order);
+    LUCIDA_HIP(hipEventRecord(ctx->ev_shadow_sorted, s2), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipEventRecord(ctx->ev_sorted, s), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_shadow_sorted, 0), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_sorted, 0), LUCIDA_VALIDATION);
+    return LUCIDA_OK;
 }
 
 // The kernels' view of a resident scene.
@@ -1618,14 +1632,15 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
                                qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
                                sort_rays ? p.qkey : nullptr);
             mark(2, false, s);
-            if (sort_rays) {  // before CONNECT starts: its long waves would hold the CUs the sort's short kernels need
+            LUCIDA_HIP(hipEventRecord(ctx->ev_shaded, s), LUCIDA_VALIDATION);
+            if (sort_rays) {
                 mark(5, true, s);
-                st = sort_queues(s, p, qout, Pb, b, b + 1 < depth, 16, &qnext, &order);
+                st = sort_queues(ctx, s, s2, p, qout, Pb, b, b + 1 < depth, 16, &qnext, &order);
                 if (st != LUCIDA_OK) return st;
                 mark(5, false, s);
+            } else {
+                LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_shaded, 0), LUCIDA_VALIDATION);
             }
-            LUCIDA_HIP(hipEventRecord(ctx->ev_shaded, s), LUCIDA_VALIDATION);
-            LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_shaded, 0), LUCIDA_VALIDATION);
             mark(3, true, s2);
             launch_connect<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s2, sc, nullptr, p, p.counters + SHADOW_AT + b,
                                   order, stats_on);
@@ -1880,9 +1895,11 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                     hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, uj,
                                        nullptr, p, qin, p.counters + b, qout, p.counters + b + 1,
                                        p.counters + SHADOW_AT + b, r.sort ? p.qkey : nullptr);
-                    if (r.sort) (void)sort_queues(s, p, qout, P, b, b + 1 < r.depth, 16, &qnext, &order);
                     (void)hipEventRecord(ctx->ev_shaded, s);
-                    (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
+                    if (r.sort)
+                        (void)sort_queues(ctx, s, s2, p, qout, P, b, b + 1 < r.depth, 16, &qnext, &order);
+                    else
+                        (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
                     launch_connect<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s2, sc, nullptr, p,
                                           p.counters + SHADOW_AT + b, order, stats_on);
                     (void)hipEventRecord(ctx->ev_connected, s2);
@@ -1907,9 +1924,11 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, none, nullptr, jr,
                                    p, qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
                                    r.sort ? p.qkey : nullptr);
-                if (r.sort) (void)sort_queues(s, p, qout, P, b, b + 1 < r.depth, 24, &qnext, &order);
                 (void)hipEventRecord(ctx->ev_shaded, s);
-                (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
+                if (r.sort)
+                    (void)sort_queues(ctx, s, s2, p, qout, P, b, b + 1 < r.depth, 24, &qnext, &order);
+                else
+                    (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
                 launch_connect<true>(mode, dim3(grid_for(P, TRACE_WG)), s2, none, jr, p, p.counters + SHADOW_AT + b,
                                      order, stats_on);
                 (void)hipEventRecord(ctx->ev_connected, s2);
