@@ -425,14 +425,15 @@ def main() -> int:
     ext_ms = agg["extend"]
     dominant = max(agg, key=agg.get)
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
-    # CONNECT runs beside the next EXTEND on a second stream, so each launch's
-    # span includes time shared with the other walk. The two walks together:
-    # EXTEND's bytes plus CONNECT's (shadow record 48 + radiance 24 per ray,
-    # 64 per node visit and per triangle test) over the time the device is
-    # not in GENERATE, SHADE or RESOLVE (which run alone). The extension-queue
-    # sorts run beside CONNECT and stay inside this span.
+    # CONNECT runs beside the next EXTEND on a second stream, and batches
+    # alternate between two such pipelines, so each launch's span includes
+    # time shared with other kernels. The two walks together: EXTEND's bytes
+    # plus CONNECT's (shadow record 48 + radiance 24 per ray, 64 per node
+    # visit and per triangle test) over the whole render's device time
+    # (GENERATE, SHADE, the sorts and RESOLVE included: a lower bound on the
+    # walks' own rate).
     con_bytes = shadow_rays * (48 + 24) + node_bytes * (visits - ext_visits) + 64 * (tests - ext_tests)
-    walks_ms = device_ms - agg["generate"] - agg["shade"] - agg["resolve"]
+    walks_ms = device_ms
 
     # a-trous on the final film (1024^2)
     den = L.Film(gpu, a.width, a.height)

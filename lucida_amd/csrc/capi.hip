@@ -93,13 +93,19 @@ lucida_status lucida_ctx_create(int device, lucida_ctx** out) {
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount;
     ctx->name = prop.gcnArchName;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_shaded, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_connected, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_sorted, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_shadow_sorted, hipEventDisableTiming) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&ctx->ev0) == hipSuccess && hipEventCreate(&ctx->ev1) == hipSuccess &&
+              hipEventCreateWithFlags(&ctx->ev_resolved, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < 2 && ok; ++k) {  // every stream at the default priority
+        WfPipe& w = ctx->wf[k];
+        ok = (k == 0 ? (w.s = ctx->stream, true) : hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking) == hipSuccess) &&
+             hipStreamCreateWithFlags(&w.s2, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&w.ev_shaded, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&w.ev_connected, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&w.ev_sorted, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&w.ev_shadow_sorted, hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {
         delete ctx;
         return set_error(LUCIDA_NO_DEVICE, "could not create the context's streams or events");
     }
@@ -111,29 +117,37 @@ void lucida_ctx_destroy(lucida_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamSynchronize(ctx->stream2);
+    for (WfPipe& w : ctx->wf) {
+        if (w.s) (void)hipStreamSynchronize(w.s);
+        if (w.s2) (void)hipStreamSynchronize(w.s2);
+    }
     DevBuf* bufs[] = {&ctx->lb_bounds, &ctx->lb_partial, &ctx->lb_codes, &ctx->lb_vals,   &ctx->lb_tmpk,
                       &ctx->lb_tmpv,   &ctx->lb_hist,    &ctx->lb_karras, &ctx->lb_parent, &ctx->lb_nodebox,
                       &ctx->lb_subtree, &ctx->lb_flags,  &ctx->lb_misc,  &ctx->lb_rank,   &ctx->lb_nodes,
-                      &ctx->lb_tris,   &ctx->lb_pos,     &ctx->lb_prims, &ctx->lb_level, &ctx->ggx_table, &ctx->wf_planes,
-                      &ctx->wf_hits,   &ctx->wf_shadow,  &ctx->wf_queues, &ctx->wf_counters, &ctx->wf_uniforms,
-                      &ctx->wf_spill,  &ctx->dn_buf};
+                      &ctx->lb_tris,   &ctx->lb_pos,     &ctx->lb_prims, &ctx->lb_level, &ctx->ggx_table, &ctx->dn_buf};
     for (DevBuf* b : bufs) b->release();
+    for (WfPipe& w : ctx->wf) {
+        DevBuf* wb[] = {&w.planes, &w.hits, &w.shadow, &w.queues, &w.counters, &w.uniforms, &w.spill};
+        for (DevBuf* b : wb) b->release();
+        hipEvent_t evs[] = {w.ev_shaded, w.ev_connected, w.ev_sorted, w.ev_shadow_sorted};
+        for (hipEvent_t e : evs)
+            if (e) (void)hipEventDestroy(e);
+        if (w.s2) (void)hipStreamDestroy(w.s2);
+        if (w.s && w.s != ctx->stream) (void)hipStreamDestroy(w.s);
+    }
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
-    (void)hipEventDestroy(ctx->ev_shaded);
-    (void)hipEventDestroy(ctx->ev_connected);
-    (void)hipEventDestroy(ctx->ev_sorted);
-    (void)hipEventDestroy(ctx->ev_shadow_sorted);
-    (void)hipStreamDestroy(ctx->stream2);
+    (void)hipEventDestroy(ctx->ev_resolved);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
 lucida_status lucida_ctx_synchronize(lucida_ctx* ctx) {
     if (!ctx) return set_error(LUCIDA_INVALID_ARG, "lucida_ctx_synchronize: null ctx");
-    LUCIDA_HIP(hipStreamSynchronize(ctx->stream2), LUCIDA_VALIDATION);
-    LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
+    for (WfPipe& w : ctx->wf) {
+        LUCIDA_HIP(hipStreamSynchronize(w.s2), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipStreamSynchronize(w.s), LUCIDA_VALIDATION);
+    }
     LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
     return LUCIDA_OK;
 }

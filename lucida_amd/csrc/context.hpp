@@ -46,6 +46,22 @@ struct DevBuf {
     T* as() const { return reinterpret_cast<T*>(ptr); }
 };
 
+// One wavefront pipeline: a stream for GENERATE, EXTEND, SHADE, the
+// extension sort and RESOLVE, a second one (default priority, like the
+// first) for the shadow sort and CONNECT, the events ordering them, and the
+// device buffers of its batch (path state, hits, shadow rays, queues,
+// counters, uniforms, stack spill). A render runs its batches on the
+// context's two pipelines in turn, so one pipeline's latency-bound SHADE and
+// sorts overlap the other's walks.
+struct WfPipe {
+    hipStream_t s = nullptr, s2 = nullptr;
+    hipEvent_t ev_shaded = nullptr, ev_connected = nullptr;
+    // the two queue sorts, one per stream, joined before the walks start
+    hipEvent_t ev_sorted = nullptr, ev_shadow_sorted = nullptr;
+    DevBuf planes, hits, shadow, queues, counters, uniforms, spill;
+    size_t capacity = 0;
+};
+
 }  // namespace lucida
 
 struct lucida_ctx {
@@ -54,12 +70,10 @@ struct lucida_ctx {
     std::string name;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // A second stream (default priority, like the first) on which a render's
-    // CONNECT runs beside the next bounce's EXTEND, ordered by these events.
-    hipStream_t stream2 = nullptr;
-    hipEvent_t ev_shaded = nullptr, ev_connected = nullptr;
-    // the two queue sorts, one per stream, joined before the walks start
-    hipEvent_t ev_sorted = nullptr, ev_shadow_sorted = nullptr;
+    // Wavefront pipelines; wf[0].s is `stream`. Batched renders use wf[0].
+    lucida::WfPipe wf[2];
+    // A render's latest RESOLVE (batches accumulate into the film in order).
+    hipEvent_t ev_resolved = nullptr;
     // LBVH scratch, reused between builds.
     lucida::DevBuf lb_bounds, lb_partial, lb_codes, lb_vals, lb_tmpk, lb_tmpv, lb_hist, lb_karras, lb_parent,
         lb_nodebox, lb_subtree, lb_flags, lb_misc, lb_rank, lb_nodes, lb_tris, lb_pos, lb_prims, lb_level;
@@ -67,9 +81,6 @@ struct lucida_ctx {
     // GGX directional-albedo table (energy compensation), 32 x 32 floats.
     lucida::DevBuf ggx_table;
     bool ggx_ready = false;
-    // Wavefront path state, hits, shadow rays, queues, counters, uniforms.
-    lucida::DevBuf wf_planes, wf_hits, wf_shadow, wf_queues, wf_counters, wf_uniforms, wf_spill;
-    size_t wf_capacity = 0;
     // Denoiser planes.
     lucida::DevBuf dn_buf;
 };

@@ -1371,54 +1371,56 @@ lucida_status ensure_ggx_table(lucida_ctx* ctx) {
 
 // Points the path and shadow fields into the context's buffers (capacity C)
 // as records or as arrays of fields.
-static void set_layout(lucida_ctx* ctx, Planes* out, bool records) {
-    const size_t C = ctx->wf_capacity;
-    float4* pl = ctx->wf_planes.as<float4>();
+static void set_layout(const WfPipe& w, Planes* out, bool records) {
+    const size_t C = w.capacity;
+    float4* pl = w.planes.as<float4>();
     PathField* path[7] = {&out->org, &out->dir, &out->thr, &out->prv, &out->rad, &out->gal, &out->gnr};
     for (int k = 0; k < 7; ++k)
         *path[k] = records ? PathField{pl + k, PATH_RECORD} : PathField{pl + k * C, 1u};
-    float4* sh = ctx->wf_shadow.as<float4>();
+    float4* sh = w.shadow.as<float4>();
     ShadowField* shadow[3] = {&out->sro, &out->srd, &out->src};
     for (int k = 0; k < 3; ++k)
         *shadow[k] = records ? ShadowField{sh + k, SHADOW_RECORD} : ShadowField{sh + k * C, 1u};
 }
 
-// Device buffers of the wavefront, grown on demand and kept by the context;
-// `records` picks the state layout (records for sorted queues).
-static lucida_status ensure_wavefront(lucida_ctx* ctx, size_t P, Planes* out, GpuUniforms** dU, bool records) {
-    if (P > ctx->wf_capacity) {
-        LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
-        LUCIDA_HIP(ctx->wf_planes.ensure(P * 16 * PATH_RECORD), LUCIDA_OOM);
-        LUCIDA_HIP(ctx->wf_hits.ensure(P * 20), LUCIDA_OOM);
-        LUCIDA_HIP(ctx->wf_shadow.ensure(P * 16 * SHADOW_RECORD), LUCIDA_OOM);
+// Device buffers of a wavefront pipeline, grown on demand and kept by the
+// context; `records` picks the state layout (records for sorted queues).
+static lucida_status ensure_wavefront(WfPipe& w, size_t P, Planes* out, GpuUniforms** dU, bool records) {
+    if (P > w.capacity) {
+        LUCIDA_HIP(hipStreamSynchronize(w.s), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipStreamSynchronize(w.s2), LUCIDA_VALIDATION);
+        LUCIDA_HIP(w.planes.ensure(P * 16 * PATH_RECORD), LUCIDA_OOM);
+        LUCIDA_HIP(w.hits.ensure(P * 20), LUCIDA_OOM);
+        LUCIDA_HIP(w.shadow.ensure(P * 16 * SHADOW_RECORD), LUCIDA_OOM);
         // q0, q1, job, the sorts' keys, orders and temporaries, then their scratch
-        LUCIDA_HIP(ctx->wf_queues.ensure((P * 10 + 2 * sort_counted_scratch_words(P)) * 4), LUCIDA_OOM);
+        LUCIDA_HIP(w.queues.ensure((P * 10 + 2 * sort_counted_scratch_words(P)) * 4), LUCIDA_OOM);
         // spill columns for every lane of a traversal grid over P items
-        LUCIDA_HIP(ctx->wf_spill.ensure(2 * (P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
-        ctx->wf_capacity = P;
+        LUCIDA_HIP(w.spill.ensure(2 * (P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
+        w.capacity = P;
     }
-    LUCIDA_HIP(ctx->wf_counters.ensure(COUNTER_BYTES), LUCIDA_OOM);
-    LUCIDA_HIP(ctx->wf_uniforms.ensure(sizeof(GpuUniforms) * 2), LUCIDA_OOM);
-    const size_t C = ctx->wf_capacity;
-    set_layout(ctx, out, records);
-    out->hit = ctx->wf_hits.as<float4>();
-    out->hin = reinterpret_cast<uint32_t*>(ctx->wf_hits.as<float4>() + C);
-    out->q0 = ctx->wf_queues.as<uint32_t>();
-    out->q1 = ctx->wf_queues.as<uint32_t>() + C;
-    out->job = ctx->wf_queues.as<uint32_t>() + 2 * C;
-    out->qkey = ctx->wf_queues.as<uint32_t>() + 3 * C;
-    out->qtmp = ctx->wf_queues.as<uint32_t>() + 4 * C;
-    out->ktmp = ctx->wf_queues.as<uint32_t>() + 5 * C;
-    out->skey = ctx->wf_queues.as<uint32_t>() + 6 * C;
-    out->sidx = ctx->wf_queues.as<uint32_t>() + 7 * C;
-    out->qtmp2 = ctx->wf_queues.as<uint32_t>() + 8 * C;
-    out->ktmp2 = ctx->wf_queues.as<uint32_t>() + 9 * C;
-    out->sort_scratch = ctx->wf_queues.as<uint32_t>() + 10 * C;
+    LUCIDA_HIP(w.counters.ensure(COUNTER_BYTES), LUCIDA_OOM);
+    LUCIDA_HIP(w.uniforms.ensure(sizeof(GpuUniforms) * 2), LUCIDA_OOM);
+    const size_t C = w.capacity;
+    set_layout(w, out, records);
+    uint32_t* q = w.queues.as<uint32_t>();
+    out->hit = w.hits.as<float4>();
+    out->hin = reinterpret_cast<uint32_t*>(w.hits.as<float4>() + C);
+    out->q0 = q;
+    out->q1 = q + C;
+    out->job = q + 2 * C;
+    out->qkey = q + 3 * C;
+    out->qtmp = q + 4 * C;
+    out->ktmp = q + 5 * C;
+    out->skey = q + 6 * C;
+    out->sidx = q + 7 * C;
+    out->qtmp2 = q + 8 * C;
+    out->ktmp2 = q + 9 * C;
+    out->sort_scratch = q + 10 * C;
     out->sort_scratch2 = out->sort_scratch + sort_counted_scratch_words(C);
-    out->spill = ctx->wf_spill.as<uint32_t>();
+    out->spill = w.spill.as<uint32_t>();
     out->spill2 = out->spill + (C + TRACE_WG) * (WIDE_STACK - LDS_STACK);
-    out->counters = ctx->wf_counters.as<uint32_t>();
-    *dU = ctx->wf_uniforms.as<GpuUniforms>();
+    out->counters = w.counters.as<uint32_t>();
+    *dU = w.uniforms.as<GpuUniforms>();
     return LUCIDA_OK;
 }
 
@@ -1468,31 +1470,30 @@ static bool want_ray_order(const lucida_wavefront_config& cfg, const SceneView& 
 }
 
 // The sorts after SHADE of bounce b over a batch of P paths, by `bits` key
-// bits: the extension queue SHADE wrote to `qraw` on `s` (unless this was the
-// last bounce; *next_queue receives the sorted queue) and, at the same time
-// on `s2`, the shadow rays (*shadow_order receives the record slots in
-// CONNECT's order). Each stream then waits for the other's sort, so EXTEND
-// and CONNECT start together, after both: a walk's long waves beside a sort
-// would hold the CUs its short kernels need. The caller has recorded
-// ctx->ev_shaded on `s` after SHADE. Every step is a launch or an event, so
+// bits, on pipeline `w`: the extension queue SHADE wrote to `qraw` on w.s
+// (unless this was the last bounce; *next_queue receives the sorted queue)
+// and, at the same time on w.s2, the shadow rays (*shadow_order receives the
+// record slots in CONNECT's order). Each stream then waits for the other's
+// sort, so EXTEND and CONNECT start together, after both: a walk's long waves
+// beside a sort would hold the CUs its short kernels need. The caller has
+// recorded w.ev_shaded on w.s after SHADE. Every step is a launch or an event, so
 // the sequence may be captured in a graph.
-static lucida_status sort_queues(lucida_ctx* ctx, hipStream_t s, hipStream_t s2, const Planes& p, uint32_t* qraw,
-                                 size_t P, uint32_t b, bool extend, uint32_t bits, uint32_t** next_queue,
-                                 uint32_t** shadow_order) {
+static lucida_status sort_queues(WfPipe& w, const Planes& p, uint32_t* qraw, size_t P, uint32_t b, bool extend,
+                                 uint32_t bits, uint32_t** next_queue, uint32_t** shadow_order) {
     *next_queue = qraw;
-    LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_shaded, 0), LUCIDA_VALIDATION);
-    lucida_status st = sort_pairs_counted(s2, p.skey, p.sidx, p.ktmp2, p.qtmp2, P, p.counters + SHADOW_AT + b, bits,
-                                          p.sort_scratch2, shadow_order);
+    LUCIDA_HIP(hipStreamWaitEvent(w.s2, w.ev_shaded, 0), LUCIDA_VALIDATION);
+    lucida_status st = sort_pairs_counted(w.s2, p.skey, p.sidx, p.ktmp2, p.qtmp2, P, p.counters + SHADOW_AT + b,
+                                          bits, p.sort_scratch2, shadow_order);
     if (st != LUCIDA_OK) return st;
     if (extend) {
-        st = sort_pairs_counted(s, p.qkey, qraw, p.ktmp, p.qtmp, P, p.counters + b + 1, bits, p.sort_scratch,
+        st = sort_pairs_counted(w.s, p.qkey, qraw, p.ktmp, p.qtmp, P, p.counters + b + 1, bits, p.sort_scratch,
                                 next_queue);
         if (st != LUCIDA_OK) return st;
     }
-    LUCIDA_HIP(hipEventRecord(ctx->ev_shadow_sorted, s2), LUCIDA_VALIDATION);
-    LUCIDA_HIP(hipEventRecord(ctx->ev_sorted, s), LUCIDA_VALIDATION);
-    LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_shadow_sorted, 0), LUCIDA_VALIDATION);
-    LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_sorted, 0), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipEventRecord(w.ev_shadow_sorted, w.s2), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipEventRecord(w.ev_sorted, w.s), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipStreamWaitEvent(w.s, w.ev_shadow_sorted, 0), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipStreamWaitEvent(w.s2, w.ev_sorted, 0), LUCIDA_VALIDATION);
     return LUCIDA_OK;
 }
 
@@ -1572,17 +1573,20 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
                                                            : TRAV_BINARY;
     const bool sort_rays = want_ray_order(cfg, sc, pixels * spb);
 
-    Planes p;
-    GpuUniforms* dU = nullptr;
-    st = ensure_wavefront(ctx, P, &p, &dU, sort_rays);
-    if (st != LUCIDA_OK) return st;
+    // Batches alternate between the context's two pipelines: while one runs
+    // its latency-bound SHADE and sorts, the other's walks fill the device.
+    const uint32_t batches = (samples + spb - 1) / spb;
+    const int npipes = batches >= 2 ? 2 : 1;
+    Planes pl[2];
+    GpuUniforms* dUs[2] = {nullptr, nullptr};
+    for (int q = 0; q < npipes; ++q) {
+        st = ensure_wavefront(ctx->wf[q], P, &pl[q], &dUs[q], sort_rays);
+        if (st != LUCIDA_OK) return st;
+    }
 
-    hipStream_t s = ctx->stream;
-    hipStream_t s2 = ctx->stream2;
     const uint32_t depth = uniforms->max_depth;
     const uint32_t stats_on = (stats && cfg.count_traversal) ? 1u : 0u;
     const bool profile = stats && cfg.profile_kernels;
-    const uint32_t batches = (samples + spb - 1) / spb;
 
     // Optional per-kernel events: one start/stop pair per launch.
     std::vector<hipEvent_t> evs;
@@ -1600,9 +1604,19 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         (void)hipEventRecord(evs[ev_next++], on);
     };
 
-    LUCIDA_HIP(hipEventRecord(ctx->ev0, s), LUCIDA_VALIDATION);
-    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT + 24u);
-    for (uint32_t done = 0; done < samples; done += spb) {
+    hipStream_t s0 = ctx->wf[0].s;
+    LUCIDA_HIP(hipEventRecord(ctx->ev0, s0), LUCIDA_VALIDATION);
+    for (int q = 0; q < npipes; ++q) {
+        if (q > 0) LUCIDA_HIP(hipStreamWaitEvent(ctx->wf[q].s, ctx->ev0, 0), LUCIDA_VALIDATION);
+        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, ctx->wf[q].s, pl[q].counters,
+                           (uint32_t)TOTALS_AT + 24u);
+    }
+    uint32_t batch = 0;
+    for (uint32_t done = 0; done < samples; done += spb, ++batch) {
+        WfPipe& w = ctx->wf[batch % npipes];
+        const Planes& p = pl[batch % npipes];
+        GpuUniforms* dU = dUs[batch % npipes];
+        hipStream_t s = w.s, s2 = w.s2;
         const uint32_t this_spb = std::min(spb, samples - done);
         GpuUniforms u = *uniforms;
         u.sample_offset = uniforms->sample_offset + done;
@@ -1626,41 +1640,52 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
             launch_extend<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s, sc, nullptr, p, qin, p.counters + b,
                                  stats_on);
             mark(1, false, s);
-            if (b > 0) LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_connected, 0), LUCIDA_VALIDATION);
+            if (b > 0) LUCIDA_HIP(hipStreamWaitEvent(s, w.ev_connected, 0), LUCIDA_VALIDATION);
             mark(2, true, s);
             hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, dU, nullptr, p,
                                qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
                                sort_rays ? p.qkey : nullptr);
             mark(2, false, s);
-            LUCIDA_HIP(hipEventRecord(ctx->ev_shaded, s), LUCIDA_VALIDATION);
+            LUCIDA_HIP(hipEventRecord(w.ev_shaded, s), LUCIDA_VALIDATION);
             if (sort_rays) {
                 mark(5, true, s);
-                st = sort_queues(ctx, s, s2, p, qout, Pb, b, b + 1 < depth, 16, &qnext, &order);
+                st = sort_queues(w, p, qout, Pb, b, b + 1 < depth, 16, &qnext, &order);
                 if (st != LUCIDA_OK) return st;
                 mark(5, false, s);
             } else {
-                LUCIDA_HIP(hipStreamWaitEvent(s2, ctx->ev_shaded, 0), LUCIDA_VALIDATION);
+                LUCIDA_HIP(hipStreamWaitEvent(s2, w.ev_shaded, 0), LUCIDA_VALIDATION);
             }
             mark(3, true, s2);
             launch_connect<false>(mode, dim3(grid_for(Pb, TRACE_WG)), s2, sc, nullptr, p, p.counters + SHADOW_AT + b,
                                   order, stats_on);
             mark(3, false, s2);
-            LUCIDA_HIP(hipEventRecord(ctx->ev_connected, s2), LUCIDA_VALIDATION);
+            LUCIDA_HIP(hipEventRecord(w.ev_connected, s2), LUCIDA_VALIDATION);
             qin = qnext;
         }
-        if (depth > 0) LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_connected, 0), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipStreamWaitEvent(s, w.ev_connected, 0), LUCIDA_VALIDATION);
+        // batches add into the film in order, whichever pipeline ran them
+        if (batch > 0) LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_resolved, 0), LUCIDA_VALIDATION);
         mark(4, true, s);
         hipLaunchKernelGGL(k_resolve, dim3(grid_for(pixels, 256)), dim3(256), 0, s, dU, p, film->accum);
         mark(4, false, s);
+        LUCIDA_HIP(hipEventRecord(ctx->ev_resolved, s), LUCIDA_VALIDATION);
         hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, depth);
         LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
     }
-    LUCIDA_HIP(hipEventRecord(ctx->ev1, s), LUCIDA_VALIDATION);
+    for (int q = 1; q < npipes; ++q) {  // join the other pipeline
+        LUCIDA_HIP(hipEventRecord(ctx->wf[q].ev_shaded, ctx->wf[q].s), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipStreamWaitEvent(s0, ctx->wf[q].ev_shaded, 0), LUCIDA_VALIDATION);
+    }
+    LUCIDA_HIP(hipEventRecord(ctx->ev1, s0), LUCIDA_VALIDATION);
     LUCIDA_HIP(hipEventSynchronize(ctx->ev1), LUCIDA_VALIDATION);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
     if (stats) {
-        uint64_t totals[10];
-        LUCIDA_HIP(hipMemcpy(totals, p.counters + TOTALS_AT, sizeof totals, hipMemcpyDeviceToHost), LUCIDA_READBACK);
+        uint64_t totals[10] = {};
+        for (int q = 0; q < npipes; ++q) {
+            uint64_t t[10];
+            LUCIDA_HIP(hipMemcpy(t, pl[q].counters + TOTALS_AT, sizeof t, hipMemcpyDeviceToHost), LUCIDA_READBACK);
+            for (int k = 0; k < 10; ++k) totals[k] += t[k];
+        }
         float ms = 0.0f;
         LUCIDA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), LUCIDA_VALIDATION);
         stats->extension_rays = totals[0];
@@ -1850,15 +1875,16 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
 
     Planes p_fields, p_rec;  // state as arrays of fields, and as records (sorted rounds)
     GpuUniforms* dU = nullptr;
-    st = ensure_wavefront(ctx, max_paths, &p_fields, &dU, false);
+    WfPipe& w = ctx->wf[0];
+    st = ensure_wavefront(w, max_paths, &p_fields, &dU, false);
     p_rec = p_fields;
-    set_layout(ctx, &p_rec, true);
+    set_layout(w, &p_rec, true);
     if (st != LUCIDA_OK) return st;
     DevBuf d_table, d_stamps;
     LUCIDA_HIP(d_table.ensure(table.size() * sizeof(BatchJob)), LUCIDA_OOM);
     LUCIDA_HIP(d_stamps.ensure((rounds.size() + 1) * 8), LUCIDA_OOM);
     hipStream_t s = ctx->stream;
-    hipStream_t s2 = ctx->stream2;  // CONNECT, beside the next EXTEND (as in a single render)
+    hipStream_t s2 = w.s2;  // CONNECT, beside the next EXTEND (as in a single render)
     st = LUCIDA_OK;
     if (hipMemcpyAsync(d_table.ptr, table.data(), table.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s) !=
         hipSuccess)
@@ -1891,21 +1917,21 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                     uint32_t* order = nullptr;
                     launch_extend<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p, qin,
                                          p.counters + b, stats_on);
-                    if (b > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
+                    if (b > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
                     hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, uj,
                                        nullptr, p, qin, p.counters + b, qout, p.counters + b + 1,
                                        p.counters + SHADOW_AT + b, r.sort ? p.qkey : nullptr);
-                    (void)hipEventRecord(ctx->ev_shaded, s);
+                    (void)hipEventRecord(w.ev_shaded, s);
                     if (r.sort)
-                        (void)sort_queues(ctx, s, s2, p, qout, P, b, b + 1 < r.depth, 16, &qnext, &order);
+                        (void)sort_queues(w, p, qout, P, b, b + 1 < r.depth, 16, &qnext, &order);
                     else
-                        (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
+                        (void)hipStreamWaitEvent(s2, w.ev_shaded, 0);
                     launch_connect<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s2, sc, nullptr, p,
                                           p.counters + SHADOW_AT + b, order, stats_on);
-                    (void)hipEventRecord(ctx->ev_connected, s2);
+                    (void)hipEventRecord(w.ev_connected, s2);
                     qin = qnext;
                 }
-                if (r.depth > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
+                if (r.depth > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
                 hipLaunchKernelGGL(k_resolve, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, uj, p,
                                    table[r.first].film);
                 hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
@@ -1920,21 +1946,21 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 uint32_t* qnext = qout;
                 uint32_t* order = nullptr;
                 launch_extend<true>(mode, dim3(grid_for(P, TRACE_WG)), s, none, jr, p, qin, p.counters + b, stats_on);
-                if (b > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
+                if (b > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
                 hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, none, nullptr, jr,
                                    p, qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
                                    r.sort ? p.qkey : nullptr);
-                (void)hipEventRecord(ctx->ev_shaded, s);
+                (void)hipEventRecord(w.ev_shaded, s);
                 if (r.sort)
-                    (void)sort_queues(ctx, s, s2, p, qout, P, b, b + 1 < r.depth, 24, &qnext, &order);
+                    (void)sort_queues(w, p, qout, P, b, b + 1 < r.depth, 24, &qnext, &order);
                 else
-                    (void)hipStreamWaitEvent(s2, ctx->ev_shaded, 0);
+                    (void)hipStreamWaitEvent(s2, w.ev_shaded, 0);
                 launch_connect<true>(mode, dim3(grid_for(P, TRACE_WG)), s2, none, jr, p, p.counters + SHADOW_AT + b,
                                      order, stats_on);
-                (void)hipEventRecord(ctx->ev_connected, s2);
+                (void)hipEventRecord(w.ev_connected, s2);
                 qin = qnext;
             }
-            if (r.depth > 0) (void)hipStreamWaitEvent(s, ctx->ev_connected, 0);
+            if (r.depth > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
             hipLaunchKernelGGL(k_resolve_batch, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, jr, nj,
                                (uint32_t)r.pixels, p);
             hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
