@@ -39,7 +39,6 @@ using namespace dev;
 
 constexpr uint32_t MISS = 0xFFFFFFFFu;
 constexpr uint32_t ANALYTIC = 0x80000000u;
-constexpr int WG = 64;          // one wave per workgroup for the per-path kernels
 constexpr int SHADE_WG = 256;   // SHADE: four waves, one queue atomic per block
 constexpr int TRACE_WG = 128;   // traversal workgroups (two waves)
 constexpr int STACK = 32;       // traversal stack entries per lane (shaders/common/bvh.wgsl:133)
@@ -214,12 +213,18 @@ __device__ inline void flush_traversal_stats(uint32_t* counters, uint32_t visits
 // GENERATE (generate.wgsl:36-88): camera ray per (pixel, sample); identity
 // queue; bounce-0 count.
 // ---------------------------------------------------------------------------
+// The pixels a batch covers: `count` consecutive pixels from `first` (rows of
+// the image; a whole image is {0, width * height}). Path `local` of the batch
+// is sample local / count of pixel first + local % count.
+struct Band {
+    uint32_t first, count;
+};
+
 // Path `index` of the batch; `local` is its index within its own render
-// (sample-major: local = sample * pixels + pixel).
-__device__ inline void generate_path(const GpuUniforms& u, uint32_t index, uint32_t local, Planes& p) {
-    const uint32_t pixels = u.width * u.height;
-    const uint32_t pixel = local % pixels;
-    const uint32_t sample = u.sample_offset + local / pixels;
+// (sample-major over the band: local = sample * band.count + pixel - band.first).
+__device__ inline void generate_path(const GpuUniforms& u, uint32_t index, uint32_t local, Planes& p, Band band) {
+    const uint32_t pixel = band.first + local % band.count;
+    const uint32_t sample = u.sample_offset + local / band.count;
     const uint32_t x = pixel % u.width, y = pixel / u.width;
     Sampler s = make_sampler(u, pixel, sample);
     float pu, pv, lu, lv;
@@ -249,13 +254,13 @@ __device__ inline void generate_path(const GpuUniforms& u, uint32_t index, uint3
     p.q0[index] = index;
 }
 
-__global__ __launch_bounds__(256) void k_generate(const GpuUniforms* __restrict__ U, Planes p) {
+__global__ __launch_bounds__(256) void k_generate(const GpuUniforms* __restrict__ U, Planes p, Band band) {
     const GpuUniforms u = *U;
-    const uint32_t total = u.width * u.height * u.samples_per_launch;
+    const uint32_t total = band.count * u.samples_per_launch;
     const uint32_t index = blockIdx.x * 256 + threadIdx.x;
     if (index == 0) p.counters[0] = total;
     if (index >= total) return;
-    generate_path(u, index, index, p);
+    generate_path(u, index, index, p, band);
 }
 
 // Largest j < n with base(j) <= x (bases ascending, base(0) = 0).
@@ -281,7 +286,7 @@ __global__ __launch_bounds__(256) void k_generate_batch(const BatchJob* __restri
     for (;;) {
         const uint32_t j = first_job(jid);
         if (jid != j) continue;
-        generate_path(jobs[j].u, index, index - jobs[j].path_base, p);
+        generate_path(jobs[j].u, index, index - jobs[j].path_base, p, Band{0u, jobs[j].pixels});
         p.job[index] = j;
         break;
     }
@@ -865,7 +870,7 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
                                               const uint32_t* __restrict__ queue_in,
                                               const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ queue_out,
                                               uint32_t* __restrict__ next_count, uint32_t* __restrict__ shadow_count,
-                                              uint32_t* __restrict__ key_out) {
+                                              uint32_t* __restrict__ key_out, Band band0) {
     const uint32_t i = blockIdx.x * SHADE_WG + threadIdx.x;
     const uint32_t count = *count_ptr;
     bool alive = false, shadow = false;
@@ -887,12 +892,12 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
         const V3 origin = f4xyz(of), dir = f4xyz(df);
         V3 throughput = f4xyz(tf), radiance = f4xyz(rf), prev_position = f4xyz(pf);
         const float prev_bsdf_pdf = tf.w;
-        const uint32_t pixels = u.width * u.height;
+        const Band band = BATCH ? Band{0u, jobs[j].pixels} : band0;
         Sampler s;
         s.sobol = u.sampler_kind == LUCIDA_SAMPLER_SOBOL;
         s.state = __float_as_uint(df.w);
-        s.index = u.sample_offset + local / pixels;
-        s.seed = hash_u32(local % pixels + u.frame_seed * 0x9e3779b9u);
+        s.index = u.sample_offset + local / band.count;
+        s.seed = hash_u32(band.first + local % band.count + u.frame_seed * 0x9e3779b9u);
         s.dim = __float_as_uint(pf.w);
         const uint32_t mode = u.sampling_mode > 2 ? 2 : u.sampling_mode;
         const uint32_t total = (uint32_t)sc.num_lights + (sc.env.present() ? 1u : 0u);
@@ -1224,14 +1229,14 @@ __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* _
 // RESOLVE (resolve.wgsl:305-338): per pixel, sum the batch's samples in sample
 // order and add them to the film.
 // ---------------------------------------------------------------------------
-// Pixel px of a render whose paths start at `base`: its samples in order.
+// Pixel band.first + px of a render whose paths start at `base`: its samples
+// in order.
 __device__ inline void resolve_pixel(const GpuUniforms& u, const Planes& p, uint32_t base, uint32_t px,
-                                     GpuAccum* accum) {
-    const uint32_t pixels = u.width * u.height;
+                                     GpuAccum* accum, Band band) {
     V3 sum{0, 0, 0}, sq{0, 0, 0}, al{0, 0, 0}, nr{0, 0, 0};
     float dp = 0.0f;
     for (uint32_t j = 0; j < u.samples_per_launch; ++j) {
-        const uint32_t q = base + px + j * pixels;
+        const uint32_t q = base + px + j * band.count;
         const V3 r = f4xyz(p.rad[q]);
         const float4 ga = p.gal[q];
         sum = sum + r;
@@ -1240,7 +1245,7 @@ __device__ inline void resolve_pixel(const GpuUniforms& u, const Planes& p, uint
         nr = nr + f4xyz(p.gnr[q]);
         dp = dp + ga.w;
     }
-    float4* a = reinterpret_cast<float4*>(accum + px);
+    float4* a = reinterpret_cast<float4*>(accum + band.first + px);
     float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
     a0.x += sum.x; a0.y += sum.y; a0.z += sum.z; a0.w += (float)u.samples_per_launch;
     a1.x += al.x; a1.y += al.y; a1.z += al.z; a1.w += dp;
@@ -1249,11 +1254,12 @@ __device__ inline void resolve_pixel(const GpuUniforms& u, const Planes& p, uint
     a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
 }
 
-__global__ __launch_bounds__(256) void k_resolve(const GpuUniforms* __restrict__ U, Planes p, GpuAccum* accum) {
+__global__ __launch_bounds__(256) void k_resolve(const GpuUniforms* __restrict__ U, Planes p, GpuAccum* accum,
+                                                 Band band) {
     const GpuUniforms& u = *U;
     const uint32_t px = blockIdx.x * 256 + threadIdx.x;
-    if (px >= u.width * u.height) return;
-    resolve_pixel(u, p, 0u, px, accum);
+    if (px >= band.count) return;
+    resolve_pixel(u, p, 0u, px, accum, band);
 }
 
 // RESOLVE for one round of a batched render: every job's pixels, each into its film.
@@ -1265,7 +1271,7 @@ __global__ __launch_bounds__(256) void k_resolve_batch(const BatchJob* __restric
     for (;;) {
         const uint32_t j = first_job(jid);
         if (jid != j) continue;
-        resolve_pixel(jobs[j].u, p, jobs[j].path_base, g - jobs[j].pixel_base, jobs[j].film);
+        resolve_pixel(jobs[j].u, p, jobs[j].path_base, g - jobs[j].pixel_base, jobs[j].film, Band{0u, jobs[j].pixels});
         break;
     }
 }
@@ -1571,11 +1577,29 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     const int mode = scene->num_instances ? TRAV_INSTANCED
                      : (scene->wide && cfg.bvh_width == 4) ? TRAV_WIDE
                                                            : TRAV_BINARY;
-    const bool sort_rays = want_ray_order(cfg, sc, pixels * spb);
 
-    // Batches alternate between the context's two pipelines: while one runs
-    // its latency-bound SHADE and sorts, the other's walks fill the device.
-    const uint32_t batches = (samples + spb - 1) / spb;
+    // The render's batches: spb samples of every pixel each. They alternate
+    // between the context's two pipelines: while one runs its latency-bound
+    // SHADE and sorts, the other's walks fill the device. A render of one
+    // large batch is split into two bands of rows, one per pipeline.
+    struct Item {
+        uint32_t done, spb;  // first sample (relative to the render's) and count
+        Band band;
+    };
+    std::vector<Item> items;
+    const uint32_t width = film->width, height = film->height;
+    if (samples <= spb && P >= ((size_t)1 << 19) && height >= 2) {
+        const uint32_t top = height / 2;
+        items.push_back({0u, samples, Band{0u, top * width}});
+        items.push_back({0u, samples, Band{top * width, (height - top) * width}});
+    } else {
+        for (uint32_t done = 0; done < samples; done += spb)
+            items.push_back({done, std::min(spb, samples - done), Band{0u, (uint32_t)pixels}});
+    }
+    size_t item_paths = 0;
+    for (const Item& it : items) item_paths = std::max(item_paths, (size_t)it.band.count * it.spb);
+    const bool sort_rays = want_ray_order(cfg, sc, item_paths);
+    const uint32_t batches = (uint32_t)items.size();
     const int npipes = batches >= 2 ? 2 : 1;
     Planes pl[2];
     GpuUniforms* dUs[2] = {nullptr, nullptr};
@@ -1611,21 +1635,20 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, ctx->wf[q].s, pl[q].counters,
                            (uint32_t)TOTALS_AT + 24u);
     }
-    uint32_t batch = 0;
-    for (uint32_t done = 0; done < samples; done += spb, ++batch) {
+    for (uint32_t batch = 0; batch < batches; ++batch) {
+        const Item& it = items[batch];
         WfPipe& w = ctx->wf[batch % npipes];
         const Planes& p = pl[batch % npipes];
         GpuUniforms* dU = dUs[batch % npipes];
         hipStream_t s = w.s, s2 = w.s2;
-        const uint32_t this_spb = std::min(spb, samples - done);
         GpuUniforms u = *uniforms;
-        u.sample_offset = uniforms->sample_offset + done;
-        u.samples_per_launch = this_spb;
-        const size_t Pb = pixels * this_spb;
+        u.sample_offset = uniforms->sample_offset + it.done;
+        u.samples_per_launch = it.spb;
+        const size_t Pb = (size_t)it.band.count * it.spb;
         LUCIDA_HIP(hipMemcpyAsync(dU, &u, sizeof u, hipMemcpyHostToDevice, s), LUCIDA_VALIDATION);
         hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT);
         mark(0, true, s);
-        hipLaunchKernelGGL(k_generate, dim3(grid_for(Pb, 256)), dim3(256), 0, s, dU, p);
+        hipLaunchKernelGGL(k_generate, dim3(grid_for(Pb, 256)), dim3(256), 0, s, dU, p, it.band);
         mark(0, false, s);
         // CONNECT of bounce b runs on the second stream beside EXTEND of
         // bounce b + 1 (shadow planes and radiance against queues, rays and
@@ -1644,7 +1667,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
             mark(2, true, s);
             hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(Pb, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, dU, nullptr, p,
                                qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
-                               sort_rays ? p.qkey : nullptr);
+                               sort_rays ? p.qkey : nullptr, it.band);
             mark(2, false, s);
             LUCIDA_HIP(hipEventRecord(w.ev_shaded, s), LUCIDA_VALIDATION);
             if (sort_rays) {
@@ -1666,7 +1689,8 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         // batches add into the film in order, whichever pipeline ran them
         if (batch > 0) LUCIDA_HIP(hipStreamWaitEvent(s, ctx->ev_resolved, 0), LUCIDA_VALIDATION);
         mark(4, true, s);
-        hipLaunchKernelGGL(k_resolve, dim3(grid_for(pixels, 256)), dim3(256), 0, s, dU, p, film->accum);
+        hipLaunchKernelGGL(k_resolve, dim3(grid_for(it.band.count, 256)), dim3(256), 0, s, dU, p, film->accum,
+                           it.band);
         mark(4, false, s);
         LUCIDA_HIP(hipEventRecord(ctx->ev_resolved, s), LUCIDA_VALIDATION);
         hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, depth);
@@ -1909,7 +1933,8 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 const GpuUniforms* uj =
                     reinterpret_cast<const GpuUniforms*>(reinterpret_cast<const char*>(jr) + offsetof(BatchJob, u));
                 const SceneView& sc = table[r.first].sc;
-                hipLaunchKernelGGL(k_generate, dim3(grid_for(P, 256)), dim3(256), 0, s, uj, p);
+                const Band whole{0u, (uint32_t)r.pixels};
+                hipLaunchKernelGGL(k_generate, dim3(grid_for(P, 256)), dim3(256), 0, s, uj, p, whole);
                 uint32_t* qin = p.q0;
                 for (uint32_t b = 0; b < r.depth; ++b) {
                     uint32_t* qout = qin == p.q1 ? p.q0 : p.q1;
@@ -1920,7 +1945,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                     if (b > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
                     hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, uj,
                                        nullptr, p, qin, p.counters + b, qout, p.counters + b + 1,
-                                       p.counters + SHADOW_AT + b, r.sort ? p.qkey : nullptr);
+                                       p.counters + SHADOW_AT + b, r.sort ? p.qkey : nullptr, whole);
                     (void)hipEventRecord(w.ev_shaded, s);
                     if (r.sort)
                         (void)sort_queues(w, p, qout, P, b, b + 1 < r.depth, 16, &qnext, &order);
@@ -1933,7 +1958,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 }
                 if (r.depth > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
                 hipLaunchKernelGGL(k_resolve, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, uj, p,
-                                   table[r.first].film);
+                                   table[r.first].film, whole);
                 hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
                 hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, (uint32_t)(ri + 1));
                 continue;
@@ -1949,7 +1974,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 if (b > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
                 hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, none, nullptr, jr,
                                    p, qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
-                                   r.sort ? p.qkey : nullptr);
+                                   r.sort ? p.qkey : nullptr, Band{0u, 0u});
                 (void)hipEventRecord(w.ev_shaded, s);
                 if (r.sort)
                     (void)sort_queues(w, p, qout, P, b, b + 1 < r.depth, 24, &qnext, &order);
