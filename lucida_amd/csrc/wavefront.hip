@@ -1897,20 +1897,26 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     size_t max_paths = 0;
     for (const Round& r : rounds) max_paths = std::max(max_paths, r.paths);
 
-    Planes p_fields, p_rec;  // state as arrays of fields, and as records (sorted rounds)
-    GpuUniforms* dU = nullptr;
-    WfPipe& w = ctx->wf[0];
-    st = ensure_wavefront(w, max_paths, &p_fields, &dU, false);
-    p_rec = p_fields;
-    set_layout(w, &p_rec, true);
-    if (st != LUCIDA_OK) return st;
+    // Rounds alternate between the context's two pipelines (as the batches of
+    // a single render do); their RESOLVEs run in round order, so each film
+    // receives its job's chunks in order. A batch captured in a graph keeps
+    // to one pipeline: captured with two, the process crashed (host
+    // segmentation fault, cause not found).
+    const int npipes = (rounds.size() >= 2 && !cfg.use_graph) ? 2 : 1;
+    Planes p_fields[2], p_rec[2];  // state as arrays of fields, and as records (sorted rounds)
+    for (int q = 0; q < npipes; ++q) {
+        GpuUniforms* dU = nullptr;
+        st = ensure_wavefront(ctx->wf[q], max_paths, &p_fields[q], &dU, false);
+        if (st != LUCIDA_OK) return st;
+        p_rec[q] = p_fields[q];
+        set_layout(ctx->wf[q], &p_rec[q], true);
+    }
     DevBuf d_table, d_stamps;
     LUCIDA_HIP(d_table.ensure(table.size() * sizeof(BatchJob)), LUCIDA_OOM);
     LUCIDA_HIP(d_stamps.ensure((rounds.size() + 1) * 8), LUCIDA_OOM);
-    hipStream_t s = ctx->stream;
-    hipStream_t s2 = w.s2;  // CONNECT, beside the next EXTEND (as in a single render)
+    hipStream_t s0 = ctx->stream;  // pipeline 0's; the graph is captured from it
     st = LUCIDA_OK;
-    if (hipMemcpyAsync(d_table.ptr, table.data(), table.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s) !=
+    if (hipMemcpyAsync(d_table.ptr, table.data(), table.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s0) !=
         hipSuccess)
         st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: job table upload failed");
     const BatchJob* dt = d_table.as<BatchJob>();
@@ -1919,11 +1925,21 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     const SceneView none{};
 
     auto enqueue_all = [&]() {
-        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p_fields.counters, (uint32_t)TOTALS_AT + 24u);
-        hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, 0u);
+        hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s0, stamps, 0u);
+        for (int q = 0; q < npipes; ++q) {
+            if (q > 0) {  // fork the other pipeline off pipeline 0's stream
+                (void)hipEventRecord(ctx->wf[q].ev_sorted, s0);
+                (void)hipStreamWaitEvent(ctx->wf[q].s, ctx->wf[q].ev_sorted, 0);
+            }
+            hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, ctx->wf[q].s, p_fields[q].counters,
+                               (uint32_t)TOTALS_AT + 24u);
+        }
         for (size_t ri = 0; ri < rounds.size(); ++ri) {
             const Round& r = rounds[ri];
-            const Planes& p = r.sort ? p_rec : p_fields;  // every round rewrites all path state
+            const int q = (int)(ri % npipes);
+            WfPipe& w = ctx->wf[q];
+            hipStream_t s = w.s, s2 = w.s2;
+            const Planes& p = r.sort ? p_rec[q] : p_fields[q];  // every round rewrites all path state
             const BatchJob* jr = dt + r.first;
             const uint32_t nj = (uint32_t)r.count, P = (uint32_t)r.paths;
             hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, p.counters, (uint32_t)TOTALS_AT);
@@ -1957,8 +1973,10 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                     qin = qnext;
                 }
                 if (r.depth > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
+                if (npipes > 1 && ri > 0) (void)hipStreamWaitEvent(s, ctx->ev_resolved, 0);
                 hipLaunchKernelGGL(k_resolve, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, uj, p,
                                    table[r.first].film, whole);
+                if (npipes > 1) (void)hipEventRecord(ctx->ev_resolved, s);
                 hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
                 hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, (uint32_t)(ri + 1));
                 continue;
@@ -1986,49 +2004,59 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 qin = qnext;
             }
             if (r.depth > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
+            if (npipes > 1 && ri > 0) (void)hipStreamWaitEvent(s, ctx->ev_resolved, 0);
             hipLaunchKernelGGL(k_resolve_batch, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, jr, nj,
                                (uint32_t)r.pixels, p);
+            if (npipes > 1) (void)hipEventRecord(ctx->ev_resolved, s);
             hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
             hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, (uint32_t)(ri + 1));
+        }
+        for (int q = 1; q < npipes; ++q) {  // join: pipeline 0's stream waits for the others
+            (void)hipEventRecord(ctx->wf[q].ev_shaded, ctx->wf[q].s);
+            (void)hipStreamWaitEvent(s0, ctx->wf[q].ev_shaded, 0);
         }
     };
 
     if (st == LUCIDA_OK) {
-        LUCIDA_HIP(hipEventRecord(ctx->ev0, s), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipEventRecord(ctx->ev0, s0), LUCIDA_VALIDATION);
         if (cfg.use_graph) {
             // One graph for the whole batch: every launch above is a kernel
             // node (no memset, no host copy), replayed with one launch.
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
-            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            if (hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                 st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: stream capture failed to start");
             } else {
                 enqueue_all();
-                const hipError_t ec = hipStreamEndCapture(s, &graph);
+                const hipError_t ec = hipStreamEndCapture(s0, &graph);
                 if (ec != hipSuccess || !graph)
                     st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: stream capture failed");
                 else if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
                     st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: graph instantiation failed");
-                else if (hipGraphLaunch(exec, s) != hipSuccess)
+                else if (hipGraphLaunch(exec, s0) != hipSuccess)
                     st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: graph launch failed");
             }
-            (void)hipStreamSynchronize(s);
+            (void)hipStreamSynchronize(s0);
             if (exec) (void)hipGraphExecDestroy(exec);
             if (graph) (void)hipGraphDestroy(graph);
         } else {
             enqueue_all();
             if (hipGetLastError() != hipSuccess) st = set_error(LUCIDA_SHADER, "lucida_render_batch: launch failed");
         }
-        LUCIDA_HIP(hipEventRecord(ctx->ev1, s), LUCIDA_VALIDATION);
+        LUCIDA_HIP(hipEventRecord(ctx->ev1, s0), LUCIDA_VALIDATION);
         LUCIDA_HIP(hipEventSynchronize(ctx->ev1), LUCIDA_VALIDATION);
     }
     if (st == LUCIDA_OK) LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
     if (st == LUCIDA_OK && (stats || job_done_ms)) {
         std::vector<unsigned long long> h_stamps(rounds.size() + 1);
-        uint64_t totals[10];
+        uint64_t totals[10] = {};
         LUCIDA_HIP(hipMemcpy(h_stamps.data(), stamps, h_stamps.size() * 8, hipMemcpyDeviceToHost), LUCIDA_READBACK);
-        LUCIDA_HIP(hipMemcpy(totals, p_fields.counters + TOTALS_AT, sizeof totals, hipMemcpyDeviceToHost),
-                   LUCIDA_READBACK);
+        for (int q = 0; q < npipes; ++q) {
+            uint64_t t[10];
+            LUCIDA_HIP(hipMemcpy(t, p_fields[q].counters + TOTALS_AT, sizeof t, hipMemcpyDeviceToHost),
+                       LUCIDA_READBACK);
+            for (int k = 0; k < 10; ++k) totals[k] += t[k];
+        }
         int rate_khz = 0;
         LUCIDA_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device), LUCIDA_VALIDATION);
         if (job_done_ms)
