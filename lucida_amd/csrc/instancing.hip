@@ -220,7 +220,8 @@ static lucida_status build_two_level(lucida_ctx* ctx, lucida_scene* s, const luc
     s->primitives = dprims;
     s->num_instances = n;
     s->tlas_root = (uint32_t)blas_total;
-    return build_tri_edges(ctx, s);
+    const lucida_status es = build_tri_edges(ctx, s);
+    return es != LUCIDA_OK ? es : build_prim_planes(ctx, s);
 }
 
 }  // namespace

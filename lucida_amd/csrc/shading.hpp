@@ -518,16 +518,28 @@ __device__ inline bool intersect_sphere(const GpuPrimitive& s, V3 o, V3 d, float
     *out = t;
     return true;
 }
-__device__ inline bool intersect_quad(const GpuPrimitive& q, V3 o, V3 d, float t_min, float t_max, float* out) {
+// The quad's w = n / |n|^2 (n = edge_u x edge_v), a constant of the quad: a
+// scene computes it once (quad_w) with the same operations, so the test
+// below sees the same value it would compute itself.
+__device__ inline V3 quad_w(const GpuPrimitive& q) {
+    const V3 n_raw = cross(ld3(q.edge_u), ld3(q.edge_v));
+    return n_raw / dot(n_raw, n_raw);
+}
+
+// `w` is quad_w(q). With t_min > 0, a plane behind the origin (t's numerator
+// and denominator of opposite signs, or a zero numerator) is rejected before
+// the division: t would be negative or zero and fail t >= t_min anyway.
+__device__ inline bool intersect_quad(const GpuPrimitive& q, V3 w, V3 o, V3 d, float t_min, float t_max,
+                                      float* out) {
     V3 origin = ld3(q.position), eu = ld3(q.edge_u), ev = ld3(q.edge_v), normal = ld3(q.normal);
     float denom = dot(normal, d);
     if (fabsf(denom) < 1.0e-8f) return false;
-    float t = (dot(normal, origin) - dot(normal, o)) / denom;
+    const float num = dot(normal, origin) - dot(normal, o);
+    if (t_min > 0.0f && (num == 0.0f || signbit(num) != signbit(denom))) return false;
+    float t = num / denom;
     if (t < t_min || t > t_max) return false;
     V3 p = o + t * d;
     V3 dd = p - origin;
-    V3 n_raw = cross(eu, ev);
-    V3 w = n_raw / dot(n_raw, n_raw);
     float a = dot(w, cross(dd, ev));
     float b = dot(w, cross(eu, dd));
     if (!(a >= 0.0f && a <= 1.0f) || !(b >= 0.0f && b <= 1.0f)) return false;

@@ -112,6 +112,7 @@ struct Planes {
 struct SceneView {
     const GpuMaterial* __restrict__ materials;
     const GpuPrimitive* __restrict__ prims;
+    const float4* __restrict__ prim_w;  // a quad's w (quad_w), per analytic primitive
     const float4* __restrict__ pos;
     const GpuVertexAttr* __restrict__ attrs;
     const GpuTriangle* __restrict__ tris;
@@ -793,7 +794,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJ
         for (uint32_t k = 0; k < sc.num_prims; ++k) {
             const GpuPrimitive& pr = sc.prims[k];
             float t;
-            bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, o, d, T_MIN, best_t, &t)
+            bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, best_t, &t)
                                                       : intersect_sphere(pr, o, d, T_MIN, best_t, &t);
             if (h) {
                 best_t = t;
@@ -1140,7 +1141,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
         for (uint32_t k = 0; k < sc.num_prims && !blocked; ++k) {
             const GpuPrimitive& pr = sc.prims[k];
             float t;
-            blocked = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, o, d, T_MIN, t_max, &t)
+            blocked = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, t_max, &t)
                                                        : intersect_sphere(pr, o, d, T_MIN, t_max, &t);
         }
         if (!blocked) {
@@ -1191,7 +1192,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* _
     for (uint32_t k = 0; k < sc.num_prims; ++k) {
         const GpuPrimitive& pr = sc.prims[k];
         float t;
-        const bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, o, d, T_MIN, best_t, &t)
+        const bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, best_t, &t)
                                                         : intersect_sphere(pr, o, d, T_MIN, best_t, &t);
         if (h) {
             best_t = t;
@@ -1339,6 +1340,14 @@ __global__ __launch_bounds__(256) void k_ggx_table(float* table) {
 
 inline uint32_t grid_for(size_t n, int block) { return (uint32_t)((n + block - 1) / block); }
 
+// Per analytic primitive: a quad's w (quad_w); zero for spheres.
+__global__ void k_prim_planes(const GpuPrimitive* __restrict__ prims, size_t n, float4* __restrict__ w) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V3 q = prims[i].kind == LUCIDA_PRIM_KIND_QUAD ? quad_w(prims[i]) : V3{0.0f, 0.0f, 0.0f};
+    w[i] = make_float4(q.x, q.y, q.z, 0.0f);
+}
+
 // Per triangle (in the scene's leaf order): p0, p1 - p0, p2 - p0.
 __global__ void k_tri_edges(const GpuTriangle* __restrict__ tris, const float4* __restrict__ pos, size_t n,
                             float4* __restrict__ edges) {
@@ -1354,6 +1363,16 @@ __global__ void k_tri_edges(const GpuTriangle* __restrict__ tris, const float4* 
 }
 
 }  // namespace
+
+lucida_status build_prim_planes(lucida_ctx* ctx, lucida_scene* scene) {
+    const size_t n = scene->num_primitives;
+    if (n == 0) return LUCIDA_OK;
+    if (!scene->prim_w) LUCIDA_HIP(hipMalloc(&scene->prim_w, n * sizeof(float4)), LUCIDA_OOM);
+    hipLaunchKernelGGL(k_prim_planes, dim3(grid_for(n, 64)), dim3(64), 0, ctx->stream, scene->primitives, n,
+                       scene->prim_w);
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    return LUCIDA_OK;
+}
 
 lucida_status build_tri_edges(lucida_ctx* ctx, lucida_scene* scene) {
     const size_t n = scene->num_triangles;
@@ -1508,6 +1527,7 @@ static SceneView make_view(lucida_ctx* ctx, const lucida_scene* scene) {
     SceneView sc;
     sc.materials = scene->materials;
     sc.prims = scene->primitives;
+    sc.prim_w = scene->prim_w;
     sc.pos = reinterpret_cast<const float4*>(scene->positions);
     sc.attrs = scene->vertex_attrs;
     sc.tris = scene->triangles;
