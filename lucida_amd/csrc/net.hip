@@ -93,7 +93,10 @@ __global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ in, int c
     for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int m = 0; m < MT; ++m) acc[t][m] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int c = 0; c < chunks; ++c) {
+    // A chunk's weight fragments and input columns; chunk c + 1's loads are
+    // issued before chunk c's products, so a wave keeps its next loads in
+    // flight while its MFMAs run.
+    auto load_chunk = [&](int c, bf16x8* av, bf16x8* bv) {
         const int kb = 2 * c + (g >> 1);
         const bool kok = kb < kbs;
         int tap = 0, cb = 0;
@@ -103,23 +106,33 @@ __global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ in, int c
         }
         const int sy = y + tap / 3 - 1;
         const bool yok = kok && sy >= 0 && sy < h;
-        bf16x8 av[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m) av[m] = frag[(c * MT + m) * 64 + lane];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int sx = x0 + 16 * t + r + tap % 3 - 1;
-            bf16x8 bv;
             if (yok && sx >= 0 && sx < w) {
-                bv = *reinterpret_cast<const bf16x8*>(in + (((size_t)img * h + sy) * w + sx) * cin + cb * 16 +
-                                                      8 * (g & 1));
+                bv[t] = *reinterpret_cast<const bf16x8*>(in + (((size_t)img * h + sy) * w + sx) * cin + cb * 16 +
+                                                         8 * (g & 1));
             } else {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) bv[i] = f2b(0.0f);
+                for (int i = 0; i < 8; ++i) bv[t][i] = f2b(0.0f);
             }
-#pragma unroll
-            for (int m = 0; m < MT; ++m) acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[t][m], 0, 0, 0);
         }
+    };
+    bf16x8 av[MT], bv[NT], av_next[MT], bv_next[NT];
+    load_chunk(0, av_next, bv_next);
+    for (int c = 0; c < chunks; ++c) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m) av[m] = av_next[m];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bv[t] = bv_next[t];
+        if (c + 1 < chunks) load_chunk(c + 1, av_next, bv_next);
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[t], acc[t][m], 0, 0, 0);
     }
     const int cout = MT * 16;
 #pragma unroll
