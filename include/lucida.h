@@ -459,7 +459,7 @@ typedef struct lucida_render_stats {
 
 /* Tunables of the wavefront. Zero fields take the defaults. */
 typedef struct lucida_wavefront_config {
-    uint32_t samples_per_batch; /* samples of every pixel in flight at once (0: about 2^22 paths) */
+    uint32_t samples_per_batch; /* samples of every pixel in flight at once (0: about 2^23 paths) */
     uint32_t max_paths;         /* batched renders: paths per round at most (0: 2^24)            */
     uint32_t bvh_width;         /* 0 or 2: the binary tree (the reference's order, whose node     *
                                  * visit counts the oracle repeats); 4: the scene's 4-wide BVH    */

@@ -62,7 +62,7 @@ def probe_jobs(gpu: L.Gpu, jobs):
     out = []
     for scene, u, spp, film in jobs:
         pixels = u.width * u.height
-        chunk = max(1, min(spp, (1 << 22) // pixels))
+        chunk = max(1, min(spp, (1 << 23) // pixels))
         out.append((scene, u, chunk, L.Film(gpu, u.width, u.height)))
     return out
 
