@@ -312,6 +312,7 @@ def main() -> int:
     ap.add_argument("--net-scenes", type=int, default=32, help="training scenes for the learned denoiser")
     ap.add_argument("--net-ref-spp", type=int, default=1024, help="target samples per pixel of the training pairs")
     ap.add_argument("--net-steps", type=int, default=3000, help="Adam steps (0: no learned denoiser)")
+    ap.add_argument("--net-crop", type=int, default=128, help="training crop side (8 crops per step)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched request stream")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the Cornell-box and instanced-scene throughput figures")
@@ -491,15 +492,21 @@ def main() -> int:
         if a.net_steps > 0:
             from lucida_amd import denoiser_training as T
             t0 = time.perf_counter()
-            pairs, data_paths = T.make_training_set(gpu, a.net_scenes, ref_spp=a.net_ref_spp)
+            progress = lambda msg: print(f"denoiser training: {msg}", file=sys.stderr, flush=True)
+            pairs, data_paths = T.make_training_set(
+                gpu, a.net_scenes, ref_spp=a.net_ref_spp,
+                log=lambda msg: progress(msg) if msg.startswith("scene ") and int(msg.split()[1][:-1]) % 16 == 15
+                else None)
             t1 = time.perf_counter()
             net = L.LearnedDenoiser(gpu, seed=1)
-            losses = T.train(gpu, net, pairs, a.net_steps)
+            losses = T.train(gpu, net, pairs, a.net_steps, crop_w=a.net_crop, crop_h=a.net_crop, log_every=500,
+                             log=progress)
             t2 = time.perf_counter()
             del pairs
             out["learned_denoiser"] = {
                 "params": L.LearnedDenoiser.param_count(),
-                "training_scenes": a.net_scenes, "crops": "8 x 128x128 per step", "steps": a.net_steps,
+                "training_scenes": a.net_scenes, "crops": f"8 x {a.net_crop}x{a.net_crop} per step",
+                "steps": a.net_steps,
                 "target_spp": a.net_ref_spp, "data_paths": data_paths, "data_s": round(t1 - t0, 2),
                 "train_s": round(t2 - t1, 2), "ms_per_step": round((t2 - t1) / a.net_steps * 1e3, 3),
                 "loss_first_100": float(np.mean(losses[:100])), "loss_last_100": float(np.mean(losses[-100:])),
