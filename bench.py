@@ -325,7 +325,7 @@ def main() -> int:
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              f"(the path shards by sample range with one Accum reduction, see DESIGN.md)", file=sys.stderr)
+              f"(the path would shard by sample range, merging the films with one sum all-reduce)", file=sys.stderr)
         return 2
 
     import lucida_amd as L
