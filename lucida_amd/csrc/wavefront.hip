@@ -1671,13 +1671,16 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, ctx->wf[q].s, pl[q].counters,
                            (uint32_t)TOTALS_AT + 24u);
     }
+    // each batch's uniforms, kept until the render has finished (their
+    // uploads are asynchronous)
+    std::vector<GpuUniforms> host_u(batches, *uniforms);
     for (uint32_t batch = 0; batch < batches; ++batch) {
         const Item& it = items[batch];
         WfPipe& w = ctx->wf[batch % npipes];
         const Planes& p = pl[batch % npipes];
         GpuUniforms* dU = dUs[batch % npipes];
         hipStream_t s = w.s, s2 = w.s2;
-        GpuUniforms u = *uniforms;
+        GpuUniforms& u = host_u[batch];
         u.sample_offset = uniforms->sample_offset + it.done;
         u.samples_per_launch = it.spb;
         const size_t Pb = (size_t)it.band.count * it.spb;
