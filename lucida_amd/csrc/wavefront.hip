@@ -324,12 +324,30 @@ __device__ inline V3 safe_inv_dir(V3 d) {
 // e1 = p1 - p0 and e2 = p2 - p0 come precomputed (the same f32 subtractions,
 // done once per triangle by k_tri_edges), so a test is three independent
 // 16-byte loads instead of an index load followed by three gathers.
+// When every active lane tests the same triangle its 48 bytes are read once,
+// through the scalar cache (the uniform fast path of load4 below).
+__device__ inline void load_tri(const float4* edges, uint32_t ti, V3* p0, V3* e1, V3* e2) {
+#if __HIP_DEVICE_COMPILE__
+    const uint32_t t0 = __builtin_amdgcn_readfirstlane(ti);
+    if (__builtin_amdgcn_ballot_w64(ti != t0) == 0) {
+        typedef const __attribute__((address_space(4))) float4 ConstF4;
+        const ConstF4* e = reinterpret_cast<const ConstF4*>(reinterpret_cast<uintptr_t>(edges + 3 * (size_t)t0));
+        *p0 = f4xyz(e[0]);
+        *e1 = f4xyz(e[1]);
+        *e2 = f4xyz(e[2]);
+        return;
+    }
+#endif
+    const float4* e = edges + 3 * (size_t)ti;
+    *p0 = f4xyz(e[0]);
+    *e1 = f4xyz(e[1]);
+    *e2 = f4xyz(e[2]);
+}
+
 __device__ inline bool tri_test(const float4* __restrict__ edges, uint32_t ti, V3 o, V3 d, float t_min,
                                 float t_max, float* t_out, float* u_out, float* v_out) {
-    const float4* e = edges + 3 * (size_t)ti;
-    const V3 p0 = f4xyz(e[0]);
-    const V3 e1 = f4xyz(e[1]);
-    const V3 e2 = f4xyz(e[2]);
+    V3 p0, e1, e2;
+    load_tri(edges, ti, &p0, &e1, &e2);
     V3 pv = cross(d, e2);
     float det = dot(e1, pv);
     if (fabsf(det) < 1.0e-12f) return false;
@@ -346,6 +364,37 @@ __device__ inline bool tri_test(const float4* __restrict__ edges, uint32_t ti, V
     *u_out = u;
     *v_out = v;
     return true;
+}
+
+// Four consecutive float4s q[k .. k+3]. When every active lane of the wave
+// reads the same ones (common once the queues are sorted) they are read once
+// through the scalar cache into SGPRs (the offset made uniform with
+// readfirstlane, the loads through a constant-address-space pointer) instead
+// of by every lane through the vector memory path.
+__device__ inline void load4(const float4* q, uint32_t k, float4* a, float4* b, float4* c, float4* e) {
+#if __HIP_DEVICE_COMPILE__
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
+    if (__builtin_amdgcn_ballot_w64(k != k0) == 0) {
+        typedef const __attribute__((address_space(4))) float4 ConstF4;
+        const ConstF4* r = reinterpret_cast<const ConstF4*>(reinterpret_cast<uintptr_t>(q + k0));
+        *a = r[0];
+        *b = r[1];
+        *c = r[2];
+        *e = r[3];
+        return;
+    }
+#endif
+    const float4* r = q + k;
+    *a = r[0];
+    *b = r[1];
+    *c = r[2];
+    *e = r[3];
+}
+
+// The child pair of a node: nodes[first] and nodes[first + 1], 32 bytes each.
+__device__ inline void load_pair(const GpuBvhNode* nodes, uint32_t first, float4* a0, float4* a1, float4* b0,
+                                 float4* b1) {
+    load4(reinterpret_cast<const float4*>(nodes), 2 * first, a0, a1, b0, b1);
 }
 
 // A child carried as (first << 5) | count: count <= 31 (0: internal), first < 2^27.
@@ -417,8 +466,8 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
                     }
                 }
             } else {
-                const float4* cn = reinterpret_cast<const float4*>(sc.nodes + first);
-                const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
+                float4 a0, a1, b0, b1;
+                load_pair(sc.nodes, first, &a0, &a1, &b0, &b1);
                 float ea, eb;
                 const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
                 const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
@@ -448,8 +497,8 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
         // inner nodes, until a leaf
         while (more && (cur & COUNT_MASK) == 0) {
             *visits += 1;
-            const float4* cn = reinterpret_cast<const float4*>(sc.nodes + (cur >> COUNT_BITS));
-            const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
+            float4 a0, a1, b0, b1;
+            load_pair(sc.nodes, cur >> COUNT_BITS, &a0, &a1, &b0, &b1);
             float ea, eb;
             const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
             const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
@@ -686,8 +735,8 @@ __device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 
     };
     // Visits the inner node cur; false when neither child is hit.
     auto inner = [&]() -> bool {
-        const float4* cn = reinterpret_cast<const float4*>(sc.nodes + (cur >> COUNT_BITS));
-        const float4 a0 = cn[0], a1 = cn[1], b0 = cn[2], b1 = cn[3];
+        float4 a0, a1, b0, b1;
+        load_pair(sc.nodes, cur >> COUNT_BITS, &a0, &a1, &b0, &b1);
         float ea, eb;
         const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
         const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
