@@ -320,10 +320,6 @@ __device__ inline V3 safe_inv_dir(V3 d) {
               fabsf(d.z) > SMALL ? 1.0f / d.z : copysignf(BIG, d.z)};
 }
 
-// bvh.rs:930 intersect_triangle, in the twin's operation order. The edges
-// e1 = p1 - p0 and e2 = p2 - p0 come precomputed (the same f32 subtractions,
-// done once per triangle by k_tri_edges), so a test is three independent
-// 16-byte loads instead of an index load followed by three gathers.
 // When every active lane tests the same triangle its 48 bytes are read once,
 // through the scalar cache (the uniform fast path of load4 below).
 __device__ inline void load_tri(const float4* edges, uint32_t ti, V3* p0, V3* e1, V3* e2) {
@@ -344,6 +340,10 @@ __device__ inline void load_tri(const float4* edges, uint32_t ti, V3* p0, V3* e1
     *e2 = f4xyz(e[2]);
 }
 
+// bvh.rs:930 intersect_triangle, in the twin's operation order. The edges
+// e1 = p1 - p0 and e2 = p2 - p0 come precomputed (the same f32 subtractions,
+// done once per triangle by k_tri_edges), so a test is three independent
+// 16-byte loads instead of an index load followed by three gathers.
 __device__ inline bool tri_test(const float4* __restrict__ edges, uint32_t ti, V3 o, V3 d, float t_min,
                                 float t_max, float* t_out, float* u_out, float* v_out) {
     V3 p0, e1, e2;
@@ -370,7 +370,9 @@ __device__ inline bool tri_test(const float4* __restrict__ edges, uint32_t ti, V
 // reads the same ones (common once the queues are sorted) they are read once
 // through the scalar cache into SGPRs (the offset made uniform with
 // readfirstlane, the loads through a constant-address-space pointer) instead
-// of by every lane through the vector memory path.
+// of by every lane through the vector memory path. Nodes, triangle edges and
+// instances are written only by earlier launches, and every launch starts
+// with the scalar cache invalidated, so these reads never see stale data.
 __device__ inline void load4(const float4* q, uint32_t k, float4* a, float4* b, float4* c, float4* e) {
 #if __HIP_DEVICE_COMPILE__
     const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
