@@ -399,6 +399,37 @@ __device__ inline void load_pair(const GpuBvhNode* nodes, uint32_t first, float4
     load4(reinterpret_cast<const float4*>(nodes), 2 * first, a0, a1, b0, b1);
 }
 
+// Slab tests of a node's child pair, the pair read as load_pair reads it. In
+// the uniform case the tests take the planes straight from SGPRs; the empty
+// asm markers (distinct per branch) keep the compiler from merging the two
+// branches' tests into one on copies of the planes.
+__device__ inline void test_pair(const GpuBvhNode* nodes, uint32_t first, V3 o, V3 inv, float t_min, float t_max,
+                                 bool* ha, bool* hb, float* ea, float* eb, uint32_t* ca, uint32_t* cb) {
+#if __HIP_DEVICE_COMPILE__
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(first);
+    if (__builtin_amdgcn_ballot_w64(first != k0) == 0) {
+        typedef const __attribute__((address_space(4))) float4 ConstF4;
+        const ConstF4* r = reinterpret_cast<const ConstF4*>(reinterpret_cast<uintptr_t>(nodes + k0));
+        const float4 a0 = r[0], a1 = r[1], b0 = r[2], b1 = r[3];
+        *ha = slab(a0, a1, o, inv, t_min, t_max, ea);
+        *hb = slab(b0, b1, o, inv, t_min, t_max, eb);
+        *ca = (__float_as_uint(a0.w) << 5) | __float_as_uint(a1.w);
+        *cb = (__float_as_uint(b0.w) << 5) | __float_as_uint(b1.w);
+        asm volatile("; uniform pair");
+        return;
+    }
+#endif
+    const float4* r = reinterpret_cast<const float4*>(nodes + first);
+    const float4 a0 = r[0], a1 = r[1], b0 = r[2], b1 = r[3];
+    *ha = slab(a0, a1, o, inv, t_min, t_max, ea);
+    *hb = slab(b0, b1, o, inv, t_min, t_max, eb);
+    *ca = (__float_as_uint(a0.w) << 5) | __float_as_uint(a1.w);
+    *cb = (__float_as_uint(b0.w) << 5) | __float_as_uint(b1.w);
+#if __HIP_DEVICE_COMPILE__
+    asm volatile("; per-lane pair");
+#endif
+}
+
 // A child carried as (first << 5) | count: count <= 31 (0: internal), first < 2^27.
 constexpr uint32_t COUNT_BITS = 5, COUNT_MASK = 31u;
 __device__ inline uint32_t pack_child(uint32_t first, uint32_t count) { return (first << COUNT_BITS) | count; }
@@ -499,13 +530,10 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
         // inner nodes, until a leaf
         while (more && (cur & COUNT_MASK) == 0) {
             *visits += 1;
-            float4 a0, a1, b0, b1;
-            load_pair(sc.nodes, cur >> COUNT_BITS, &a0, &a1, &b0, &b1);
+            bool ha, hb;
             float ea, eb;
-            const bool ha = slab(a0, a1, o, inv, t_min, closest, &ea);
-            const bool hb = slab(b0, b1, o, inv, t_min, closest, &eb);
-            const uint32_t ca = pack_child(__float_as_uint(a0.w), __float_as_uint(a1.w));
-            const uint32_t cb = pack_child(__float_as_uint(b0.w), __float_as_uint(b1.w));
+            uint32_t ca, cb;
+            test_pair(sc.nodes, cur >> COUNT_BITS, o, inv, t_min, closest, &ha, &hb, &ea, &eb, &ca, &cb);
             if (ha && hb) {
                 const bool a_near = ea <= eb;
                 if (!st.push(a_near ? cb : ca)) *overflow += 1;
