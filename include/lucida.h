@@ -303,6 +303,20 @@ lucida_status lucida_sort_u32(lucida_ctx* ctx, const uint32_t* keys, const uint3
 lucida_status lucida_sort_u32_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys,
                                      uint32_t* tmp_vals, size_t n);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lucida_sort_u32_device with its histogram and scan
+ * scratch in a fixed per-GPU array inside the library (8 MiB + 4 KiB, held
+ * until the process ends) instead of the context's growing buffer. It calls
+ * no allocator. n may be at most LUCIDA_SORT_NOALLOC_MAX_KEYS (2^24); larger
+ * n is LUCIDA_INVALID_ARG. Queued on the context's stream like
+ * lucida_sort_u32_device. Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other streams, other contexts, other host
+ * threads) race and must be ordered by the caller. */
+#define LUCIDA_SORT_NOALLOC_MAX_KEYS (1u << 24)
+lucida_status lucida_sort_u32_device_noalloc(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys,
+                                             uint32_t* tmp_vals, size_t n);
+
 /* GpuBuildOutput (crates/gpu/src/lbvh.rs:724-733). Caller-owned host arrays of
  * the stated lengths (n = triangle count). */
 typedef struct lucida_lbvh_parts {

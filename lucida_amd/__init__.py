@@ -120,6 +120,7 @@ class BlobParams(C.Structure):
 
 SAMPLING_BSDF, SAMPLING_NEE, SAMPLING_MIS = 0, 1, 2
 SAMPLER_INDEPENDENT, SAMPLER_SOBOL = 0, 1
+SORT_NOALLOC_MAX_KEYS = 1 << 24  # LUCIDA_SORT_NOALLOC_MAX_KEYS
 
 # ---------------------------------------------------------------------------
 # Library loading
@@ -161,6 +162,7 @@ def lib() -> C.CDLL:
             "lucida_ctx_set_lbvh_fit": ([P, I], I),
             "lucida_sort_u32": ([P, P, P, S, P, P], I),
             "lucida_sort_u32_device": ([P, P, P, P, P, S], I),
+            "lucida_sort_u32_device_noalloc": ([P, P, P, P, P, S], I),
             "lucida_lbvh_build_parts": ([P, P, S, P, S, C.POINTER(LbvhParts)], I),
             "lucida_lbvh_build": ([P, P, S, P, S, P, C.POINTER(S), P], I),
             "lucida_lbvh_build_device": ([P, P, S, P, P, C.POINTER(S), P, P, P], I),

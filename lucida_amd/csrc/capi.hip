@@ -110,6 +110,11 @@ lucida_status lucida_ctx_create(int device, lucida_ctx** out) {
         delete ctx;
         return set_error(LUCIDA_NO_DEVICE, "could not create the context's streams or events");
     }
+    if (sort_scratch_init(ctx) != LUCIDA_OK) {
+        std::string msg = lucida_last_error();
+        lucida_ctx_destroy(ctx);
+        return set_error(LUCIDA_NO_DEVICE, msg.c_str());
+    }
     *out = ctx;
     return LUCIDA_OK;
 }

@@ -121,6 +121,10 @@ lucida_status ensure_ggx_table(lucida_ctx* ctx);
 lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
                                 GpuBvhNode* nodes, size_t* num_nodes, uint32_t* prim_indices,
                                 GpuTriangle* tris_sorted, float* stage_us);
+// Resolves the current device's copy of the sort's fixed scratch array
+// (lbvh.hip), once per device; lucida_ctx_create calls it with ctx->device
+// current.
+lucida_status sort_scratch_init(lucida_ctx* ctx);
 // Exclusive prefix sum of device words in place (lbvh.hip); `partials` holds
 // ceil(count / 2048) words.
 lucida_status scan_u32_device(lucida_ctx* ctx, uint32_t* data, uint32_t count, uint32_t* partials);

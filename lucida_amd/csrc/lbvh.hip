@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "context.hpp"
@@ -685,17 +686,21 @@ lucida_status scan_u32_device(lucida_ctx* ctx, uint32_t* data, uint32_t count, u
     return LUCIDA_OK;
 }
 
+// Words of histogram + scan scratch a sort of n keys uses.
+static size_t sort_scratch_words(size_t n) {
+    const size_t hist_words = blocks_for(n, SORT_TILE) * (size_t)RADIX;
+    return hist_words + blocks_for(hist_words, SCAN_SEG);
+}
+
 // Radix-sorts device arrays keys/vals in place over bits [0, bits), using
-// tmp arrays of the same length and ctx's histogram scratch.
-static lucida_status sort_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
-                                 size_t n, uint32_t bits) {
-    if (n == 0) return LUCIDA_OK;
-    if (n > 0xFFFFFFFFull / 2) return set_error(LUCIDA_INVALID_ARG, "sort: more than 2^31 keys");
+// tmp arrays of the same length and `work` (sort_scratch_words(n) words,
+// every word written before it is read, so it needs no clearing).
+static lucida_status sort_device_with(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
+                                      size_t n, uint32_t bits, uint32_t* work) {
     const uint32_t tiles = blocks_for(n, SORT_TILE);
     const uint32_t hist_words = tiles * RADIX;
     const uint32_t segs = blocks_for(hist_words, SCAN_SEG);
-    LUCIDA_HIP(ctx->lb_hist.ensure(((size_t)hist_words + segs) * 4), LUCIDA_OOM);
-    uint32_t* hist = ctx->lb_hist.as<uint32_t>();
+    uint32_t* hist = work;
     uint32_t* partials = hist + hist_words;
     uint32_t *src_k = keys, *src_v = vals, *dst_k = tk, *dst_v = tv;
     const uint32_t passes = (bits + 7) / 8;
@@ -719,6 +724,51 @@ static lucida_status sort_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals
     return LUCIDA_OK;
 }
 
+// The same sort with its scratch in the context's (growing) histogram buffer.
+static lucida_status sort_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
+                                 size_t n, uint32_t bits) {
+    if (n == 0) return LUCIDA_OK;
+    if (n > 0xFFFFFFFFull / 2) return set_error(LUCIDA_INVALID_ARG, "sort: more than 2^31 keys");
+    LUCIDA_HIP(ctx->lb_hist.ensure(sort_scratch_words(n) * 4), LUCIDA_OOM);
+    return sort_device_with(ctx, keys, vals, tk, tv, n, bits, ctx->lb_hist.as<uint32_t>());
+}
+
+// ---------------------------------------------------------------------------
+// Sort scratch that is never allocated: a file-scope device array, one copy
+// per GPU, for callers that cannot allocate (code under graph capture, tight
+// serving loops). 8 MiB + 4 KiB covers LUCIDA_SORT_NOALLOC_MAX_KEYS keys.
+// Host code never passes `scratch` itself (its host-side name is not a
+// device address): each device's address comes from hipGetSymbolAddress,
+// made once per device with that device current, and is kept in
+// g_scratch_ptr.
+// ---------------------------------------------------------------------------
+
+constexpr size_t SORT_NOALLOC_WORDS =
+    (LUCIDA_SORT_NOALLOC_MAX_KEYS / SORT_TILE) * RADIX + (LUCIDA_SORT_NOALLOC_MAX_KEYS / SORT_TILE) * RADIX / SCAN_SEG;
+static_assert(LUCIDA_SORT_NOALLOC_MAX_KEYS % SORT_TILE == 0, "max keys is a whole number of tiles");
+static_assert((LUCIDA_SORT_NOALLOC_MAX_KEYS / SORT_TILE) * RADIX % SCAN_SEG == 0, "whole scan segments");
+
+static __device__ __align__(256) uint32_t scratch[SORT_NOALLOC_WORDS];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static uint32_t* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Resolves this device's copy of `scratch` (the current device must be
+// ctx->device). Called from lucida_ctx_create; later contexts on the same
+// device reuse the address.
+lucida_status sort_scratch_init(lucida_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= MAX_DEVICES)
+        return set_error(LUCIDA_NO_DEVICE, "sort scratch: device index beyond the supported 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[ctx->device]) {
+        uint32_t* ptr = nullptr;
+        LUCIDA_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)), LUCIDA_NO_DEVICE);
+        g_scratch_ptr[ctx->device] = ptr;
+    }
+    return LUCIDA_OK;
+}
+
 size_t sort_counted_scratch_words(size_t capacity) {
     const size_t hist_words = blocks_for(capacity, SORT_TILE) * (size_t)RADIX;
     return hist_words + blocks_for(hist_words, SCAN_SEG);
@@ -733,7 +783,7 @@ size_t sort_counted_scratch_words(size_t capacity) {
 // values. Every launch is a kernel (no memset or copy), so the sort may be
 // captured in a graph.
 lucida_status sort_pairs_counted(hipStream_t stream, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
-                                 size_t capacity, const uint32_t* count, uint32_t bits, uint32_t* scratch,
+                                 size_t capacity, const uint32_t* count, uint32_t bits, uint32_t* work,
                                  uint32_t** sorted_vals) {
     if (sorted_vals) *sorted_vals = vals;
     if (capacity == 0) return LUCIDA_OK;
@@ -742,8 +792,8 @@ lucida_status sort_pairs_counted(hipStream_t stream, uint32_t* keys, uint32_t* v
     const uint32_t tiles = blocks_for(capacity, SORT_TILE);
     const uint32_t hist_words = tiles * RADIX;
     const uint32_t segs = blocks_for(hist_words, SCAN_SEG);
-    uint32_t* hist = scratch;
-    uint32_t* partials = scratch + hist_words;
+    uint32_t* hist = work;
+    uint32_t* partials = work + hist_words;
     uint32_t *src_k = keys, *src_v = vals, *dst_k = tk, *dst_v = tv;
     for (uint32_t shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_BLOCK), 0, stream, src_k, (uint32_t)capacity, count,
@@ -964,6 +1014,23 @@ lucida_status lucida_sort_u32_device(lucida_ctx* ctx, uint32_t* keys, uint32_t* 
         return set_error(LUCIDA_INVALID_ARG, "lucida_sort_u32_device: null argument");
     clear_error();
     return sort_device(ctx, keys, vals, tk, tv, n, 32);
+}
+
+lucida_status lucida_sort_u32_device_noalloc(lucida_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* tk,
+                                             uint32_t* tv, size_t n) {
+    if (!ctx || (n && (!keys || !vals || !tk || !tv)))
+        return set_error(LUCIDA_INVALID_ARG, "lucida_sort_u32_device_noalloc: null argument");
+    if (n > LUCIDA_SORT_NOALLOC_MAX_KEYS)
+        return set_error(LUCIDA_INVALID_ARG, "lucida_sort_u32_device_noalloc: more than LUCIDA_SORT_NOALLOC_MAX_KEYS "
+                                             "(2^24) keys; use lucida_sort_u32_device");
+    clear_error();
+    if (n == 0) return LUCIDA_OK;
+    uint32_t* ptr = ctx->device >= 0 && ctx->device < MAX_DEVICES ? g_scratch_ptr[ctx->device] : nullptr;
+    if (!ptr || sort_scratch_words(n) > SORT_NOALLOC_WORDS)
+        return set_error(LUCIDA_VALIDATION, "lucida_sort_u32_device_noalloc: the device's sort scratch is not set up");
+    // 32 key bits: four passes, so the result lands back in keys/vals and no
+    // copy is queued.
+    return sort_device_with(ctx, keys, vals, tk, tv, n, 32, ptr);
 }
 
 lucida_status lucida_sort_u32(lucida_ctx* ctx, const uint32_t* keys, const uint32_t* vals, size_t n,
