@@ -30,6 +30,18 @@ typedef struct lucida_blob_params {
  *   "cornell-box"   the reference's cornell_box() (scenes.rs:56), analytic
  *   "cornell-mesh"  the reference's cornell_mesh() (scenes.rs:174), triangles
  *                   in generation order (no BVH, no compaction)
+ *   "bvh-stress"    bvh_stress() (scenes.rs:218): 72 level-4 icospheres
+ *                   (368,640 triangles) in the analytic Cornell box
+ *   "furnace-test"  furnace_test() (scenes.rs:275): white metal spheres of
+ *                   roughness 0..1 under a uniform environment of 1
+ *   "metal-sweep"   metal_sweep() (scenes.rs:327): copper and gold conductors
+ *   "mis-scene"     mis_scene() (scenes.rs:410): Veach's plates and emitters
+ *   "glass-box"     glass_box() (scenes.rs:536): clear, diamond, frosted and
+ *                   tinted dielectrics in a Cornell box
+ *   "sunset"        sunset() (scenes.rs:651): lit only by procedural_sky
+ *                   (envmap.rs:343), 256 x 128
+ *   "instance-forest" instance_forest() (scenes.rs:733): 49 placements of one
+ *                   icosphere under a procedural sky (instanced)
  *   "box-c1"        the 1x1x1 triangle-quad Cornell box with two spheres
  *   "sphere-c2"     the noise-displaced geodesic sphere in a 1,024-triangle
  *                   room lit by 16 emissive quads
@@ -60,6 +72,25 @@ lucida_status lucida_blob_set_bvh(lucida_blob* blob, const GpuBvhNode* nodes, si
 /* The meshes and placements of an instanced scene (empty otherwise). */
 lucida_status lucida_blob_instancing(const lucida_blob* blob, const lucida_mesh_range** meshes, size_t* num_meshes,
                                      const lucida_instance_placement** placements, size_t* num_placements);
+
+/* Camera (camera.rs:14-26): eye, look-at point, up hint, vertical field of
+ * view in degrees, lens aperture (diameter), focus distance. */
+typedef struct lucida_camera {
+    float eye[3];
+    float look_at[3];
+    float up[3];
+    float vfov_deg;
+    float aperture;
+    float focus_distance;
+} lucida_camera;
+
+/* Camera::write_uniforms (camera.rs:55): writes cam_origin, cam_upper_left,
+ * cam_horizontal, cam_vertical, lens_radius and focus_distance of *u for an
+ * image of the given aspect (width / height); other fields are untouched. */
+lucida_status lucida_camera_write_uniforms(const lucida_camera* cam, float aspect, GpuUniforms* u);
+
+/* The camera a built-in scene carries (SceneDef::camera, scenes.rs:11-18). */
+lucida_status lucida_blob_camera(const lucida_blob* blob, lucida_camera* out);
 
 /* Replaces the blob's environment map (RGBA32F, w*h) and builds its 2D CDF
  * (envmap.rs:112 EnvMap::new). */

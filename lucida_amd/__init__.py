@@ -114,6 +114,17 @@ class DenoiseParams(C.Structure):
         return cls(4, 64.0, 0.08, 4.0, 0.1)
 
 
+class Camera(C.Structure):
+    """lucida_camera: Camera (crates/core/src/camera.rs:14-26)."""
+    _fields_ = [("eye", C.c_float * 3), ("look_at", C.c_float * 3), ("up", C.c_float * 3), ("vfov_deg", C.c_float),
+                ("aperture", C.c_float), ("focus_distance", C.c_float)]
+
+    def write_uniforms(self, u: "Uniforms", aspect: float) -> "Uniforms":
+        """Camera::write_uniforms (camera.rs:55) into the camera fields of u."""
+        _check(lib().lucida_camera_write_uniforms(C.byref(self), aspect, C.byref(u)))
+        return u
+
+
 class BlobParams(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("frequency", C.c_uint32), ("displacement", C.c_float)]
 
@@ -202,6 +213,8 @@ def lib() -> C.CDLL:
             "lucida_blob_uniforms": ([P, U32, U32, C.POINTER(Uniforms)], I),
             "lucida_blob_set_bvh": ([P, P, S, P], I),
             "lucida_blob_set_env": ([P, U32, U32, P], I),
+            "lucida_blob_camera": ([P, C.POINTER(Camera)], I),
+            "lucida_camera_write_uniforms": ([C.POINTER(Camera), C.c_float, C.POINTER(Uniforms)], I),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(L, name, None)
@@ -308,6 +321,12 @@ class Blob:
         u.sampler_kind = sampler
         u.sample_offset = sample_offset
         return u
+
+    def camera(self) -> Camera:
+        """The scene's camera (SceneDef::camera)."""
+        c = Camera()
+        _check(lib().lucida_blob_camera(self._h, C.byref(c)))
+        return c
 
     def set_bvh(self, nodes: np.ndarray, triangles_in_leaf_order: np.ndarray | None) -> None:
         nodes = np.ascontiguousarray(nodes, dtype=BVH_NODE)
@@ -430,6 +449,18 @@ class Scene:
         self._h = h
         gpu._adopt(self)
         self.num_triangles = d.num_triangles
+
+    @classmethod
+    def from_desc(cls, gpu: Gpu, desc: SceneDesc) -> "Scene":
+        """Uploads a caller-built lucida_scene_desc (its arrays are copied)."""
+        self = cls.__new__(cls)
+        self.gpu = gpu
+        h = C.c_void_p()
+        _check(lib().lucida_scene_create(gpu._h, C.byref(desc), C.byref(h)))
+        self._h = h
+        gpu._adopt(self)
+        self.num_triangles = desc.num_triangles
+        return self
 
     def counts(self) -> dict:
         nt, nn, ni, root = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_uint32()

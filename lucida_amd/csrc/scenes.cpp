@@ -5,7 +5,8 @@
 // GpuPrimitive::quad/sphere (gpu_layout.rs:214-240), MeshBuilder::append_to,
 // quad and icosphere (mesh.rs:20-249), build_lights (light.rs:62-120),
 // Camera::look_at / basis / write_uniforms (camera.rs:28-80), EnvMap::new
-// (envmap.rs:112-180), cornell_box / cornell_mesh (scenes.rs:56-214). The two
+// (envmap.rs:112-180), procedural_sky (envmap.rs:343), the reference's nine
+// built-in scenes (scenes.rs:56-815) and the rng.rs PCG stream. The two
 // benchmark scenes ("box-c1", "sphere-c2") are new generators written in the
 // same idioms.
 #include <algorithm>
@@ -46,6 +47,54 @@ static GpuMaterial mat_emissive(f3 radiance) {
     radiance.store(m.emissive);
     m.roughness = 1.0f;
     m.ior = 1.0f;
+    return m;
+}
+static GpuMaterial mat_metal(f3 c, float roughness) {
+    GpuMaterial m{};
+    c.store(m.base_color);
+    m.metallic = 1.0f;
+    m.roughness = roughness;
+    m.ior = 1.5f;
+    return m;
+}
+static GpuMaterial mat_glass(f3 tint, float roughness, float ior) {
+    GpuMaterial m{};
+    tint.store(m.base_color);
+    m.roughness = roughness;
+    m.ior = ior;
+    m.transmission = 1.0f;
+    return m;
+}
+// bsdf.rs:358 fresnel_conductor, needed on the host for the base colour the
+// conductor constructor stores (its normal-incidence reflectance).
+static f3 fresnel_conductor_host(float cos_i, f3 eta, f3 k) {
+    const float c = std::min(std::max(cos_i, 0.0f), 1.0f);
+    const float c2 = c * c, sin2 = 1.0f - c2;
+    f3 out;
+    for (int a = 0; a < 3; ++a) {
+        const float e = eta[a], kk = k[a];
+        const float eta2 = e * e, k2 = kk * kk;
+        const float t0 = eta2 - k2 - sin2;
+        const float a2pb2 = std::pow(std::max(t0 * t0 + 4.0f * eta2 * k2, 0.0f), 0.5f);
+        const float t1 = a2pb2 + c2;
+        const float aa = std::pow(std::max(0.5f * (a2pb2 + t0), 0.0f), 0.5f);
+        const float t2 = 2.0f * c * aa;
+        const float rs = (t1 - t2) / (t1 + t2);
+        const float t3 = c2 * a2pb2 + sin2 * sin2;
+        const float t4 = t2 * sin2;
+        const float rp = rs * (t3 - t4) / (t3 + t4);
+        (a == 0 ? out.x : (a == 1 ? out.y : out.z)) = 0.5f * (rs + rp);
+    }
+    return out;
+}
+static GpuMaterial mat_conductor(f3 eta, f3 k, float roughness) {
+    GpuMaterial m{};
+    fresnel_conductor_host(1.0f, eta, k).store(m.base_color);
+    m.metallic = 1.0f;
+    m.roughness = roughness;
+    m.ior = 1.5f;
+    eta.store(m.eta);
+    k.store(m.k);
     return m;
 }
 static GpuPrimitive prim_quad(f3 origin, f3 eu, f3 ev, uint32_t material) {
@@ -206,23 +255,29 @@ static void camera_look_at(Blob& b, f3 eye, f3 look_at, float vfov_deg) {
     b.cam_focus_distance = length(look_at - eye);
 }
 
-static void camera_write_uniforms(const Blob& b, GpuUniforms& u, float aspect) {
-    f3 forward = normalize(b.cam_look_at - b.cam_eye);
-    f3 right = normalize(cross(forward, b.cam_up));
+// Camera::basis + Camera::write_uniforms (camera.rs:44-80).
+static void write_camera(f3 eye, f3 look_at, f3 up_hint, float vfov_deg, float aperture, float focus_distance,
+                         GpuUniforms& u, float aspect) {
+    f3 forward = normalize(look_at - eye);
+    f3 right = normalize(cross(forward, up_hint));
     f3 up = cross(right, forward);
     f3 down = -up;
     const float DEG = 3.14159265358979323846f / 180.0f;
-    float half_height = std::tan(b.cam_vfov_deg * DEG * 0.5f) * b.cam_focus_distance;
+    float half_height = std::tan(vfov_deg * DEG * 0.5f) * focus_distance;
     float half_width = half_height * aspect;
     f3 horizontal = right * (2.0f * half_width);
     f3 vertical = down * (2.0f * half_height);
-    f3 upper_left = b.cam_eye + forward * b.cam_focus_distance - right * half_width - down * half_height;
-    b.cam_eye.store(u.cam_origin);
+    f3 upper_left = eye + forward * focus_distance - right * half_width - down * half_height;
+    eye.store(u.cam_origin);
     upper_left.store(u.cam_upper_left);
     horizontal.store(u.cam_horizontal);
     vertical.store(u.cam_vertical);
-    u.lens_radius = b.cam_aperture * 0.5f;
-    u.focus_distance = b.cam_focus_distance;
+    u.lens_radius = aperture * 0.5f;
+    u.focus_distance = focus_distance;
+}
+
+static void camera_write_uniforms(const Blob& b, GpuUniforms& u, float aspect) {
+    write_camera(b.cam_eye, b.cam_look_at, b.cam_up, b.cam_vfov_deg, b.cam_aperture, b.cam_focus_distance, u, aspect);
 }
 
 // --- environment map (envmap.rs:112-180) --------------------------------------
@@ -268,7 +323,65 @@ static void env_build(Blob& b, uint32_t width, uint32_t height, const float* rgb
     }
 }
 
-// --- reference scenes (scenes.rs:56-214) ---------------------------------------
+// procedural_sky (envmap.rs:343): a sky gradient over a dim ground with a sun
+// disc widened to at least one texel and dimmed by the area ratio, then the
+// CDF build. Pixels' fourth channel is 0, as in the reference.
+static void procedural_sky(Blob& b, uint32_t width, uint32_t height, f3 sun_direction, float sun_intensity,
+                           float sun_radius_degrees) {
+    const float PI_F = 3.14159265358979323846f, TAU_F = 6.28318530717958647692f;
+    const f3 sun = normalize_or(sun_direction, f3(0, 1, 0));
+    const float texel = std::max(PI_F / (float)height, TAU_F / (float)width);
+    const float requested = sun_radius_degrees * (PI_F / 180.0f);
+    const float radius = std::max(requested, texel);
+    const float area_ratio = (1.0f - std::cos(requested)) / std::max(1.0f - std::cos(radius), 1e-20f);
+    const float intensity = sun_intensity * std::min(area_ratio, 1.0f);
+    const float cos_sun_radius = std::cos(radius);
+    std::vector<float> px;
+    px.reserve((size_t)width * height * 4);
+    for (uint32_t j = 0; j < height; ++j) {
+        const float v = ((float)j + 0.5f) / (float)height;
+        for (uint32_t i = 0; i < width; ++i) {
+            const float u = ((float)i + 0.5f) / (float)width;
+            const float theta = v * PI_F, phi = u * TAU_F;
+            const float st = std::sin(theta), ct = std::cos(theta);
+            const f3 d(st * std::cos(phi), ct, st * std::sin(phi));
+            f3 c;
+            if (d.y >= 0.0f) {
+                const float t = std::pow(d.y, 0.45f);
+                c = f3(0.75f, 0.86f, 1.00f) * t + f3(0.85f, 0.70f, 0.55f) * (1.0f - t);
+            } else {
+                c = f3::splat(0.12f);
+            }
+            c = c * 0.40f;
+            if (dot(d, sun) >= cos_sun_radius) c = c + f3(1.0f, 0.92f, 0.78f) * intensity;
+            px.insert(px.end(), {c.x, c.y, c.z, 0.0f});
+        }
+    }
+    env_build(b, width, height, px.data());
+}
+
+// rng.rs:28-85 PCG-RXS-M-XS, counter-seeded; the instance scatter draws from it.
+struct Pcg32 {
+    uint32_t state;
+    static uint32_t hash(uint32_t x) {
+        uint32_t st = x * 747796405u + 2891336453u;
+        uint32_t w = ((st >> ((st >> 28) + 4u)) ^ st) * 277803737u;
+        return (w >> 22) ^ w;
+    }
+    Pcg32(uint32_t frame, uint32_t pixel, uint32_t sample) {
+        uint32_t h = hash(frame);
+        h = hash(h + pixel);
+        state = hash(h + sample);
+    }
+    uint32_t next_u32() {
+        state = state * 747796405u + 2891336453u;
+        uint32_t w = ((state >> ((state >> 28) + 4u)) ^ state) * 277803737u;
+        return (w >> 22) ^ w;
+    }
+    float next_f32() { return (float)(next_u32() >> 8) * (1.0f / 16777216.0f); }
+};
+
+// --- reference scenes (scenes.rs:56-815) ---------------------------------------
 
 static void cornell_box_ref(Blob& b) {
     const float S = 555.0f;
@@ -304,6 +417,166 @@ static void cornell_mesh_ref(Blob& b) {
         mesh_icosphere(f3(p.position[0], p.position[1], p.position[2]), p.radius, 4).append_to(b, sphere_mats[k]);
         ++k;
     }
+}
+
+// bvh_stress (scenes.rs:218): the Cornell box's quads with a 6 x 2 x 6 lattice
+// of level-4 icospheres (radius 30) in two alternating materials.
+static void bvh_stress_ref(Blob& b) {
+    const float S = 555.0f;
+    cornell_box_ref(b);
+    std::vector<GpuPrimitive> quads;
+    for (const auto& p : b.primitives)
+        if (p.kind == LUCIDA_PRIM_KIND_QUAD) quads.push_back(p);
+    b.primitives.swap(quads);
+    const uint32_t nx = 6, ny = 2, nz = 6;
+    auto f = [&](uint32_t i, uint32_t count) { return ((float)i + 1.0f) / ((float)count + 1.0f) * S; };
+    for (uint32_t ix = 0; ix < nx; ++ix)
+        for (uint32_t iy = 0; iy < ny; ++iy)
+            for (uint32_t iz = 0; iz < nz; ++iz) {
+                const f3 centre(f(ix, nx), f(iy, ny) * 0.62f + 95.0f, f(iz, nz));
+                const uint32_t material = (ix + iy + iz) % 2 == 0 ? 4u : 5u;
+                mesh_icosphere(centre, 30.0f, 4).append_to(b, material);
+            }
+}
+
+// furnace_test (scenes.rs:275): five white metal spheres, roughness 0..1, in
+// a uniform environment of radiance 1.
+static void furnace_test_ref(Blob& b) {
+    const int N = 5;
+    for (int i = 0; i < N; ++i) {
+        const float roughness = (float)i / (float)(N - 1);
+        b.materials.push_back(mat_metal(f3::splat(1.0f), roughness));
+        const float x = ((float)i - (float)(N - 1) * 0.5f) * 2.4f;
+        b.primitives.push_back(prim_sphere(f3(x, 0.0f, 0.0f), 1.0f, (uint32_t)i));
+    }
+    camera_look_at(b, f3(0.0f, 0.8f, -12.0f), f3(0, 0, 0), 45.0f);
+    b.background = f3::splat(1.0f);
+}
+
+// metal_sweep (scenes.rs:327): copper and gold rows (measured complex IOR,
+// bsdf.rs conductors) of increasing roughness over a floor.
+static void metal_sweep_ref(Blob& b) {
+    const int N = 6;
+    const f3 COPPER_ETA(0.200438f, 0.924033f, 1.102212f), COPPER_K(3.9124f, 2.44753f, 2.13764f);
+    const f3 GOLD_ETA(0.143245f, 0.375346f, 1.442386f), GOLD_K(3.98312f, 2.385704f, 1.603575f);
+    b.materials.push_back(mat_diffuse(f3::splat(0.35f)));
+    for (int row = 0; row < 2; ++row) {
+        const f3 eta = row == 0 ? COPPER_ETA : GOLD_ETA, k = row == 0 ? COPPER_K : GOLD_K;
+        for (int i = 0; i < N; ++i) {
+            const float t = (float)i / (float)(N - 1);
+            const float roughness = 0.02f + 0.78f * t * t;
+            b.materials.push_back(mat_conductor(eta, k, roughness));
+            const float x = ((float)(N - 1) * 0.5f - (float)i) * 2.3f;
+            b.primitives.push_back(
+                prim_sphere(f3(x, 1.0f, (float)row * 2.6f), 1.0f, (uint32_t)b.materials.size() - 1));
+        }
+    }
+    b.primitives.push_back(prim_quad(f3(-40.0f, 0.0f, -40.0f), f3(0.0f, 0.0f, 80.0f), f3(80.0f, 0.0f, 0.0f), 0));
+    camera_look_at(b, f3(0.0f, 3.4f, -13.0f), f3(0.0f, 1.0f, 1.3f), 40.0f);
+    b.background = f3::splat(0.75f);
+}
+
+// mis_scene (scenes.rs:410): Veach's plates of increasing roughness, each
+// tilted to reflect the emitter row to the camera, and four emitters of equal
+// power and increasing size.
+static void mis_scene_ref(Blob& b) {
+    b.materials.push_back(mat_diffuse(f3::splat(0.25f)));
+    const f3 eye(0.0f, 3.0f, -8.5f);
+    const float light_y = 5.4f, light_z = 1.2f, depth = 1.5f;
+    const float rough[4] = {0.045f, 0.09f, 0.18f, 0.35f};
+    for (int i = 0; i < 4; ++i) {
+        b.materials.push_back(mat_metal(f3::splat(0.85f), rough[i]));
+        const f3 centre(0.0f, 0.62f * (float)i, 0.6f + (float)i * 1.9f);
+        const f3 to_camera = normalize(eye - centre);
+        const f3 to_light = normalize(f3(0.0f, light_y, light_z) - centre);
+        const f3 n = normalize(to_camera + to_light);
+        const f3 edge_u(0.0f, -depth * n.z, depth * n.y);
+        const f3 edge_v(10.0f, 0.0f, 0.0f);
+        const f3 origin = centre - 0.5f * edge_u - 0.5f * edge_v;
+        b.primitives.push_back(prim_quad(origin, edge_u, edge_v, (uint32_t)b.materials.size() - 1));
+    }
+    const float POWER = 46.0f;
+    const float half[4] = {0.055f, 0.16f, 0.46f, 1.15f};
+    for (int j = 0; j < 4; ++j) {
+        const float hw = half[j];
+        const float x = 3.7f - (float)j * 2.45f;
+        const float area = 4.0f * hw * hw;
+        b.materials.push_back(mat_emissive(f3::splat(POWER / area)));
+        b.primitives.push_back(prim_quad(f3(x - hw, light_y, light_z - hw), f3(2.0f * hw, 0.0f, 0.0f),
+                                         f3(0.0f, 0.0f, 2.0f * hw), (uint32_t)b.materials.size() - 1));
+    }
+    b.primitives.push_back(prim_quad(f3(-16.0f, -3.0f, 10.5f), f3(0.0f, 16.0f, 0.0f), f3(32.0f, 0.0f, 0.0f), 0));
+    camera_look_at(b, eye, f3(0.0f, 1.35f, 3.4f), 42.0f);
+    b.background = f3::splat(0.015f);
+}
+
+// glass_box (scenes.rs:536): the Cornell walls with a wider light, clear,
+// diamond and frosted glass spheres and a tinted pane.
+static void glass_box_ref(Blob& b) {
+    const float S = 555.0f;
+    b.materials = {mat_diffuse(f3(0.725f, 0.710f, 0.680f)), mat_diffuse(f3(0.630f, 0.065f, 0.050f)),
+                   mat_diffuse(f3(0.140f, 0.450f, 0.091f)), mat_emissive(f3(30.0f, 26.0f, 18.0f)),
+                   mat_glass(f3::splat(1.0f), 0.0f, 1.5f),  mat_glass(f3::splat(1.0f), 0.0f, 2.42f),
+                   mat_glass(f3::splat(1.0f), 0.25f, 1.5f), mat_glass(f3(0.35f, 0.75f, 0.55f), 0.02f, 1.5f)};
+    b.primitives = {prim_quad(f3(0, 0, 0), f3(0, 0, S), f3(S, 0, 0), 0),
+                    prim_quad(f3(0, S, 0), f3(S, 0, 0), f3(0, 0, S), 0),
+                    prim_quad(f3(0, 0, S), f3(0, S, 0), f3(S, 0, 0), 0),
+                    prim_quad(f3(S, 0, 0), f3(0, 0, S), f3(0, S, 0), 1),
+                    prim_quad(f3(0, 0, 0), f3(0, S, 0), f3(0, 0, S), 2),
+                    prim_quad(f3(163.0f, S - 0.1f, 177.0f), f3(230.0f, 0, 0), f3(0, 0, 205.0f), 3),
+                    prim_quad(f3(60.0f, 40.0f, 430.0f), f3(0, 300.0f, 0), f3(230.0f, 0, 0), 7),
+                    prim_sphere(f3(150.0f, 110.0f, 160.0f), 110.0f, 4),
+                    prim_sphere(f3(390.0f, 95.0f, 140.0f), 95.0f, 5),
+                    prim_sphere(f3(300.0f, 85.0f, 330.0f), 85.0f, 6)};
+    camera_look_at(b, f3(278.0f, 278.0f, -800.0f), f3(278.0f, 278.0f, 0.0f), 37.0f);
+    b.background = f3(0, 0, 0);
+}
+
+// sunset (scenes.rs:651): a diffuse, a near-mirror and a glass sphere on a
+// ground quad, lit only by a 256 x 128 procedural sky with a low sun.
+static void sunset_ref(Blob& b) {
+    b.materials = {mat_diffuse(f3(0.38f, 0.35f, 0.30f)), mat_diffuse(f3::splat(0.78f)),
+                   mat_metal(f3(0.95f, 0.93f, 0.88f), 0.06f), mat_glass(f3::splat(1.0f), 0.0f, 1.5f)};
+    b.primitives = {prim_quad(f3(-40.0f, 0.0f, -40.0f), f3(80.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 80.0f), 0),
+                    prim_sphere(f3(-2.4f, 1.0f, 0.0f), 1.0f, 1), prim_sphere(f3(0.0f, 1.0f, 0.0f), 1.0f, 2),
+                    prim_sphere(f3(2.4f, 1.0f, 0.0f), 1.0f, 3)};
+    procedural_sky(b, 256, 128, f3(-0.45f, 0.22f, -0.86f), 6000.0f, 1.0f);
+    camera_look_at(b, f3(0.0f, 1.6f, 7.5f), f3(0.0f, 0.9f, 0.0f), 42.0f);
+    b.background = f3(0, 0, 0);
+}
+
+// instance_forest (scenes.rs:733): 49 placements of one level-2 icosphere on
+// a 7 x 7 jittered grid, rotated about y and scaled non-uniformly (PCG seed
+// 0x1057), every third one gold, under a procedural sky. The placement matrix
+// is translation * rotation_y * scale, whose entries are single products.
+static void instance_forest_ref(Blob& b) {
+    b.materials = {mat_diffuse(f3(0.34f, 0.32f, 0.29f)), mat_glossy(f3(0.70f, 0.35f, 0.25f), 0.35f),
+                   mat_metal(f3(0.95f, 0.80f, 0.40f), 0.18f)};
+    b.primitives = {prim_quad(f3(-30.0f, 0.0f, -30.0f), f3(60.0f, 0.0f, 0.0f), f3(0.0f, 0.0f, 60.0f), 0)};
+    mesh_icosphere(f3(0, 0, 0), 0.5f, 2).append_to(b, 1);
+    b.meshes.push_back(lucida_mesh_range{0, (uint32_t)b.triangles.size()});
+    Pcg32 rng(0x1057u, 0u, 0u);
+    const float TAU_F = 6.28318530717958647692f;
+    for (int gx = 0; gx < 7; ++gx)
+        for (int gz = 0; gz < 7; ++gz) {
+            const float x = (float)gx * 1.6f - 4.8f + (rng.next_f32() - 0.5f) * 0.5f;
+            const float z = (float)gz * 1.6f - 4.8f + (rng.next_f32() - 0.5f) * 0.5f;
+            const float sx = 0.7f + rng.next_f32() * 0.7f;
+            const float sy = 0.5f + rng.next_f32() * 1.1f;
+            const float sz = 0.7f + rng.next_f32() * 0.7f;
+            const float spin = rng.next_f32() * TAU_F;
+            const float y = 0.5f * sy;
+            const float sn = std::sin(spin), cs = std::cos(spin);
+            lucida_instance_placement pl{};
+            const float m[12] = {cs * sx, 0.0f, sn * sz, x, 0.0f, sy, 0.0f, y, -sn * sx, 0.0f, cs * sz, z};
+            std::memcpy(pl.object_to_world, m, sizeof m);
+            pl.mesh = 0;
+            pl.material = (gx + gz) % 3 == 0 ? 2u : 0xFFFFFFFFu;
+            b.placements.push_back(pl);
+        }
+    procedural_sky(b, 256, 128, f3(-0.4f, 0.35f, -0.85f), 6000.0f, 1.0f);
+    camera_look_at(b, f3(0.0f, 3.2f, -11.0f), f3(0.0f, 0.7f, 0.0f), 40.0f);
+    b.background = f3(0, 0, 0);
 }
 
 // --- benchmark scene 1: the 1x1x1 triangle-quad box ---------------------------
@@ -656,6 +929,20 @@ lucida_status lucida_blob_create(const char* name, const lucida_blob_params* par
         cornell_box_ref(*b);
     } else if (n == "cornell-mesh") {
         cornell_mesh_ref(*b);
+    } else if (n == "bvh-stress") {
+        bvh_stress_ref(*b);
+    } else if (n == "furnace-test") {
+        furnace_test_ref(*b);
+    } else if (n == "metal-sweep") {
+        metal_sweep_ref(*b);
+    } else if (n == "mis-scene") {
+        mis_scene_ref(*b);
+    } else if (n == "glass-box") {
+        glass_box_ref(*b);
+    } else if (n == "sunset") {
+        sunset_ref(*b);
+    } else if (n == "instance-forest") {
+        instance_forest_ref(*b);
     } else if (n == "box-c1") {
         box_c1(*b);
     } else if (n == "sphere-c2") {
@@ -759,6 +1046,25 @@ lucida_status lucida_blob_instancing(const lucida_blob* blob, const lucida_mesh_
     *num_meshes = b.meshes.size();
     *placements = b.placements.data();
     *num_placements = b.placements.size();
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_camera_write_uniforms(const lucida_camera* cam, float aspect, GpuUniforms* u) {
+    if (!cam || !u || !(aspect > 0.0f)) return set_error(LUCIDA_INVALID_ARG, "lucida_camera_write_uniforms: bad argument");
+    write_camera(f3(cam->eye[0], cam->eye[1], cam->eye[2]), f3(cam->look_at[0], cam->look_at[1], cam->look_at[2]),
+                 f3(cam->up[0], cam->up[1], cam->up[2]), cam->vfov_deg, cam->aperture, cam->focus_distance, *u, aspect);
+    return LUCIDA_OK;
+}
+
+lucida_status lucida_blob_camera(const lucida_blob* blob, lucida_camera* out) {
+    if (!blob || !out) return set_error(LUCIDA_INVALID_ARG, "lucida_blob_camera: null argument");
+    const Blob& b = *reinterpret_cast<const Blob*>(blob);
+    b.cam_eye.store(out->eye);
+    b.cam_look_at.store(out->look_at);
+    b.cam_up.store(out->up);
+    out->vfov_deg = b.cam_vfov_deg;
+    out->aperture = b.cam_aperture;
+    out->focus_distance = b.cam_focus_distance;
     return LUCIDA_OK;
 }
 

@@ -61,6 +61,15 @@ def lib() -> C.CDLL:
             "oracle_bsdf_sample": ([P, U32, P, F, F, F, P], F),
             "oracle_bsdf_eval_pdf": ([P, U32, P, P, P], F),
             "oracle_bsdf_albedo": ([P, F, U32, U32], C.c_double),
+            "oracle_fresnel_dielectric": ([F, F], F),
+            "oracle_fresnel_conductor": ([F, P, P, P], None),
+            "oracle_refract": ([P, P, F, P], I),
+            "oracle_bsdf_split_albedo": ([P, U32, P, U32, U32, P], None),
+            "oracle_env_build": ([U32, U32, P, P, P], F),
+            "oracle_env_pdf": ([U32, U32, P, P, P, F, P, P], F),
+            "oracle_env_sample": ([U32, U32, P, P, P, F, F, F, P], F),
+            "oracle_env_direction_from_uv": ([F, F, P], None),
+            "oracle_env_uv_from_direction": ([P, P], None),
         }
         for name, (args, res) in sigs.items():
             try:
@@ -266,3 +275,86 @@ def bsdf_eval_pdf(material: np.ndarray, front: bool, wo, wi):
 def bsdf_albedo(material: np.ndarray, cos_o: float, n: int = 200_000, seed: int = 1) -> float:
     m = np.ascontiguousarray(material)
     return lib().oracle_bsdf_albedo(_p(m), cos_o, n, seed)
+
+
+# --- closed-form pins: Fresnel, refraction, split albedo ------------------------
+
+
+def fresnel_dielectric(cos_i: float, eta: float) -> float:
+    """fresnel_dielectric (crates/core/src/bsdf.rs:314)."""
+    return lib().oracle_fresnel_dielectric(cos_i, eta)
+
+
+def fresnel_conductor(cos_i: float, eta, k) -> np.ndarray:
+    """fresnel_conductor (crates/core/src/bsdf.rs:358), per RGB channel."""
+    eta = np.asarray(eta, np.float32)
+    k = np.asarray(k, np.float32)
+    out = np.empty(3, np.float32)
+    lib().oracle_fresnel_conductor(cos_i, _p(eta), _p(k), _p(out))
+    return out
+
+
+def refract(wo, n, eta: float):
+    """refract (crates/core/src/bsdf.rs:555): the transmitted direction or None (TIR)."""
+    wo = np.asarray(wo, np.float32)
+    n = np.asarray(n, np.float32)
+    out = np.empty(3, np.float32)
+    return out if lib().oracle_refract(_p(wo), _p(n), eta, _p(out)) else None
+
+
+def bsdf_split_albedo(material: np.ndarray, front: bool, wo, n: int, seed: int) -> tuple[float, float]:
+    """(reflected, transmitted) directional albedo: the mean luminance weight of
+    surface::sample split by the side of the surface wi lands on."""
+    m = np.ascontiguousarray(material)
+    wo = np.asarray(wo, np.float32)
+    out = np.empty(2, np.float64)
+    lib().oracle_bsdf_split_albedo(_p(m), int(front), _p(wo), n, seed, _p(out))
+    return float(out[0]), float(out[1])
+
+
+# --- environment map (envmap.rs) ----------------------------------------------------
+
+
+class EnvMap:
+    """EnvMap (crates/core/src/envmap.rs:61) over RGBA32F pixels (h, w, 4), with
+    the oracle's own CDF build (EnvMap::new, envmap.rs:112)."""
+
+    def __init__(self, pixels: np.ndarray):
+        self.pixels = np.ascontiguousarray(pixels, np.float32)
+        self.height, self.width, _ = self.pixels.shape
+        self.conditional = np.zeros((self.width + 1) * self.height, np.float32)
+        self.marginal = np.zeros(self.height + 1, np.float32)
+        self.total_weight = lib().oracle_env_build(self.width, self.height, _p(self.pixels), _p(self.conditional),
+                                                   _p(self.marginal))
+
+    def _args(self):
+        return (self.width, self.height, _p(self.pixels), _p(self.conditional), _p(self.marginal), self.total_weight)
+
+    def pdf(self, direction) -> float:
+        d = np.asarray(direction, np.float32)
+        return lib().oracle_env_pdf(*self._args(), _p(d), None)
+
+    def radiance(self, direction) -> np.ndarray:
+        d = np.asarray(direction, np.float32)
+        out = np.empty(3, np.float32)
+        lib().oracle_env_pdf(*self._args(), _p(d), _p(out))
+        return out
+
+    def sample(self, u: float, v: float):
+        """EnvMap::sample (envmap.rs:235): (pdf, direction, radiance)."""
+        out = np.empty(6, np.float32)
+        pdf = lib().oracle_env_sample(*self._args(), u, v, _p(out))
+        return pdf, out[:3], out[3:]
+
+    @staticmethod
+    def direction_from_uv(u: float, v: float) -> np.ndarray:
+        out = np.empty(3, np.float32)
+        lib().oracle_env_direction_from_uv(u, v, _p(out))
+        return out
+
+    @staticmethod
+    def uv_from_direction(d) -> np.ndarray:
+        d = np.asarray(d, np.float32)
+        out = np.empty(2, np.float32)
+        lib().oracle_env_uv_from_direction(_p(d), _p(out))
+        return out

@@ -1366,6 +1366,120 @@ float oracle_bsdf_eval_pdf(const GpuMaterial* m, uint32_t front, const float wo[
 
 extern "C" {
 
+// --- closed-form checks and the environment sampler (bsdf.rs, envmap.rs) ---
+
+float oracle_fresnel_dielectric(float cos_i, float eta) { return fresnel_dielectric(cos_i, eta); }
+
+void oracle_fresnel_conductor(float cos_i, const float eta[3], const float k[3], float out[3]) {
+    fresnel_conductor(cos_i, Vec3::from(eta), Vec3::from(k)).store(out);
+}
+
+// bsdf.rs:555 refract: 1 and the transmitted direction, or 0 under total
+// internal reflection.
+int oracle_refract(const float wo[3], const float n[3], float eta, float out[3]) {
+    Vec3 wt;
+    if (!refract(Vec3::from(wo), Vec3::from(n), eta, &wt)) return 0;
+    wt.store(out);
+    return 1;
+}
+
+// The reflected and transmitted halves of a surface's directional albedo:
+// the mean over n samples of surface::sample's luminance weight, split by the
+// side of the surface wi lands on (crates/core/tests/dielectric.rs
+// total_albedo), with a PCG stream Rng::new(seed, 0, 0) drawing u_lobe then
+// the 2D sample.
+void oracle_bsdf_split_albedo(const GpuMaterial* m, uint32_t front, const float wo[3], uint32_t n, uint32_t seed,
+                              double out[2]) {
+    Surface s = Surface::from_material(*m, front != 0);
+    Rng rng(seed, 0, 0);
+    double up = 0.0, down = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        float ul = rng.next_f32();
+        float ux = rng.next_f32();
+        float uy = rng.next_f32();
+        BsdfSample b = surface_sample(s, Vec3::from(wo), ul, Vec2{ux, uy});
+        if (!(b.pdf > 0.0f)) continue;
+        double w = (double)luminance(b.weight);
+        if (b.wi.z > 0.0f) up += w; else down += w;
+    }
+    out[0] = up / (double)n;
+    out[1] = down / (double)n;
+}
+
+// EnvMap::new (envmap.rs:112): the oracle's own build of the conditional
+// (w + 1 per row) and marginal (h + 1) CDFs over luminance * sin(theta at the
+// row centre); returns total_weight.
+float oracle_env_build(uint32_t width, uint32_t height, const float* pixels, float* conditional, float* marginal) {
+    const size_t w = width, h = height;
+    std::fill(conditional, conditional + (w + 1) * h, 0.0f);
+    std::fill(marginal, marginal + h + 1, 0.0f);
+    std::vector<float> row_weight(h, 0.0f);
+    for (size_t j = 0; j < h; ++j) {
+        const float sin_theta = std::sin(((float)j + 0.5f) / (float)h * PI);
+        const size_t base = j * (w + 1);
+        float running = 0.0f;
+        for (size_t i = 0; i < w; ++i) {
+            const float* p = &pixels[4 * (j * w + i)];
+            running += std::fmax(luminance(Vec3(p[0], p[1], p[2])), 0.0f) * sin_theta;
+            conditional[base + i + 1] = running;
+        }
+        row_weight[j] = running;
+        if (running > 0.0f) {
+            const float inv = 1.0f / running;
+            for (size_t k = 1; k <= w; ++k) conditional[base + k] *= inv;
+            conditional[base + w] = 1.0f;
+        }
+    }
+    float total = 0.0f;
+    for (size_t j = 0; j < h; ++j) {
+        total += row_weight[j];
+        marginal[j + 1] = total;
+    }
+    if (total > 0.0f) {
+        const float inv = 1.0f / total;
+        for (size_t k = 1; k <= h; ++k) marginal[k] *= inv;
+        marginal[h] = 1.0f;
+    }
+    return total;
+}
+
+static Env env_view(uint32_t w, uint32_t h, const float* pixels, const float* cond, const float* marg, float total) {
+    Env e;
+    e.w = w;
+    e.h = h;
+    e.pixels = pixels;
+    e.conditional = cond;
+    e.marginal = marg;
+    e.total_weight = total;
+    return e;
+}
+
+// EnvMap::pdf / radiance (envmap.rs:211 / :188) at a direction.
+float oracle_env_pdf(uint32_t w, uint32_t h, const float* pixels, const float* cond, const float* marg, float total,
+                     const float dir[3], float radiance_out[3]) {
+    Env e = env_view(w, h, pixels, cond, marg, total);
+    if (radiance_out) e.radiance(Vec3::from(dir)).store(radiance_out);
+    return e.pdf(Vec3::from(dir));
+}
+
+// EnvMap::sample (envmap.rs:235): out = direction[3], radiance[3]; returns pdf.
+float oracle_env_sample(uint32_t w, uint32_t h, const float* pixels, const float* cond, const float* marg, float total,
+                        float u, float v, float out[6]) {
+    Env e = env_view(w, h, pixels, cond, marg, total);
+    EnvSampleR r = e.sample(Vec2{u, v});
+    r.direction.store(out);
+    r.radiance.store(out + 3);
+    return r.pdf;
+}
+
+// EnvMap::direction_from_uv / uv_from_direction (envmap.rs:89 / :98).
+void oracle_env_direction_from_uv(float u, float v, float out[3]) { Env::direction_from_uv(u, v).store(out); }
+void oracle_env_uv_from_direction(const float d[3], float out[2]) {
+    Vec2 uv = Env::uv_from_direction(Vec3::from(d));
+    out[0] = uv.x;
+    out[1] = uv.y;
+}
+
 // Directional albedo of a material's full surface BSDF at cos_o, estimated
 // from n one-sample draws of surface::sample (bsdf.rs:1571) with a PCG stream.
 // A white, energy-compensated lobe integrates to 1 (the white furnace).

@@ -48,6 +48,27 @@ void oracle_scene_cast(const lucida_scene_desc* d, const float* rays, size_t n, 
 void oracle_denoise(const float* film, const float* albedo, const float* normal, const float* depth, uint32_t width,
                     uint32_t height, const lucida_denoise_params* p, float* out);
 
+/* bsdf.rs:314 / :358 / :555 closed forms, and the reflected / transmitted
+ * halves of a surface's directional albedo (mean luminance weight of
+ * surface::sample split by the side wi lands on). */
+float oracle_fresnel_dielectric(float cos_i, float eta);
+void oracle_fresnel_conductor(float cos_i, const float eta[3], const float k[3], float out[3]);
+int oracle_refract(const float wo[3], const float n[3], float eta, float out[3]);
+void oracle_bsdf_split_albedo(const GpuMaterial* m, uint32_t front, const float wo[3], uint32_t n, uint32_t seed,
+                              double out[2]);
+
+/* envmap.rs: the oracle's own CDF build (EnvMap::new, returns total_weight;
+ * conditional holds (w+1)*h floats, marginal h+1), pdf/radiance at a
+ * direction, one sample (out = direction[3], radiance[3], returns pdf), and
+ * the uv <-> direction maps. */
+float oracle_env_build(uint32_t width, uint32_t height, const float* pixels, float* conditional, float* marginal);
+float oracle_env_pdf(uint32_t w, uint32_t h, const float* pixels, const float* cond, const float* marg, float total,
+                     const float dir[3], float radiance_out[3]);
+float oracle_env_sample(uint32_t w, uint32_t h, const float* pixels, const float* cond, const float* marg, float total,
+                        float u, float v, float out[6]);
+void oracle_env_direction_from_uv(float u, float v, float out[3]);
+void oracle_env_uv_from_direction(const float d[3], float out[2]);
+
 /* image::compare (image.rs:178): out = {rmse, max_abs, mean_rel, mean_a,
  * mean_b, mean_abs}. */
 void oracle_compare(const float* a, const float* b, size_t pixels, double* out6);

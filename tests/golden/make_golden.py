@@ -6,6 +6,8 @@ test suite never needs the reference itself.
 Sources (paths relative to the reference root):
   web/public/scenes/cornell-box.bin   packed SceneBlob of cornell_box()
   web/public/scenes/cornell-mesh.bin  packed SceneBlob of cornell_mesh()
+  web/public/scenes/{furnace-test,metal-sweep,mis-scene,glass-box,sunset,
+      instance-forest}.bin            the other packed built-in scenes
   web/src/generated/scenes.ts         section offsets and CAMERA_FIXTURES
   shaders/common/ggx_energy.wgsl      the 32x32 GGX directional-albedo table
   shaders/common/generated.wgsl       the Sobol direction numbers (SOBOL_V)
@@ -28,7 +30,8 @@ import numpy as np
 
 OUT = Path(__file__).resolve().parent
 SIZES = {"materials": 80, "primitives": 64, "lights": 64, "positions": 16, "vertexAttrs": 32, "triangles": 16,
-         "bvhNodes": 32}
+         "bvhNodes": 32, "envRadiance": 16, "envCdf": 4}
+LUM = np.array([0.2126, 0.7152, 0.0722])
 
 
 def sections(ts: str, name: str) -> dict:
@@ -43,6 +46,48 @@ def sections(ts: str, name: str) -> dict:
 def section_bytes(data: bytes, secs: dict, key: str) -> bytes:
     off, cnt = secs[key]
     return data[off:off + cnt * SIZES[key]]
+
+
+def scene_meta(ts: str, name: str) -> dict:
+    """camera, env, background and instancing records of one scene in scenes.ts."""
+    block = ts[ts.index(f'name: "{name}"'):]
+    block = block[:block.index("\n  },")]
+    f = lambda s: [float(x) for x in s.split(",")]
+    cam = re.search(r"camera: \{ eye: \[([^\]]+)\], lookAt: \[([^\]]+)\], up: \[([^\]]+)\], vfovDeg: ([\d.]+), "
+                    r"aperture: ([\d.]+), focusDistance: ([\d.]+) \}", block)
+    env = re.search(r"env: \{ width: (\d+), height: (\d+), totalWeight: ([\d.]+) \}", block)
+    inst = re.search(r"instancing: \{ count: (\d+), analyticPrimitives: (\d+), tlasRoot: (\d+) \}", block)
+    bg = re.search(r"background: \[([^\]]+)\]", block)
+    return {"camera": {"eye": f(cam.group(1)), "look_at": f(cam.group(2)), "up": f(cam.group(3)),
+                       "vfov_deg": float(cam.group(4)), "aperture": float(cam.group(5)),
+                       "focus_distance": float(cam.group(6))},
+            "env": {"width": int(env.group(1)), "height": int(env.group(2)), "total_weight": float(env.group(3))},
+            "instancing": {"count": int(inst.group(1)), "analytic_primitives": int(inst.group(2)),
+                           "tlas_root": int(inst.group(3))},
+            "background": f(bg.group(1))}
+
+
+def env_record(data: bytes, secs: dict, width: int, height: int) -> dict:
+    """The environment sections of a packed scene, reduced to small checks:
+    exact hashes, every row's luminance sum (float64), the texels brighter
+    than 50 (the sun disc), sampled texels, the whole marginal CDF and a
+    sample of the conditional CDFs (codegen.rs packs rows 0..h-1 of w+1
+    conditional entries, then the h+1 marginal entries, in a (w+1) x (h+1)
+    rectangle)."""
+    rad = np.frombuffer(section_bytes(data, secs, "envRadiance"), "<f4").reshape(height, width, 4)
+    cdf = np.frombuffer(section_bytes(data, secs, "envCdf"), "<f4").reshape(height + 1, width + 1)
+    lum = rad[..., :3].astype(np.float64) @ LUM
+    sun = np.flatnonzero(lum.ravel() > 50.0)
+    idx = np.unique(np.concatenate([np.arange(0, width * height, 61), sun]))
+    cidx = np.arange(0, height * (width + 1), 37)
+    return {"sha256_envRadiance": hashlib.sha256(rad.tobytes()).hexdigest(),
+            "sha256_envCdf": hashlib.sha256(cdf.tobytes()).hexdigest(),
+            "row_luminance_sums": lum.sum(axis=1).tolist(),
+            "sun_texels": sun.tolist(),
+            "texel_samples": {"index": idx.tolist(), "rgba_bits": rad.reshape(-1, 4)[idx].view("<u4").tolist()},
+            "marginal": cdf[height, :height + 1].tolist(),
+            "conditional_samples": {"index": cidx.tolist(),
+                                    "values": cdf[:height].reshape(-1)[cidx].tolist()}}
 
 
 def main(ref: Path) -> None:
@@ -73,6 +118,26 @@ def main(ref: Path) -> None:
     idx = np.linspace(0, len(pos) - 1, 200).astype(int)
     rec["position_samples"] = {"index": idx.tolist(), "bits": pos[idx].tolist()}
     (OUT / "cornell_mesh_sections.json").write_text(json.dumps(rec))
+
+    # --- the other packed scenes ------------------------------------------------
+    for name in ("furnace-test", "metal-sweep", "mis-scene", "glass-box", "sunset", "instance-forest"):
+        data = (ref / f"web/public/scenes/{name}.bin").read_bytes()
+        secs = sections(ts, name)
+        meta = scene_meta(ts, name)
+        rec = {"counts": {k: secs[k][1] for k in SIZES}, **meta}
+        for k in ("materials", "primitives", "lights"):
+            rec[k] = np.frombuffer(section_bytes(data, secs, k), "<u4").tolist()
+        if secs["triangles"][1]:
+            rec["sha256_positions"] = hashlib.sha256(section_bytes(data, secs, "positions")).hexdigest()
+            va = np.frombuffer(section_bytes(data, secs, "vertexAttrs"), "<u4").reshape(-1, 8).copy()
+            va[:, 4:6] = 0  # uv: libm-dependent last bits
+            rec["sha256_vertexAttrs_no_uv"] = hashlib.sha256(va.tobytes()).hexdigest()
+            tris = np.frombuffer(section_bytes(data, secs, "triangles"), "<u4").reshape(-1, 4)
+            order = np.lexsort(tris.T[::-1])
+            rec["sha256_triangles_sorted"] = hashlib.sha256(np.ascontiguousarray(tris[order]).tobytes()).hexdigest()
+        if meta["env"]["width"]:
+            rec["env_sections"] = env_record(data, secs, meta["env"]["width"], meta["env"]["height"])
+        (OUT / f"{name.replace('-', '_')}_sections.json").write_text(json.dumps(rec))
 
     # --- camera fixtures -------------------------------------------------------
     cams = []

@@ -77,25 +77,29 @@ def test_cornell_mesh_matches_reference_asset():
 
 @pytest.mark.parametrize("idx", range(7))
 def test_camera_matches_reference_fixtures(idx):
-    cams = json.loads((GOLDEN / "camera_fixtures.json").read_text())
-    c = cams[idx]
-    if c["aperture"] != 0 or c["focus_distance"] != pytest.approx(
-            float(np.linalg.norm(np.subtract(c["look_at"], c["eye"]))), rel=1e-6):
-        pytest.skip("fixture uses a focus distance the look_at constructor does not produce")
-    # Build a one-off blob with this camera through the generic icosphere scene
-    # and overwrite its camera via the uniforms path is not exposed; instead
-    # compare with a direct restatement through the library's cornell camera
-    # when the fixture is the Cornell one.
-    if c["label"] not in ("cornell", "cornell-wide", "cornell-tall"):
-        pytest.skip("the library exposes the Cornell camera only through cornell-box")
-    u = L.Blob("cornell-box").uniforms(c["width"], c["height"])
-    for key, field in (("cam_origin", "cam_origin"), ("cam_upper_left", "cam_upper_left"),
-                       ("cam_horizontal", "cam_horizontal"), ("cam_vertical", "cam_vertical")):
-        got = np.array(getattr(u, field)[:], np.float32)
+    """Camera::write_uniforms (camera.rs:55) against the reference's seven
+    CAMERA_FIXTURES, including the off-axis, near-vertical, narrow-fov and
+    thin-lens cameras with their explicit focus distances."""
+    c = json.loads((GOLDEN / "camera_fixtures.json").read_text())[idx]
+    cam = L.Camera(tuple(c["eye"]), tuple(c["look_at"]), tuple(c["up"]), c["vfov_deg"], c["aperture"],
+                   c["focus_distance"])
+    u = cam.write_uniforms(L.Uniforms(), np.float32(c["width"]) / np.float32(c["height"]))
+    for key in ("cam_origin", "cam_upper_left", "cam_horizontal", "cam_vertical"):
+        got = np.array(getattr(u, key)[:], np.float32)
         exp = np.array(c[key], np.float32)
         # tan() goes through the platform libm (the fixtures were made with
-        # Apple's): agreement to 2 ulp, not bits.
-        np.testing.assert_allclose(got, exp, rtol=2.5e-7, atol=0)
+        # Apple's): agreement to 2 ulp of each component's scale, not bits.
+        scale = np.abs(exp).max()
+        np.testing.assert_allclose(got, exp, rtol=0, atol=2.5e-7 * scale, err_msg=f"{c['label']} {key}")
+    assert u.lens_radius == np.float32(c["aperture"]) * np.float32(0.5)
+    assert u.focus_distance == np.float32(c["focus_distance"])
+
+
+def test_blob_camera_is_the_cornell_fixture_camera():
+    c = json.loads((GOLDEN / "camera_fixtures.json").read_text())[0]
+    cam = L.Blob("cornell-box").camera()
+    assert list(cam.eye) == c["eye"] and list(cam.look_at) == c["look_at"] and cam.vfov_deg == c["vfov_deg"]
+    assert cam.focus_distance == np.float32(c["focus_distance"])
 
 
 def test_uniform_counts_follow_the_blob():
