@@ -237,23 +237,14 @@ def batch_c4(L, gpu, jobs_n: int = 64, seed: int = 6) -> dict:
     lucida_render_batch with the autotuned launch shape; latency = time from
     the start until a job's film is complete, all jobs arriving at once."""
     from lucida_amd import autotune
-    rng = np.random.default_rng(seed)
-    blobs = [L.Blob("box-c1")] + [L.Blob("sphere-c2", seed=k, frequency=64) for k in range(1, 5)]
+    from lucida_amd import request_stream as RS
+    blobs = RS.stream_blobs()
     scenes = [L.Scene(gpu, b) for b in blobs]
-    reqs = []
-    for k in range(jobs_n):
-        res = int(rng.choice([128, 256, 512, 1024]))
-        spp = int(rng.choice([1, 4, 16, 64]))
-        si = int(rng.integers(0, len(blobs)))
-        reqs.append((si, res, spp, 0x1000 + k))
+    reqs = RS.stream_requests(jobs_n, seed, len(blobs))
     paths = sum(r * r * spp for _, r, spp, _ in reqs)
 
     def jobs():
-        out = []
-        for si, res, spp, fs in reqs:
-            u = blobs[si].uniforms(res, res, samples=spp, max_depth=8, frame_seed=fs)
-            out.append((scenes[si], u, spp, L.Film(gpu, res, res)))
-        return out
+        return RS.make_jobs(gpu, blobs, scenes, reqs)
 
     seq_jobs = jobs()
     L.render_wavefront(gpu, seq_jobs[0][0], seq_jobs[0][1], 1, L.Film(gpu, seq_jobs[0][1].width,

@@ -41,7 +41,7 @@ JOBS = [
     ("sphere-c2-f24", 48, 40, 2, 8, 0, 0x5EED1234),
     # chunks of 2^21 paths or more: rounds of their own, single-render kernels
     ("box-c1", 1024, 1024, 3, 4, 0, 7),
-    ("sphere-c2-f24", 1024, 1024, 6, 8, 1, 3),  # two chunks: 4 + 2 spp
+    ("sphere-c2-f24", 1024, 1024, 6, 8, 1, 3),  # one chunk (8 spp per 2^23-path chunk at 1024^2)
     ("instances-small", 1024, 768, 3, 6, 0, 21),
 ]
 
