@@ -11,7 +11,7 @@ per-kernel times) ride in "extra". "time_to_relmse" holds the metric's second
 half: device time until the plain and the denoised images of the same scene
 reach relMSE 0.01 against an independent 4096-spp render.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--cpu-seconds 15] [--no-cpu] [--no-relmse]
+    python bench.py [--gpus 1] [--steps 7] [--warmup 2] [--cpu-seconds 15] [--no-cpu] [--no-relmse]
 """
 from __future__ import annotations
 
@@ -31,21 +31,32 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Mpaths/s @1M tris 8 bounces; time-to-relMSE 0.01: denoised vs plain"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_SUMMARY = ROOT / "profiles" / "wavefront_c2_1024x16spp_hbm_bytes.md"
+SQ_SUMMARY = ROOT / "profiles" / "wavefront_c2_1024x16spp_sq_counters.md"
+# MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock; a SIMD issues one
+# wave64 VALU instruction per 4 clocks.
+SIMDS, CLOCK_HZ, CLOCKS_PER_VALU = 1024, 2.4e9, 4
+# Algorithmic bytes (the reference's struct sizes): a closest-hit ray reads
+# its queue index (4) and origin + direction (32) and writes a hit (16); a
+# node visit reads the two 32-byte children of a binary node; a triangle
+# test reads its three 16-byte edge vectors. A shadow ray reads its 48-byte
+# record and adds 24 bytes of radiance. The LBVH costs 436 bytes per
+# triangle, the a-trous filter 644 per pixel.
+RAY_BYTES, NODE_BYTES, TRI_BYTES, SHADOW_BYTES = 52, 64, 48, 72
+LBVH_BYTES_PER_TRI, ATROUS_BYTES_PER_PX = 436, 644
 
 
-def measured_extend_traffic():
-    """HBM bytes per EXTEND launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (counters scaled by the copy-kernel calibration), or None."""
+def profile_figure(path: Path, key: str):
+    """A `key: value` line of a committed rocprofv3 summary, or None."""
     try:
-        for line in TRAFFIC_SUMMARY.read_text().splitlines():
-            if line.startswith("extend_hbm_bytes_per_launch:"):
-                return int(line.split(":")[1])
+        for line in path.read_text().splitlines():
+            if line.startswith(key + ":"):
+                return float(line.split(":")[1])
     except OSError:
         pass
     return None
 
 
-def lbvh_build_rate(L, gpu, blob, runs: int = 5) -> dict:
+def lbvh_build_rate(L, gpu, blob, runs: int = 7) -> dict:
     """Device LBVH build of the scene's triangles (inputs already on the GPU)."""
     a = blob.arrays()
     tris = np.ascontiguousarray(a["triangles"])
@@ -79,34 +90,43 @@ def lbvh_build_rate(L, gpu, blob, runs: int = 5) -> dict:
     med = float(np.median(times))
     st = np.median(np.stack(stages), axis=0)
     device_us = float(st.sum())
-    return {"triangles": n, "nodes": int(count.value), "wall_ms": med * 1e3, "device_us": device_us,
+    gbs = LBVH_BYTES_PER_TRI * n / (device_us * 1e-6) / 1e9
+    return {"triangles": n, "nodes": int(count.value), "wall_ms": round(med * 1e3, 4),
+            "device_us": round(device_us, 1), "runs": runs, "statistic": "median",
             "stage_us": dict(zip(("prepare", "sort", "karras", "fit", "relayout"), [round(float(x), 1) for x in st])),
-            "mtris_per_s": n / (device_us * 1e-6) / 1e6, "mtris_per_s_wall": n / med / 1e6,
-            "bytes_per_tri": 436}
+            "mtris_per_s": round(n / (device_us * 1e-6) / 1e6, 1), "mtris_per_s_wall": round(n / med / 1e6, 1),
+            "roofline": {"bytes_per_tri": LBVH_BYTES_PER_TRI, "achieved_gbs": round(gbs, 1),
+                         "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 4)}}
 
 
-def cpu_baseline(blob, u, target_s: float) -> dict:
+def cpu_baseline(blob, u, target_s: float, runs: int = 7) -> dict:
     """The CPU oracle (a C++ restatement of the reference tracer) on a band of
-    rows of the same workload, threads = the cores this process may use."""
+    rows of the same workload, threads = the cores this process may use: one
+    warm-up, then the median of `runs` timed renders of the band
+    (crates/cli/src/bin/bench.rs:192-212), the band sized so all runs take
+    about target_s."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
     threads = len(os.sched_getaffinity(0))
     threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
     desc = blob.desc()
-    rows = 4
+    rows = 2
     t0 = time.perf_counter()
-    pyoracle.render(desc, u, threads=threads, rows=(0, rows), guides=False)
+    pyoracle.render(desc, u, threads=threads, rows=(0, rows), guides=False)  # warm-up and sizing
     dt = time.perf_counter() - t0
-    # scale the band so the timed sample takes about target_s
-    rows = int(max(4, min(u.height, rows * target_s / max(dt, 1e-3))))
+    rows = int(max(2, min(u.height, rows * target_s / runs / max(dt, 1e-3))))
     r0 = (u.height - rows) // 2
-    t0 = time.perf_counter()
-    pyoracle.render(desc, u, threads=threads, rows=(r0, r0 + rows), guides=False)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        pyoracle.render(desc, u, threads=threads, rows=(r0, r0 + rows), guides=False)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     paths = rows * u.width * u.samples_per_launch
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
             "sample": f"rows {r0}..{r0 + rows} of the {u.width}x{u.height} image at {u.samples_per_launch} spp, "
-                      f"{u.max_depth} bounces ({paths} paths, {dt:.1f} s)"}
+                      f"{u.max_depth} bounces ({paths} paths): median of {runs} runs after a warm-up, "
+                      f"{dt:.2f} s each"}
 
 
 def other_configs(L, gpu) -> dict:
@@ -151,14 +171,17 @@ def relmse(x: np.ndarray, ref: np.ndarray) -> float:
     return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
 
 
-def time_to_relmse(L, gpu, scene, blob, width: int, height: int, depth: int, threshold: float, ref_spp: int,
+def time_to_relmse(L, gpu, scene, blob, width: int, height: int, depth: int, thresholds, ref_spp: int,
                    max_spp: int, net=None) -> dict:
     """Progressive render of the same scene, sample counts 1, 2, 4, ..., max_spp,
     scored after each doubling against an independent ref_spp render (another
     frame seed, so no sample is shared). Time = device render time of all
     samples so far (+ the denoiser's time for the denoised variants); the
-    crossing of `threshold` is interpolated log-log between the bracketing
-    sample counts, with time linear in samples."""
+    crossing of each threshold is interpolated log-log between the
+    bracketing sample counts, with time linear in samples. The first
+    threshold is the metric's (0.01); a tighter one (1e-3) separates the
+    denoisers, which both reach 0.01 at one sample."""
+    threshold = min(thresholds)
     mk = lambda spp, off, seed: blob.uniforms(width, height, samples=spp, max_depth=depth, sampling=L.SAMPLING_MIS,
                                               sampler=L.SAMPLER_SOBOL, sample_offset=off, frame_seed=seed)
     ref_film = L.Film(gpu, width, height)
@@ -204,7 +227,7 @@ def time_to_relmse(L, gpu, scene, blob, width: int, height: int, depth: int, thr
             break
         spp *= 2
 
-    def crossing(pts):
+    def crossing(pts, threshold):
         for i, (s, t, e) in enumerate(pts):
             if e <= threshold:
                 if i == 0:
@@ -216,15 +239,24 @@ def time_to_relmse(L, gpu, scene, blob, width: int, height: int, depth: int, thr
                         "relmse_at_bracket": [e0, e]}
         return None
 
-    out = {"threshold": threshold, "definition": "mean (x-r)^2/(r^2+1e-2) over pixels and channels",
+    out = {"threshold": thresholds[0], "thresholds": list(thresholds),
+           "definition": "mean (x-r)^2/(r^2+1e-2) over pixels and channels",
            "resolution": [width, height], "reference_spp": ref_spp, "reference_render_s": round(ref_s, 2),
            "max_spp": max_spp}
     for name, pts in variants.items():
-        out[name] = {"reached": crossing(pts), "curve": [[s, round(t, 3), round(e, 6)] for s, t, e in pts]}
-    p = out["plain"]["reached"]
-    for name in [k for k in variants if k != "plain"]:
-        r = out[name]["reached"]
-        out[name]["speedup_vs_plain"] = round(p["ms"] / r["ms"], 2) if p and r else None
+        out[name] = {"curve": [[s, round(t, 3), round(e, 7)] for s, t, e in pts]}
+        for k, th in enumerate(thresholds):
+            out[name]["reached" if k == 0 else f"reached_{th:g}"] = crossing(pts, th)
+    for k, th in enumerate(thresholds):
+        key = "reached" if k == 0 else f"reached_{th:g}"
+        p = out["plain"][key]
+        for name in [v for v in variants if v != "plain"]:
+            r = out[name][key]
+            out[name]["speedup_vs_plain" if k == 0 else f"speedup_vs_plain_{th:g}"] = \
+                round(p["ms"] / r["ms"], 2) if p and r else None
+    if "learned" in variants:
+        out["learned_vs_atrous_relmse"] = {str(s): round(e / a, 4) for (s, _, e), (_, _, a)
+                                           in zip(variants["learned"], variants["atrous"])}
     return out
 
 
@@ -262,6 +294,7 @@ def batch_c4(L, gpu, jobs_n: int = 64, seed: int = 6) -> dict:
     t_tune = time.perf_counter()
     cfg = autotune.tune(gpu, bjobs)
     tune_s = time.perf_counter() - t_tune
+    trials = autotune.last_trials(gpu, bjobs)
     for _, _, _, f in bjobs:
         f.clear()
     gpu.check_errors()
@@ -277,14 +310,16 @@ def batch_c4(L, gpu, jobs_n: int = 64, seed: int = 6) -> dict:
                         "mpaths_per_s": round(paths / batch_ms / 1e3, 2), "rounds": int(st.kernel_launches[0]),
                         "latency_p50_ms": q(done, 50), "latency_p99_ms": q(done, 99),
                         "config": {"max_paths": cfg.max_paths, "use_graph": cfg.use_graph},
-                        "autotune_s": round(tune_s, 2)}}
+                        "autotune_s": round(tune_s, 2),
+                        "autotune_trials_ms": {f"max_paths {t['max_paths']} graph {t['use_graph']}": t["ms"]
+                                               for t in trials}}}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=7)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--spp", type=int, default=16)
@@ -300,10 +335,13 @@ def main() -> int:
     ap.add_argument("--relmse-ref-spp", type=int, default=4096)
     ap.add_argument("--relmse-max-spp", type=int, default=1024)
     ap.add_argument("--no-relmse", action="store_true")
-    ap.add_argument("--net-scenes", type=int, default=32, help="training scenes for the learned denoiser")
-    ap.add_argument("--net-ref-spp", type=int, default=1024, help="target samples per pixel of the training pairs")
-    ap.add_argument("--net-steps", type=int, default=3000, help="Adam steps (0: no learned denoiser)")
-    ap.add_argument("--net-crop", type=int, default=128, help="training crop side (8 crops per step)")
+    ap.add_argument("--net-scenes", type=int, default=96, help="training scenes for the learned denoiser")
+    ap.add_argument("--net-ref-spp", type=int, default=4096, help="target samples per pixel of the training pairs")
+    ap.add_argument("--net-steps", type=int, default=4000, help="Adam steps (0: no learned denoiser)")
+    ap.add_argument("--net-crop", type=int, default=256, help="training crop side (8 crops per step)")
+    ap.add_argument("--serialize", action="store_true",
+                    help="time the steps serialized (one pipeline, every kernel alone on one stream), as the "
+                         "rocprofv3 counter runs see them")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched request stream")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the Cornell-box and instanced-scene throughput figures")
@@ -366,78 +404,94 @@ def main() -> int:
     film = L.Film(gpu, a.width, a.height)
     u = blob.uniforms(a.width, a.height, samples=a.spp, max_depth=a.depth, sampling=L.SAMPLING_MIS,
                       sampler=L.SAMPLER_SOBOL)
-    # Timed renders run as a caller would (no traversal counters; per-kernel
-    # HIP events only); node visits and triangle tests come from one more,
-    # untimed, counted render of the same (deterministic) step.
-    cfg = L.WavefrontConfig(a.samples_per_batch, 0, a.bvh_width, 0, 0, 1, a.ray_order)
+    # Timed renders run as a caller runs them: no traversal counters, no
+    # per-kernel events. Kernel times come from untimed renders of the same
+    # (deterministic) step: one as the bench runs it (kernels overlapping
+    # across streams and pipelines) and one serialized (every kernel alone on
+    # one stream), whose times are solo times; node visits and triangle tests
+    # from one more, counted, render.
+    mk_cfg = lambda count=0, profile=0, serialize=0: L.WavefrontConfig(
+        a.samples_per_batch, 0, a.bvh_width, 0, count, profile, a.ray_order, serialize)
+    cfg = mk_cfg(serialize=1 if a.serialize else 0)
 
     for _ in range(a.warmup):
         film.clear()
         L.render_wavefront(gpu, scene, u, a.spp, film, cfg)
     gpu.check_errors()
 
-    agg = {k: 0.0 for k in L.RenderStats.KERNELS}
-    launches = {k: 0 for k in L.RenderStats.KERNELS}
-    device_ms = 0.0
-    sort_ms, sorts = 0.0, 0
+    step_s = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
         film.clear()
-        st = L.render_wavefront(gpu, scene, u, a.spp, film, cfg)
-        for i, k in enumerate(L.RenderStats.KERNELS):
-            agg[k] += st.kernel_ms[i]
-            launches[k] += st.kernel_launches[i]
-        device_ms += st.elapsed_ms
-        sort_ms += st.queue_sort_ms
-        sorts += st.queue_sorts
-        if st.stack_overflows:
-            print(f"warning: {st.stack_overflows} traversal stack overflows", file=sys.stderr)
+        ts = time.perf_counter()
+        L.render_wavefront(gpu, scene, u, a.spp, film, cfg)  # returns when the render is complete
+        step_s.append(time.perf_counter() - ts)
     gpu.check_errors()
-    t1 = time.perf_counter()
-    sec = (t1 - t0) / a.steps
+    total_s = time.perf_counter() - t0
+    sec = float(np.median(step_s))
+
+    def kernel_times(serialize: int) -> dict:
+        film.clear()
+        st = L.render_wavefront(gpu, scene, u, a.spp, film, mk_cfg(profile=1, serialize=serialize))
+        gpu.check_errors()
+        ms = {k: float(st.kernel_ms[i]) for i, k in enumerate(L.RenderStats.KERNELS)}
+        ms["queue_sort"] = float(st.queue_sort_ms)
+        n = {k: int(st.kernel_launches[i]) for i, k in enumerate(L.RenderStats.KERNELS)}
+        n["queue_sort"] = int(st.queue_sorts)
+        return {"ms": ms, "launches": n, "device_ms": float(st.elapsed_ms)}
+
+    overlapped = kernel_times(0)
+    solo = kernel_times(1)
     film.clear()
-    cst = L.render_wavefront(gpu, scene, u, a.spp, film,
-                             L.WavefrontConfig(a.samples_per_batch, 0, a.bvh_width, 0, 1, 0, a.ray_order))
+    cst = L.render_wavefront(gpu, scene, u, a.spp, film, mk_cfg(count=1))
     gpu.check_errors()
-    n = a.steps  # every step traces the same rays
-    ext_rays, shadow_rays = cst.extension_rays * n, cst.shadow_rays * n
-    ext_visits, ext_tests = cst.extend_node_visits * n, cst.extend_triangle_tests * n
-    visits, tests = cst.node_visits * n, cst.triangle_tests * n
-    ext_slots, con_slots = cst.extend_lane_slots * n, cst.connect_lane_slots * n
+    ext_rays, shadow_rays = cst.extension_rays, cst.shadow_rays
+    ext_visits, ext_tests = cst.extend_node_visits, cst.extend_triangle_tests
+    con_visits, con_tests = cst.node_visits - ext_visits, cst.triangle_tests - ext_tests
+    ext_slots, con_slots = cst.extend_lane_slots, cst.connect_lane_slots
     paths = a.width * a.height * a.spp
     value = paths / sec / 1e6
 
-    # Roofline of the dominant kernel. EXTEND's algorithmic bytes per ray:
-    # queue index 4 + origin/direction planes 32 + hit write 16, plus 64 per
-    # node visit (the two 32-byte children of a binary node, or one 64-byte
-    # compressed 4-wide node) and 64 per triangle test (16-byte triangle +
-    # three 16-byte positions).
-    node_bytes = 64
-    ext_bytes = ext_rays * (4 + 32 + 16) + node_bytes * ext_visits + 64 * ext_tests
-    ext_ms = agg["extend"]
-    dominant = max(agg, key=agg.get)
-    achieved = ext_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
-    # CONNECT runs beside the next EXTEND on a second stream, and batches
-    # alternate between two such pipelines, so each launch's span includes
-    # time shared with other kernels. The two walks together: EXTEND's bytes
-    # plus CONNECT's (shadow record 48 + radiance 24 per ray, 64 per node
-    # visit and per triangle test) over the whole render's device time
-    # (GENERATE, SHADE, the sorts and RESOLVE included: a lower bound on the
-    # walks' own rate).
-    con_bytes = shadow_rays * (48 + 24) + node_bytes * (visits - ext_visits) + 64 * (tests - ext_tests)
-    walks_ms = device_ms
+    # Roofline of the dominant kernel, EXTEND (the closest-hit walk), from its
+    # solo launch time: algorithmic bytes per launch / average solo launch.
+    ext_launches = max(solo["launches"]["extend"], 1)
+    ext_solo_ms = solo["ms"]["extend"] / ext_launches
+    ext_bytes = (ext_rays * RAY_BYTES + NODE_BYTES * ext_visits + TRI_BYTES * ext_tests) / ext_launches
+    achieved = ext_bytes / (ext_solo_ms * 1e-3) / 1e9
+    con_launches = max(solo["launches"]["connect"], 1)
+    con_solo_ms = solo["ms"]["connect"] / con_launches
+    con_bytes = (shadow_rays * SHADOW_BYTES + NODE_BYTES * con_visits + TRI_BYTES * con_tests) / con_launches
+    dominant = max((k for k in solo["ms"] if k != "queue_sort"), key=lambda k: solo["ms"][k])
 
-    # a-trous on the final film (1024^2)
+    def measured(kernel: str, ms: float) -> dict:
+        """HBM bytes and VALU wave instructions per launch from the committed
+        rocprofv3 summaries (serialized launches of the same step), over this
+        run's solo launch time."""
+        traffic = profile_figure(TRAFFIC_SUMMARY, f"{kernel}_hbm_bytes_per_launch")
+        valu = profile_figure(SQ_SUMMARY, f"{kernel}_valu_insts_per_launch")
+        active = profile_figure(SQ_SUMMARY, f"{kernel}_active_lane_fraction")
+        out = {"hbm_bytes_per_launch": traffic, "valu_insts_per_launch": valu, "active_lane_fraction": active}
+        if traffic:
+            out["hbm_gbs"] = round(traffic / (ms * 1e-3) / 1e9, 1)
+            out["hbm_frac_of_8tbs"] = round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        if valu:
+            out["valu_issue_frac"] = round(valu / (SIMDS * CLOCK_HZ / CLOCKS_PER_VALU * ms * 1e-3), 4)
+        return out
+
+    # a-trous on the final film (1024^2): median of 7 after a warm-up
     den = L.Film(gpu, a.width, a.height)
     dn = L.Denoiser(gpu)
     dn.run(film, den)
     gpu.check_errors()
-    td = time.perf_counter()
-    for _ in range(5):
+    at = []
+    for _ in range(7):
+        td = time.perf_counter()
         dn.run(film, den)
-    gpu.check_errors()
-    atrous_ms = (time.perf_counter() - td) / 5 * 1e3
+        at.append(time.perf_counter() - td)
+    atrous_ms = float(np.median(at)) * 1e3
+    atrous_gbs = ATROUS_BYTES_PER_PX * a.width * a.height / (atrous_ms * 1e-3) / 1e9
 
+    r3 = lambda d: {k: round(v, 3) for k, v in d.items()}
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(sec * 1e3, 3), "higher_is_better": True, "scaling": "weak",
@@ -446,34 +500,43 @@ def main() -> int:
                                f"16 emissive quads, {a.width}x{a.height}, {a.spp} spp, {a.depth} bounces, "
                                f"NEE+MIS, Sobol", "triangles": ntris, "width": a.width, "height": a.height,
                    "spp": a.spp, "max_depth": a.depth, "parallelism": "single GPU"},
+        "timing": {"statistic": "median of the timed steps' wall times (each step a complete render)",
+                   "mean_ms_per_step": round(total_s / a.steps * 1e3, 3),
+                   "min_ms": round(min(step_s) * 1e3, 3), "max_ms": round(max(step_s) * 1e3, 3),
+                   "serialized": bool(a.serialize)},
         "roofline": {"bound": "hbm", "kernel": "extend (closest-hit traversal)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_extend_traffic(),
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the copy calibration "
-                                       "(profiles/wavefront_c2_1024x16spp_hbm_bytes.md)",
-                     "avg_launch_ms": round(ext_ms / max(launches["extend"], 1), 4),
-                     "bytes_per_launch": int(ext_bytes / max(launches["extend"], 1))},
+                     "traffic": profile_figure(TRAFFIC_SUMMARY, "extend_hbm_bytes_per_launch"),
+                     "achieved_is": "algorithmic bytes (queue index, ray, hit, 64 B per node visit, 48 B per "
+                                    "triangle test) per launch over the solo launch time; most node and triangle "
+                                    "reads are served by L2 and the MALL, so this is not HBM traffic",
+                     "limiter": "VALU issue over diverged lanes (see measured.valu_issue_frac and "
+                                "active_lane_fraction), not HBM bandwidth (measured.hbm_frac_of_8tbs)",
+                     "avg_launch_ms": round(ext_solo_ms, 4), "launch_time": "solo (serialized render)",
+                     "bytes_per_launch": int(ext_bytes),
+                     "measured": measured("extend", ext_solo_ms)},
         "extra": {
             "device": dev_name, "compute_units": cus,
-            "mrays_per_s": round((ext_rays + shadow_rays) / a.steps / sec / 1e6, 2),
-            "extension_rays_per_step": ext_rays // a.steps, "shadow_rays_per_step": shadow_rays // a.steps,
+            "mrays_per_s": round((ext_rays + shadow_rays) / sec / 1e6, 2),
+            "extension_rays_per_step": ext_rays, "shadow_rays_per_step": shadow_rays,
             "bvh_walk": "4-wide" if a.bvh_width == 4 else "binary",
             "walk_simd_efficiency": {"extend": round(ext_visits / max(ext_slots, 1), 3),
-                                     "connect": round((visits - ext_visits) / max(con_slots, 1), 3)},
-            "node_visits_per_ray": round(visits / max(ext_rays + shadow_rays, 1), 2),
-            "triangle_tests_per_ray": round(tests / max(ext_rays + shadow_rays, 1), 2),
-            "kernel_ms_per_step": {**{k: round(v / a.steps, 3) for k, v in agg.items()},
-                                   "queue_sort": round(sort_ms / a.steps, 3)},
-            "queue_sorts_per_step": sorts // a.steps,
-            "walks_combined": {"bytes_per_step": int((ext_bytes + con_bytes) / a.steps),
-                               "ms_per_step": round(walks_ms / a.steps, 3),
-                               "gbs": round((ext_bytes + con_bytes) / max(walks_ms * 1e-3, 1e-9) / 1e9, 1),
-                               "frac_of_8tbs": round((ext_bytes + con_bytes) / max(walks_ms * 1e-3, 1e-9) / 1e9
-                                                     / HBM_PEAK_GBS, 4)},
+                                     "connect": round(con_visits / max(con_slots, 1), 3)},
+            "node_visits_per_ray": round(cst.node_visits / max(ext_rays + shadow_rays, 1), 2),
+            "triangle_tests_per_ray": round(cst.triangle_tests / max(ext_rays + shadow_rays, 1), 2),
+            "kernel_ms_per_step_solo": r3(solo["ms"]),
+            "kernel_ms_per_step_overlapped": r3(overlapped["ms"]),
+            "kernel_launches_per_step": solo["launches"],
+            "device_ms_per_step": {"as_timed": round(overlapped["device_ms"], 3),
+                                   "serialized": round(solo["device_ms"], 3)},
             "dominant_kernel": dominant,
-            "device_ms_per_step": round(device_ms / a.steps, 3),
+            "connect_roofline": {"achieved": round(con_bytes / (con_solo_ms * 1e-3) / 1e9, 1),
+                                 "frac": round(con_bytes / (con_solo_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                 "avg_launch_ms": round(con_solo_ms, 4), "bytes_per_launch": int(con_bytes),
+                                 "measured": measured("connect", con_solo_ms)},
             "lbvh": lbvh,
-            "atrous_ms": round(atrous_ms, 3),
+            "atrous": {"ms": round(atrous_ms, 3), "bytes_per_px": ATROUS_BYTES_PER_PX,
+                       "achieved_gbs": round(atrous_gbs, 1), "frac_of_8tbs": round(atrous_gbs / HBM_PEAK_GBS, 4)},
         },
     }
     if not a.no_configs:
@@ -486,11 +549,11 @@ def main() -> int:
             progress = lambda msg: print(f"denoiser training: {msg}", file=sys.stderr, flush=True)
             pairs, data_paths = T.make_training_set(
                 gpu, a.net_scenes, ref_spp=a.net_ref_spp,
-                log=lambda msg: progress(msg) if msg.startswith("scene ") and int(msg.split()[1][:-1]) % 16 == 15
+                log=lambda msg: progress(msg) if msg.startswith("scene ") and int(msg.split()[1][:-1]) % 8 == 7
                 else None)
             t1 = time.perf_counter()
             net = L.LearnedDenoiser(gpu, seed=1)
-            losses = T.train(gpu, net, pairs, a.net_steps, crop_w=a.net_crop, crop_h=a.net_crop, log_every=500,
+            losses = T.train(gpu, net, pairs, a.net_steps, crop_w=a.net_crop, crop_h=a.net_crop, log_every=1000,
                              log=progress)
             t2 = time.perf_counter()
             del pairs
@@ -501,8 +564,9 @@ def main() -> int:
                 "target_spp": a.net_ref_spp, "data_paths": data_paths, "data_s": round(t1 - t0, 2),
                 "train_s": round(t2 - t1, 2), "ms_per_step": round((t2 - t1) / a.net_steps * 1e3, 3),
                 "loss_first_100": float(np.mean(losses[:100])), "loss_last_100": float(np.mean(losses[-100:])),
+                "input_spp": [1, 2, 4, 8, 16, 32, 64], "input_channels": 16,
                 "held_out": "the benchmark scene (seed 0, frequency 229) is not in the training set"}
-        out["time_to_relmse"] = time_to_relmse(L, gpu, scene, blob, a.width, a.height, a.depth, 0.01,
+        out["time_to_relmse"] = time_to_relmse(L, gpu, scene, blob, a.width, a.height, a.depth, (0.01, 1e-3),
                                                a.relmse_ref_spp, a.relmse_max_spp, net)
     if not a.no_batch:
         out["batch"] = batch_c4(L, gpu)

@@ -488,6 +488,10 @@ typedef struct lucida_wavefront_config {
                                  * a path's result: images are bit-identical either way. Rounds of *
                                  * a batched render that mix jobs sort by (job, ray), up to 256    *
                                  * jobs per round.                                                 */
+    uint32_t serialize;         /* single renders, for measurement: one pipeline, and CONNECT and  *
+                                 * the shadow sort on the main stream instead of beside the next  *
+                                 * EXTEND, so per-kernel times (profile_kernels) are solo times.   *
+                                 * Films are bit-identical either way.                             */
 } lucida_wavefront_config;
 
 /* wavefront::render_native_on (crates/gpu/src/wavefront.rs:134): renders

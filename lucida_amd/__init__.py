@@ -94,7 +94,7 @@ class RenderStats(C.Structure):
 class WavefrontConfig(C.Structure):
     _fields_ = [("samples_per_batch", C.c_uint32), ("max_paths", C.c_uint32), ("bvh_width", C.c_uint32),
                 ("use_graph", C.c_uint32), ("count_traversal", C.c_uint32), ("profile_kernels", C.c_uint32),
-                ("ray_order", C.c_uint32)]
+                ("ray_order", C.c_uint32), ("serialize", C.c_uint32)]
 
     RAY_ORDER_AUTO, RAY_ORDER_NEVER, RAY_ORDER_ALWAYS = 0, 1, 2
 

@@ -335,13 +335,14 @@ __global__ void k_upcat_bwd(const bf16* __restrict__ dcat, int c1, int c2, int n
 // Input features from an Accum film (means): log1p(radiance / albedo) (3),
 // albedo (3), normal (3), depth / (1 + depth) (1), log1p of the demodulated
 // luminance's standard error (1), log2(samples) / 16 (1), the a-trous
-// estimate log1p(filtered / albedo) (3), one zero channel. With
+// estimate log1p(filtered / albedo) (3), one zero channel; the a-trous
+// estimate again, in FP32, into base_log (3 floats per pixel). With
 // `target` non-null also writes the target log1p(ref radiance / albedo) (3
 // floats per pixel) from the reference film, demodulated by the INPUT film's
 // albedo so inference can remodulate with the same guide.
 __global__ void k_features(const GpuAccum* __restrict__ film, const GpuAccum* __restrict__ base,
                            const GpuAccum* __restrict__ ref, int fw, int x0, int y0, int cw, int ch, int img,
-                           bf16* __restrict__ feat, float* __restrict__ target) {
+                           bf16* __restrict__ feat, float* __restrict__ base_log, float* __restrict__ target) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= cw || y >= ch) return;
     const size_t src = (size_t)(y0 + y) * fw + (x0 + x);
@@ -376,6 +377,9 @@ __global__ void k_features(const GpuAccum* __restrict__ film, const GpuAccum* __
     f[14] = log1pf(fmaxf(b0.z * binv / al[2], 0.0f));
     f[15] = 0.0f;
     const size_t dst = ((size_t)img * ch + y) * cw + x;
+    // The estimate the output corrects, kept in FP32: in BF16 (8 significant
+    // bits) a zero correction could not even reproduce it.
+    for (int k = 0; k < 3; ++k) base_log[dst * 3 + k] = f[12 + k];
     bf16x8 lo, hi;
     for (int k = 0; k < 8; ++k) {
         lo[k] = f2b(f[k]);
@@ -391,11 +395,12 @@ __global__ void k_features(const GpuAccum* __restrict__ film, const GpuAccum* __
     }
 }
 
-// Mean squared error of (output + a-trous estimate) against the target over
-// the 3 used channels; writes dL/dout (bf16,
-// zero on the padding channels) and a per-block partial loss.
-__global__ void k_loss(const bf16* __restrict__ out, const bf16* __restrict__ feat, const float* __restrict__ target,
-                       size_t pixels, bf16* __restrict__ dout, float* __restrict__ loss_partial) {
+// Mean squared error of (output + the FP32 a-trous estimate) against the
+// target over the 3 used channels; writes dL/dout (bf16, zero on the
+// padding channels) and a per-block partial loss.
+__global__ void k_loss(const bf16* __restrict__ out, const float* __restrict__ base_log,
+                       const float* __restrict__ target, size_t pixels, bf16* __restrict__ dout,
+                       float* __restrict__ loss_partial) {
     __shared__ float red[256];
     float s = 0.0f;
     const float scale = 2.0f / (float)(pixels * 3);
@@ -406,7 +411,7 @@ __global__ void k_loss(const bf16* __restrict__ out, const bf16* __restrict__ fe
             d1[k] = f2b(0.0f);
         }
         for (int k = 0; k < 3; ++k) {
-            const float e = (b2f(out[p * 16 + k]) + b2f(feat[p * 16 + 12 + k])) - target[p * 3 + k];
+            const float e = (b2f(out[p * 16 + k]) + base_log[p * 3 + k]) - target[p * 3 + k];
             s += e * e;
             d0[k] = f2b(scale * e);
         }
@@ -423,9 +428,9 @@ __global__ void k_loss(const bf16* __restrict__ out, const bf16* __restrict__ fe
 }
 
 // Remodulated prediction into an Accum-shaped film: radiance =
-// expm1(max(out + a-trous estimate, 0)) * albedo, samples = 1, guides copied
-// as means.
-__global__ void k_compose(const bf16* __restrict__ out, const bf16* __restrict__ feat,
+// expm1(max(out + FP32 a-trous estimate, 0)) * albedo, samples = 1, guides
+// copied as means.
+__global__ void k_compose(const bf16* __restrict__ out, const float* __restrict__ base_log,
                           const GpuAccum* __restrict__ film, size_t pixels, GpuAccum* __restrict__ dst) {
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pixels) return;
@@ -435,7 +440,7 @@ __global__ void k_compose(const bf16* __restrict__ out, const bf16* __restrict__
     const float al[3] = {fmaxf(a1.x * inv, 1e-3f), fmaxf(a1.y * inv, 1e-3f), fmaxf(a1.z * inv, 1e-3f)};
     float rad[3];
     for (int k = 0; k < 3; ++k)
-        rad[k] = expm1f(fmaxf(b2f(out[p * 16 + k]) + b2f(feat[p * 16 + 12 + k]), 0.0f)) * al[k];
+        rad[k] = expm1f(fmaxf(b2f(out[p * 16 + k]) + base_log[p * 3 + k], 0.0f)) * al[k];
     float4* o = reinterpret_cast<float4*>(dst + p);
     o[0] = make_float4(rad[0], rad[1], rad[2], 1.0f);
     o[1] = make_float4(al[0], al[1], al[2], a1.w * inv);
@@ -538,6 +543,7 @@ struct lucida_net {
     size_t frag_fwd_off[kNetNumLayers], frag_bwd_off[kNetNumLayers], param_off[kNetNumLayers];
     DevBuf acts, dacts, target, loss_partial;
     DevBuf base;  // a-trous estimate of the film being loaded (Accum-shaped)
+    DevBuf base_log;  // its log1p(estimate / albedo) per pixel of the batch, FP32 (3 floats)
     size_t act_capacity = 0;
     uint32_t step = 0;
     bool frags_valid = false;
@@ -606,6 +612,7 @@ static lucida_status net_ensure_acts(lucida_net* net, int n, int h, int w) {
         net->act_capacity = a.total;
     }
     LUCIDA_HIP(net->target.ensure((size_t)n * h * w * 3 * 4), LUCIDA_OOM);
+    LUCIDA_HIP(net->base_log.ensure((size_t)n * h * w * 3 * 4), LUCIDA_OOM);
     LUCIDA_HIP(net->loss_partial.ensure(4096 * 4), LUCIDA_OOM);
     net->n = n;
     net->h = h;
@@ -760,7 +767,9 @@ lucida_status lucida_net_create(lucida_ctx* ctx, uint32_t seed, lucida_net** out
     }
     // He-normal initialisation from a counter-based stream (Box-Muller over
     // PCG draws); the input's six padding channels and the output's thirteen
-    // padding channels start (and, receiving no gradient, stay) at zero.
+    // padding channels start (and, receiving no gradient, stay) at zero. The
+    // output layer starts at zero, so an untrained network returns the a-trous
+    // estimate it corrects.
     std::vector<float> host(np, 0.0f);
     uint32_t state = seed * 747796405u + 1u;
     auto uni = [&]() {
@@ -772,7 +781,7 @@ lucida_status lucida_net_create(lucida_ctx* ctx, uint32_t seed, lucida_net** out
         const NetLayer& L = kNetLayers[i];
         const int cin_used = i == 0 ? kNetInputsUsed : L.cin;
         const int cout_used = i == kNetNumLayers - 1 ? 3 : L.cout;
-        const float stdv = std::sqrt(2.0f / (9.0f * cin_used)) * (i == kNetNumLayers - 1 ? 0.1f : 1.0f);
+        const float stdv = i == kNetNumLayers - 1 ? 0.0f : std::sqrt(2.0f / (9.0f * cin_used));
         float* W = host.data() + net->param_off[i];
         for (int t = 0; t < 9; ++t)
             for (int ci = 0; ci < L.cin; ++ci)
@@ -806,7 +815,7 @@ void lucida_net_destroy(lucida_net* net) {
     if (!net) return;
     (void)hipStreamSynchronize(net->ctx->stream);
     DevBuf* b[] = {&net->params, &net->grads, &net->adam_m, &net->adam_v, &net->frag_fwd, &net->frag_bwd,
-                   &net->acts, &net->dacts, &net->target, &net->loss_partial, &net->base};
+                   &net->acts, &net->dacts, &net->target, &net->loss_partial, &net->base, &net->base_log};
     for (DevBuf* x : b) x->release();
     delete net;
 }
@@ -865,7 +874,8 @@ static lucida_status load_batch(lucida_net* net, const lucida_film* const* input
         atrous_enqueue(net->ctx, f->accum, f->width, f->height, kAtrousDefaults, net->base.as<GpuAccum>());
         hipLaunchKernelGGL(k_features, dim3((w + 127) / 128, h), dim3(128), 0, net->ctx->stream, f->accum,
                            net->base.as<GpuAccum>(), refs ? refs[k]->accum : nullptr, (int)f->width, (int)cx, (int)cy,
-                           (int)w, (int)h, (int)k, X, refs ? net->target.as<float>() : nullptr);
+                           (int)w, (int)h, (int)k, X, net->base_log.as<float>(),
+                           refs ? net->target.as<float>() : nullptr);
     }
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
     return LUCIDA_OK;
@@ -884,7 +894,7 @@ lucida_status lucida_net_forward_backward(lucida_net* net, const lucida_film* co
     const size_t pixels = (size_t)net->n * net->h * net->w;
     const unsigned blocks = 1024;
     hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, net->ctx->stream, net->acts.as<bf16>() + a.l[14],
-                       net->acts.as<bf16>() + a.x, net->target.as<float>(), pixels, net->dacts.as<bf16>() + a.l[14],
+                       net->base_log.as<float>(), net->target.as<float>(), pixels, net->dacts.as<bf16>() + a.l[14],
                        net->loss_partial.as<float>());
     st = net_backward(net);
     if (st != LUCIDA_OK) return st;
@@ -935,7 +945,7 @@ lucida_status lucida_net_denoise(lucida_net* net, const lucida_film* in, lucida_
     const ActPlan a = plan_acts(net->n, net->h, net->w);
     const size_t pixels = (size_t)in->width * in->height;
     hipLaunchKernelGGL(k_compose, dim3(grid1(pixels)), dim3(256), 0, net->ctx->stream, net->acts.as<bf16>() + a.l[14],
-                       net->acts.as<bf16>() + a.x, in->accum, pixels, out->accum);
+                       net->base_log.as<float>(), in->accum, pixels, out->accum);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
     LUCIDA_HIP(hipStreamSynchronize(net->ctx->stream), LUCIDA_VALIDATION);
     return LUCIDA_OK;

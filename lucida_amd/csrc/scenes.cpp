@@ -888,7 +888,7 @@ static void instance_field(Blob& b, uint32_t f, uint32_t count, float extent, ui
     }
     // environment
     std::vector<float> rgba((size_t)env_w * env_h * 4);
-    uint32_t es = pcg_hash(5u * 0x9e3779b9u + 11u);
+    uint32_t es = pcg_hash((5u + seed) * 0x9e3779b9u + 11u);  // seed 0: the config-3 sky (seed 5)
     for (size_t i = 0; i < (size_t)env_w * env_h; ++i) {
         const float z = std::sqrt(-2.0f * std::log(std::max(rng_f32(es), 1e-12f))) * std::cos(TWO_PI * rng_f32(es));
         const float L = std::exp(1.5f * z);

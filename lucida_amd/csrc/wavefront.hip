@@ -1590,17 +1590,18 @@ static bool want_ray_order(const lucida_wavefront_config& cfg, const SceneView& 
 // The sorts after SHADE of bounce b over a batch of P paths, by `bits` key
 // bits, on pipeline `w`: the extension queue SHADE wrote to `qraw` on w.s
 // (unless this was the last bounce; *next_queue receives the sorted queue)
-// and, at the same time on w.s2, the shadow rays (*shadow_order receives the
-// record slots in CONNECT's order). Each stream then waits for the other's
-// sort, so EXTEND and CONNECT start together, after both: a walk's long waves
-// beside a sort would hold the CUs its short kernels need. The caller has
-// recorded w.ev_shaded on w.s after SHADE. Every step is a launch or an event, so
-// the sequence may be captured in a graph.
-static lucida_status sort_queues(WfPipe& w, const Planes& p, uint32_t* qraw, size_t P, uint32_t b, bool extend,
-                                 uint32_t bits, uint32_t** next_queue, uint32_t** shadow_order) {
+// and, at the same time on `side` (w.s2, or w.s in a serialized render), the
+// shadow rays (*shadow_order receives the record slots in CONNECT's order).
+// Each stream then waits for the other's sort, so EXTEND and CONNECT start
+// together, after both: a walk's long waves beside a sort would hold the CUs
+// its short kernels need. The caller has recorded w.ev_shaded on w.s after
+// SHADE. Every step is a launch or an event, so the sequence may be captured
+// in a graph.
+static lucida_status sort_queues(WfPipe& w, hipStream_t side, const Planes& p, uint32_t* qraw, size_t P, uint32_t b,
+                                 bool extend, uint32_t bits, uint32_t** next_queue, uint32_t** shadow_order) {
     *next_queue = qraw;
-    LUCIDA_HIP(hipStreamWaitEvent(w.s2, w.ev_shaded, 0), LUCIDA_VALIDATION);
-    lucida_status st = sort_pairs_counted(w.s2, p.skey, p.sidx, p.ktmp2, p.qtmp2, P, p.counters + SHADOW_AT + b,
+    LUCIDA_HIP(hipStreamWaitEvent(side, w.ev_shaded, 0), LUCIDA_VALIDATION);
+    lucida_status st = sort_pairs_counted(side, p.skey, p.sidx, p.ktmp2, p.qtmp2, P, p.counters + SHADOW_AT + b,
                                           bits, p.sort_scratch2, shadow_order);
     if (st != LUCIDA_OK) return st;
     if (extend) {
@@ -1608,10 +1609,10 @@ static lucida_status sort_queues(WfPipe& w, const Planes& p, uint32_t* qraw, siz
                                 next_queue);
         if (st != LUCIDA_OK) return st;
     }
-    LUCIDA_HIP(hipEventRecord(w.ev_shadow_sorted, w.s2), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipEventRecord(w.ev_shadow_sorted, side), LUCIDA_VALIDATION);
     LUCIDA_HIP(hipEventRecord(w.ev_sorted, w.s), LUCIDA_VALIDATION);
     LUCIDA_HIP(hipStreamWaitEvent(w.s, w.ev_shadow_sorted, 0), LUCIDA_VALIDATION);
-    LUCIDA_HIP(hipStreamWaitEvent(w.s2, w.ev_sorted, 0), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipStreamWaitEvent(side, w.ev_sorted, 0), LUCIDA_VALIDATION);
     return LUCIDA_OK;
 }
 
@@ -1716,7 +1717,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     for (const Item& it : items) item_paths = std::max(item_paths, (size_t)it.band.count * it.spb);
     const bool sort_rays = want_ray_order(cfg, sc, item_paths);
     const uint32_t batches = (uint32_t)items.size();
-    const int npipes = batches >= 2 ? 2 : 1;
+    const int npipes = batches >= 2 && !cfg.serialize ? 2 : 1;
     Planes pl[2];
     GpuUniforms* dUs[2] = {nullptr, nullptr};
     for (int q = 0; q < npipes; ++q) {
@@ -1728,8 +1729,16 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     const uint32_t stats_on = (stats && cfg.count_traversal) ? 1u : 0u;
     const bool profile = stats && cfg.profile_kernels;
 
-    // Optional per-kernel events: one start/stop pair per launch.
-    std::vector<hipEvent_t> evs;
+    // Optional per-kernel events: one start/stop pair per launch, destroyed
+    // on every return path.
+    struct EventSet {
+        std::vector<hipEvent_t> e;
+        ~EventSet() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } event_set;
+    std::vector<hipEvent_t>& evs = event_set.e;
     std::vector<int> ev_kind;
     if (profile) {
         const size_t launches = (size_t)batches * (2 + 4 * depth);
@@ -1759,7 +1768,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
         WfPipe& w = ctx->wf[batch % npipes];
         const Planes& p = pl[batch % npipes];
         GpuUniforms* dU = dUs[batch % npipes];
-        hipStream_t s = w.s, s2 = w.s2;
+        hipStream_t s = w.s, s2 = cfg.serialize ? w.s : w.s2;
         GpuUniforms& u = host_u[batch];
         u.sample_offset = uniforms->sample_offset + it.done;
         u.samples_per_launch = it.spb;
@@ -1791,7 +1800,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
             LUCIDA_HIP(hipEventRecord(w.ev_shaded, s), LUCIDA_VALIDATION);
             if (sort_rays) {
                 mark(5, true, s);
-                st = sort_queues(w, p, qout, Pb, b, b + 1 < depth, RAY_KEY_BITS, &qnext, &order);
+                st = sort_queues(w, s2, p, qout, Pb, b, b + 1 < depth, RAY_KEY_BITS, &qnext, &order);
                 if (st != LUCIDA_OK) return st;
                 mark(5, false, s);
             } else {
@@ -1853,7 +1862,6 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
             }
         }
     }
-    for (auto& e : evs) (void)hipEventDestroy(e);
     return LUCIDA_OK;
 }
 
@@ -2043,12 +2051,22 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     const uint32_t stats_on = (stats && cfg.count_traversal) ? 1u : 0u;
     const SceneView none{};
 
-    auto enqueue_all = [&]() {
+    // Queues every round; returns the first failing status (a HIP call, a
+    // launch or a sort), so an error under capture comes back as a status.
+    auto enqueue_all = [&]() -> lucida_status {
+        hipError_t first = hipSuccess;
+        auto chk = [&](hipError_t e) {
+            if (e != hipSuccess && first == hipSuccess) first = e;
+        };
+        lucida_status sort_st = LUCIDA_OK;
+        auto chk_sort = [&](lucida_status v) {
+            if (v != LUCIDA_OK && sort_st == LUCIDA_OK) sort_st = v;
+        };
         hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s0, stamps, 0u);
         for (int q = 0; q < npipes; ++q) {
             if (q > 0) {  // fork the other pipeline off pipeline 0's stream
-                (void)hipEventRecord(ctx->wf[q].ev_sorted, s0);
-                (void)hipStreamWaitEvent(ctx->wf[q].s, ctx->wf[q].ev_sorted, 0);
+                chk(hipEventRecord(ctx->wf[q].ev_sorted, s0));
+                chk(hipStreamWaitEvent(ctx->wf[q].s, ctx->wf[q].ev_sorted, 0));
             }
             hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, ctx->wf[q].s, p_fields[q].counters,
                                (uint32_t)TOTALS_AT + 24u);
@@ -2077,27 +2095,28 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                     uint32_t* order = nullptr;
                     launch_extend<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s, sc, nullptr, p, qin,
                                          p.counters + b, stats_on);
-                    if (b > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
+                    if (b > 0) chk(hipStreamWaitEvent(s, w.ev_connected, 0));
                     hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, sc, uj,
                                        nullptr, p, qin, p.counters + b, qout, p.counters + b + 1,
                                        p.counters + SHADOW_AT + b, r.sort ? p.qkey : nullptr, whole);
-                    (void)hipEventRecord(w.ev_shaded, s);
+                    chk(hipEventRecord(w.ev_shaded, s));
                     if (r.sort)
-                        (void)sort_queues(w, p, qout, P, b, b + 1 < r.depth, RAY_KEY_BITS, &qnext, &order);
+                        chk_sort(sort_queues(w, w.s2, p, qout, P, b, b + 1 < r.depth, RAY_KEY_BITS, &qnext, &order));
                     else
-                        (void)hipStreamWaitEvent(s2, w.ev_shaded, 0);
+                        chk(hipStreamWaitEvent(s2, w.ev_shaded, 0));
                     launch_connect<false>(r.solo_mode, dim3(grid_for(P, TRACE_WG)), s2, sc, nullptr, p,
                                           p.counters + SHADOW_AT + b, order, stats_on);
-                    (void)hipEventRecord(w.ev_connected, s2);
+                    chk(hipEventRecord(w.ev_connected, s2));
                     qin = qnext;
                 }
-                if (r.depth > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
-                if (npipes > 1 && ri > 0) (void)hipStreamWaitEvent(s, ctx->ev_resolved, 0);
+                if (r.depth > 0) chk(hipStreamWaitEvent(s, w.ev_connected, 0));
+                if (npipes > 1 && ri > 0) chk(hipStreamWaitEvent(s, ctx->ev_resolved, 0));
                 hipLaunchKernelGGL(k_resolve, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, uj, p,
                                    table[r.first].film, whole);
-                if (npipes > 1) (void)hipEventRecord(ctx->ev_resolved, s);
+                if (npipes > 1) chk(hipEventRecord(ctx->ev_resolved, s));
                 hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
                 hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, (uint32_t)(ri + 1));
+                chk(hipGetLastError());
                 continue;
             }
             hipLaunchKernelGGL(k_generate_batch, dim3(grid_for(P, 256)), dim3(256), 0, s, jr, nj, P, p);
@@ -2108,32 +2127,40 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
                 uint32_t* qnext = qout;
                 uint32_t* order = nullptr;
                 launch_extend<true>(mode, dim3(grid_for(P, TRACE_WG)), s, none, jr, p, qin, p.counters + b, stats_on);
-                if (b > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
+                if (b > 0) chk(hipStreamWaitEvent(s, w.ev_connected, 0));
                 hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(P, SHADE_WG)), dim3(SHADE_WG), 0, s, none, nullptr, jr,
                                    p, qin, p.counters + b, qout, p.counters + b + 1, p.counters + SHADOW_AT + b,
                                    r.sort ? p.qkey : nullptr, Band{0u, 0u});
-                (void)hipEventRecord(w.ev_shaded, s);
+                chk(hipEventRecord(w.ev_shaded, s));
                 if (r.sort)
-                    (void)sort_queues(w, p, qout, P, b, b + 1 < r.depth, 24, &qnext, &order);
+                    chk_sort(sort_queues(w, w.s2, p, qout, P, b, b + 1 < r.depth, 24, &qnext, &order));
                 else
-                    (void)hipStreamWaitEvent(s2, w.ev_shaded, 0);
+                    chk(hipStreamWaitEvent(s2, w.ev_shaded, 0));
                 launch_connect<true>(mode, dim3(grid_for(P, TRACE_WG)), s2, none, jr, p, p.counters + SHADOW_AT + b,
                                      order, stats_on);
-                (void)hipEventRecord(w.ev_connected, s2);
+                chk(hipEventRecord(w.ev_connected, s2));
                 qin = qnext;
             }
-            if (r.depth > 0) (void)hipStreamWaitEvent(s, w.ev_connected, 0);
-            if (npipes > 1 && ri > 0) (void)hipStreamWaitEvent(s, ctx->ev_resolved, 0);
+            if (r.depth > 0) chk(hipStreamWaitEvent(s, w.ev_connected, 0));
+            if (npipes > 1 && ri > 0) chk(hipStreamWaitEvent(s, ctx->ev_resolved, 0));
             hipLaunchKernelGGL(k_resolve_batch, dim3(grid_for(r.pixels, 256)), dim3(256), 0, s, jr, nj,
                                (uint32_t)r.pixels, p);
-            if (npipes > 1) (void)hipEventRecord(ctx->ev_resolved, s);
+            if (npipes > 1) chk(hipEventRecord(ctx->ev_resolved, s));
             hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, p.counters, r.depth);
             hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, stamps, (uint32_t)(ri + 1));
+            chk(hipGetLastError());
         }
         for (int q = 1; q < npipes; ++q) {  // join: pipeline 0's stream waits for the others
-            (void)hipEventRecord(ctx->wf[q].ev_shaded, ctx->wf[q].s);
-            (void)hipStreamWaitEvent(s0, ctx->wf[q].ev_shaded, 0);
+            chk(hipEventRecord(ctx->wf[q].ev_shaded, ctx->wf[q].s));
+            chk(hipStreamWaitEvent(s0, ctx->wf[q].ev_shaded, 0));
         }
+        if (sort_st != LUCIDA_OK) return sort_st;
+        if (first != hipSuccess) {
+            char msg[256];
+            snprintf(msg, sizeof msg, "lucida_render_batch: queueing the rounds failed: %s", hipGetErrorString(first));
+            return set_error(LUCIDA_VALIDATION, msg);
+        }
+        return LUCIDA_OK;
     };
 
     if (st == LUCIDA_OK) {
@@ -2146,9 +2173,12 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
             if (hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                 st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: stream capture failed to start");
             } else {
-                enqueue_all();
+                const lucida_status qs = enqueue_all();
+                // the capture is ended whatever happened, so the stream leaves capture mode
                 const hipError_t ec = hipStreamEndCapture(s0, &graph);
-                if (ec != hipSuccess || !graph)
+                if (qs != LUCIDA_OK)
+                    st = qs;
+                else if (ec != hipSuccess || !graph)
                     st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: stream capture failed");
                 else if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
                     st = set_error(LUCIDA_VALIDATION, "lucida_render_batch: graph instantiation failed");
@@ -2159,8 +2189,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
             if (exec) (void)hipGraphExecDestroy(exec);
             if (graph) (void)hipGraphDestroy(graph);
         } else {
-            enqueue_all();
-            if (hipGetLastError() != hipSuccess) st = set_error(LUCIDA_SHADER, "lucida_render_batch: launch failed");
+            st = enqueue_all();
         }
         LUCIDA_HIP(hipEventRecord(ctx->ev1, s0), LUCIDA_VALIDATION);
         LUCIDA_HIP(hipEventSynchronize(ctx->ev1), LUCIDA_VALIDATION);

@@ -37,18 +37,24 @@ def orbit_camera(u: L.Uniforms, angle: float, lift: float = 0.0) -> L.Uniforms:
 
 
 def scene_variant(k: int, rng: np.random.Generator):
-    """Scene k of the training set: mostly displaced spheres in the room with
-    random seeds, noise amplitudes and tessellations, some Cornell boxes."""
+    """Scene k of the training set (seeds 1000 + k, so the benchmark scene,
+    seed 0, is never in it): mostly displaced spheres in the room with random
+    noise amplitudes, tessellations and orbiting cameras (the config-2
+    generator), some instanced fields of rough dielectric and GGX meshes under
+    a log-normal sky with hot spots (the config-3 generator), some Cornell
+    boxes."""
     kind = rng.random()
-    if kind < 0.8:
+    if kind < 0.7:
         blob = L.Blob("sphere-c2", seed=1000 + k, frequency=int(rng.integers(24, 96)),
                       displacement=float(rng.uniform(0.01, 0.12)))
         return blob, float(rng.uniform(-0.9, 0.9)), float(rng.uniform(-0.2, 0.4))
-    return L.Blob("box-c1" if kind < 0.9 else "cornell-mesh"), 0.0, 0.0
+    if kind < 0.85:
+        return L.Blob("instances-small", seed=1000 + k, frequency=int(rng.integers(3, 8))), 0.0, 0.0
+    return L.Blob("box-c1" if kind < 0.93 else "cornell-mesh"), 0.0, 0.0
 
 
 def make_training_set(gpu: L.Gpu, scenes: int = 24, width: int = 256, height: int = 256,
-                      input_spp=(1, 2, 4, 8, 16, 32), ref_spp: int = 1024, depth: int = 8, seed: int = 7,
+                      input_spp=(1, 2, 4, 8, 16, 32, 64), ref_spp: int = 4096, depth: int = 8, seed: int = 7,
                       log=None):
     """Renders the (input, target) film pairs; returns them and the paths traced."""
     rng = np.random.default_rng(seed)

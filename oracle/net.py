@@ -98,12 +98,15 @@ def target(accum_in: np.ndarray, accum_ref: np.ndarray) -> np.ndarray:
     return np.log1p(np.maximum(rr / al, 0.0))
 
 
-def compose(out: np.ndarray, accum_in: np.ndarray, x: np.ndarray) -> np.ndarray:
-    """Network output + the a-trous channels of its features `x` -> Accum-shaped
-    film (radiance, samples = 1, guides)."""
+def compose(out: np.ndarray, accum_in: np.ndarray, x: np.ndarray, base: np.ndarray | None = None) -> np.ndarray:
+    """Network output + the a-trous estimate -> Accum-shaped film (radiance,
+    samples = 1, guides). `base` is the estimate's log channels at full
+    precision (the device keeps them in FP32); by default the BF16-stored
+    channels 12..14 of the features `x`."""
     a, _, inv, al, _ = _means(accum_in)
     res = np.zeros(a.shape[:-1] + (16,))
-    res[..., 0:3] = np.expm1(np.maximum(out[..., 0:3] + x[..., 12:15], 0.0)) * al
+    b = x[..., 12:15] if base is None else base
+    res[..., 0:3] = np.expm1(np.maximum(out[..., 0:3] + b, 0.0)) * al
     res[..., 3] = 1.0
     res[..., 4:7] = al
     res[..., 7] = a[..., 7] * inv
@@ -211,18 +214,20 @@ def activations_from_layers(x, layer_outputs, bf16=False):
     return A
 
 
-def loss_and_grad(params, x, tgt, bf16=False, acts=None):
-    """Mean squared error of (output + a-trous channels of x) against the
+def loss_and_grad(params, x, tgt, bf16=False, acts=None, base=None):
+    """Mean squared error of (output + the a-trous estimate) against the
     target over the 3 used output channels, and its gradient
     with respect to the flat parameters. Also returns the activation
     gradients ('d*' keys), computed as the device stores them. `acts`
     (from activations_from_layers) replaces the forward pass, so the backward
-    pass can be checked on its own."""
+    pass can be checked on its own. `base` is the estimate's log channels at
+    full precision (the device keeps them in FP32), by default channels
+    12..14 of x."""
     R = to_bf16 if bf16 else (lambda v: v)
     A = forward(params, x, bf16) if acts is None else acts
     layers = [(R(W), b) for W, b in unpack(np.asarray(params, np.float64))]
     out = A["l14"]
-    e = (out[..., :3] + A["x"][..., 12:15]) - tgt
+    e = (out[..., :3] + (A["x"][..., 12:15] if base is None else base)) - tgt
     loss = float((e * e).mean())
     d = np.zeros_like(out)
     d[..., :3] = R(2.0 / e.size * e)

@@ -99,7 +99,8 @@ def setup(gpu):
     base = [atrous_of(gpu, f) for f in ins]
     x_o = np.stack([N.features(crop(r, c), crop(b, c)) for r, b, c in zip(raw_in, base, CROPS)])
     t_o = np.stack([N.target(crop(ri, c), crop(rr, c)) for ri, rr, c in zip(raw_in, raw_ref, CROPS)])
-    return dict(ins=ins, refs=refs, raw_in=raw_in, net=net, p=p, loss=loss, x_o=x_o, t_o=t_o)
+    return dict(ins=ins, refs=refs, raw_in=raw_in, net=net, p=p, loss=loss, x_o=x_o, t_o=t_o,
+                base_o=x_o[..., 12:15])
 
 
 def test_parameter_count(setup):
@@ -127,7 +128,8 @@ def test_forward_matches_oracle_layer_by_layer(setup):
 def test_loss_and_gradients_match_oracle(setup):
     net = setup["net"]
     x_g = net.tensor(0).astype(np.float64)
-    loss_o, g_o, _, D = N.loss_and_grad(setup["p"].astype(np.float64), x_g, setup["t_o"], bf16=True)
+    loss_o, g_o, _, D = N.loss_and_grad(setup["p"].astype(np.float64), x_g, setup["t_o"], bf16=True,
+                                        base=setup["base_o"])
     assert abs(setup["loss"] - loss_o) <= 2e-3 * loss_o, (setup["loss"], loss_o)
     g_g = net.params(gradients=True).astype(np.float64)
     off = 0
@@ -148,7 +150,8 @@ def test_backward_on_device_activations_matches_oracle(setup):
     net = setup["net"]
     x_g = net.tensor(0).astype(np.float64)
     A = N.activations_from_layers(x_g, [net.tensor(i + 1) for i in range(15)], bf16=True)
-    _, g_o, _, D = N.loss_and_grad(setup["p"].astype(np.float64), x_g, setup["t_o"], bf16=True, acts=A)
+    _, g_o, _, D = N.loss_and_grad(setup["p"].astype(np.float64), x_g, setup["t_o"], bf16=True, acts=A,
+                                   base=setup["base_o"])
     g_g = net.params(gradients=True).astype(np.float64)
     off = 0
     for i, (ci, co, _) in enumerate(N.LAYERS):
@@ -192,7 +195,9 @@ def test_partial_weight_gradient_chunk(gpu):
     say("net forward_backward case: 1 crop 128x8")
     loss = net.forward_backward(ins, refs, CW, 8)
     x_g = net.tensor(0).astype(np.float64)
-    loss_o, g_o, _, _ = N.loss_and_grad(p.astype(np.float64), x_g, N.target(raw_in[0], raw_ref[0])[None], bf16=True)
+    base_o = N.features(raw_in[0], atrous_of(gpu, ins[0]))[None, ..., 12:15]
+    loss_o, g_o, _, _ = N.loss_and_grad(p.astype(np.float64), x_g, N.target(raw_in[0], raw_ref[0])[None], bf16=True,
+                                        base=base_o)
     assert abs(loss - loss_o) <= 2e-3 * loss_o
     assert rel_l2(net.params(gradients=True).astype(np.float64), g_o) < 3e-2
 
@@ -228,7 +233,8 @@ def test_denoise_matches_oracle(gpu):
     got = as_float16(out.download()).reshape(32, 128, 16)
     x_g = net.tensor(0).astype(np.float64)
     A = N.forward(p.astype(np.float64), x_g, bf16=True)
-    exp = N.compose(A["l14"][0], raw[0], x_g[0])
+    base_o = N.features(raw[0], atrous_of(gpu, ins[0]))[..., 12:15]
+    exp = N.compose(A["l14"][0], raw[0], x_g[0], base=base_o)
     assert rel_l2(got[..., 0:3], exp[..., 0:3]) < 1.5e-2
     np.testing.assert_array_equal(got[..., 3], 1.0)
     np.testing.assert_allclose(got[..., 4:11], exp[..., 4:11], rtol=1e-6, atol=1e-7)
