@@ -503,6 +503,18 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
                                       uint32_t samples, lucida_film* film, const lucida_wavefront_config* config,
                                       lucida_render_stats* stats);
 
+/* The megakernel render (pt_gpu::render_native_on, crates/gpu/src/lib.rs:391,
+ * shaders/trace/megakernel.wgsl): the same render as lucida_render_wavefront,
+ * with every path traced start to finish by one thread (one thread per
+ * pixel, its samples in order). The films are equal; the comparison is the
+ * architecture trade-off (README.md:675-714). With `render_with`
+ * (lib.rs:69) Architecture::Megakernel maps here and ::Wavefront to
+ * lucida_render_wavefront. Of config only samples_per_batch, bvh_width and
+ * count_traversal apply; stats->kernel_ms[1] holds the device time. */
+lucida_status lucida_render_megakernel(lucida_ctx* ctx, const lucida_scene* scene, const GpuUniforms* uniforms,
+                                       uint32_t samples, lucida_film* film, const lucida_wavefront_config* config,
+                                       lucida_render_stats* stats);
+
 /* One render request of a batch: like one lucida_render_wavefront call. */
 typedef struct lucida_job {
     const lucida_scene* scene;
@@ -534,6 +546,16 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
  * (energy::compute_table, crates/core/src/bsdf.rs:1200-1227), built on the
  * device at first use: 32 x 32 floats, row = roughness, column = sqrt(cos). */
 lucida_status lucida_energy_table(lucida_ctx* ctx, float* out_1024);
+
+/* ConvergenceMeter::record + read (crates/gpu/src/stats.rs:104, :153;
+ * shaders/stats/convergence.wgsl): the mean over the film's pixels of the
+ * relative standard error of each pixel's mean, from its accumulated sum and
+ * sum of squares, sqrt(luminance(radiance_sq/N - mean^2) / N) /
+ * luminance(mean); pixels with N < 2 or a mean luminance <= 1e-3 are left
+ * out, and *out is -1 when none qualifies. Assumes independent samples, so
+ * with the Sobol sampler it over-reports (a conservative figure). Waits for
+ * the result. */
+lucida_status lucida_film_convergence(lucida_film* film, float* out);
 
 /* ------------------------------------------------------------------------ */
 /* À-trous denoiser (crates/gpu/src/denoise.rs)                             */

@@ -214,6 +214,9 @@ def lib() -> C.CDLL:
             "lucida_blob_set_bvh": ([P, P, S, P], I),
             "lucida_blob_set_env": ([P, U32, U32, P], I),
             "lucida_blob_camera": ([P, C.POINTER(Camera)], I),
+            "lucida_film_convergence": ([P, C.POINTER(C.c_float)], I),
+            "lucida_render_megakernel": ([P, P, C.POINTER(Uniforms), U32, P, C.POINTER(WavefrontConfig),
+                                          C.POINTER(RenderStats)], I),
             "lucida_camera_write_uniforms": ([C.POINTER(Camera), C.c_float, C.POINTER(Uniforms)], I),
         }
         for name, (args, res) in sigs.items():
@@ -534,6 +537,13 @@ class Film:
         _check(lib().lucida_film_download(self._h, _ptr(a)))
         return a
 
+    def convergence(self) -> float:
+        """ConvergenceMeter (crates/gpu/src/stats.rs): the mean relative standard
+        error of the film's pixels, -1 when no pixel is measurable."""
+        v = C.c_float()
+        _check(lib().lucida_film_convergence(self._h, C.byref(v)))
+        return float(v.value)
+
     def upload(self, accum: np.ndarray) -> None:
         a = np.ascontiguousarray(accum, dtype=ACCUM)
         assert len(a) == self.width * self.height
@@ -548,6 +558,30 @@ def render_wavefront(gpu: Gpu, scene: Scene, uniforms: Uniforms, samples: int, f
     _check(lib().lucida_render_wavefront(gpu._h, scene._h, C.byref(uniforms), samples, film._h, cfg,
                                          C.byref(st)))
     return st
+
+
+def render_megakernel(gpu: Gpu, scene: Scene, uniforms: Uniforms, samples: int, film: Film,
+                      config: WavefrontConfig | None = None) -> RenderStats:
+    """The megakernel render (pt_gpu::render_native_on, crates/gpu/src/lib.rs:391): one
+    thread per pixel traces its paths start to finish; the film equals the wavefront's."""
+    st = RenderStats()
+    cfg = C.byref(config) if config is not None else None
+    _check(lib().lucida_render_megakernel(gpu._h, scene._h, C.byref(uniforms), samples, film._h, cfg,
+                                          C.byref(st)))
+    return st
+
+
+ARCHITECTURES = ("megakernel", "wavefront")
+
+
+def render_with(gpu: Gpu, scene: Scene, uniforms: Uniforms, samples: int, film: Film, architecture: str = "wavefront",
+                config: WavefrontConfig | None = None) -> RenderStats:
+    """render_with (crates/gpu/src/lib.rs:69): either architecture; both give the same film."""
+    if architecture == "megakernel":
+        return render_megakernel(gpu, scene, uniforms, samples, film, config)
+    if architecture == "wavefront":
+        return render_wavefront(gpu, scene, uniforms, samples, film, config)
+    raise ValueError(f"unknown architecture {architecture!r}")
 
 
 def render_batch(gpu: Gpu, jobs, config: WavefrontConfig | None = None):

@@ -130,7 +130,8 @@ void lucida_ctx_destroy(lucida_ctx* ctx) {
     DevBuf* bufs[] = {&ctx->lb_bounds, &ctx->lb_partial, &ctx->lb_codes, &ctx->lb_vals,   &ctx->lb_tmpk,
                       &ctx->lb_tmpv,   &ctx->lb_hist,    &ctx->lb_karras, &ctx->lb_parent, &ctx->lb_nodebox,
                       &ctx->lb_subtree, &ctx->lb_flags,  &ctx->lb_misc,  &ctx->lb_rank,   &ctx->lb_nodes,
-                      &ctx->lb_tris,   &ctx->lb_pos,     &ctx->lb_prims, &ctx->lb_level, &ctx->ggx_table, &ctx->dn_buf};
+                      &ctx->lb_tris,   &ctx->lb_pos,     &ctx->lb_prims, &ctx->lb_level, &ctx->ggx_table, &ctx->dn_buf,
+                      &ctx->conv_buf, &ctx->mk_spill, &ctx->mk_state};
     for (DevBuf* b : bufs) b->release();
     for (WfPipe& w : ctx->wf) {
         DevBuf* wb[] = {&w.planes, &w.hits, &w.shadow, &w.queues, &w.counters, &w.uniforms, &w.spill};

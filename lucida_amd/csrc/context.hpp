@@ -83,6 +83,11 @@ struct lucida_ctx {
     bool ggx_ready = false;
     // Denoiser planes.
     lucida::DevBuf dn_buf;
+    // Convergence meter partials and result.
+    lucida::DevBuf conv_buf;
+    // Megakernel renders: traversal stack spill columns; counters and the
+    // batches' uniforms.
+    lucida::DevBuf mk_spill, mk_state;
 };
 
 struct lucida_scene {

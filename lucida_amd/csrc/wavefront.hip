@@ -221,17 +221,15 @@ struct Band {
     uint32_t first, count;
 };
 
-// Path `index` of the batch; `local` is its index within its own render
-// (sample-major over the band: local = sample * band.count + pixel - band.first).
-__device__ inline void generate_path(const GpuUniforms& u, uint32_t index, uint32_t local, Planes& p, Band band) {
-    const uint32_t pixel = band.first + local % band.count;
-    const uint32_t sample = u.sample_offset + local / band.count;
+// The primary ray of (pixel, sample) (camera.rs:84 generate_ray): the
+// sampler is created and its pixel and lens draws taken.
+__device__ inline void camera_ray(const GpuUniforms& u, uint32_t pixel, uint32_t sample, Sampler* s, V3* origin_out,
+                                  V3* dir_out) {
     const uint32_t x = pixel % u.width, y = pixel / u.width;
-    Sampler s = make_sampler(u, pixel, sample);
+    *s = make_sampler(u, pixel, sample);
     float pu, pv, lu, lv;
-    s.next_vec2(&pu, &pv);
-    s.next_vec2(&lu, &lv);
-    // camera.rs:84 generate_ray
+    s->next_vec2(&pu, &pv);
+    s->next_vec2(&lu, &lv);
     float sx = ((float)x + pu) / (float)u.width;
     float ty = ((float)y + pv) / (float)u.height;
     V3 origin = ld3(u.cam_origin);
@@ -244,7 +242,18 @@ __device__ inline void generate_path(const GpuUniforms& u, uint32_t index, uint3
         V3 right = normalize(ld3(u.cam_horizontal)), down = normalize(ld3(u.cam_vertical));
         origin = origin + (right * dx + down * dy);
     }
-    V3 d = normalize(target - origin);
+    *origin_out = origin;
+    *dir_out = normalize(target - origin);
+}
+
+// Path `index` of the batch; `local` is its index within its own render
+// (sample-major over the band: local = sample * band.count + pixel - band.first).
+__device__ inline void generate_path(const GpuUniforms& u, uint32_t index, uint32_t local, Planes& p, Band band) {
+    const uint32_t pixel = band.first + local % band.count;
+    const uint32_t sample = u.sample_offset + local / band.count;
+    Sampler s;
+    V3 origin, d;
+    camera_ray(u, pixel, sample, &s, &origin, &d);
     p.org[index] = make_float4(origin.x, origin.y, origin.z, __uint_as_float(0u));
     p.dir[index] = make_float4(d.x, d.y, d.z, __uint_as_float(s.state));
     p.thr[index] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -953,10 +962,225 @@ __device__ inline uint32_t queue_key(const SceneView& sc, int job, V3 o, V3 d) {
 
 // ---------------------------------------------------------------------------
 // SHADE (shade.wgsl:120-348 = integrator.rs radiance_with_guide, one bounce).
+// ---------------------------------------------------------------------------
+// A path between bounces, in registers: its ray, throughput, radiance so
+// far, the previous vertex and BSDF pdf (for MIS on emission), its bounce
+// depth and its sampler.
+struct PathRegs {
+    V3 origin, dir, throughput, radiance, prev_position;
+    float prev_bsdf_pdf;
+    uint32_t depth;
+    Sampler s;
+};
+// A deferred shadow ray: origin, direction, t_max and the contribution it
+// adds to its path's radiance when unoccluded.
+struct ShadowRay {
+    V3 o, d, c;
+    float t_max;
+};
+
+// One bounce of radiance_with_guide for the ray's hit (t, u, v, id; `inst`
+// is the hit's instance in an instanced scene): adds emission to
+// P.radiance, emits at most one shadow ray (`shadow`, `sr`), writes the
+// depth-0 guides (`guide`, `gal` = albedo and hit distance, `gnr` = normal)
+// and samples the BSDF. Returns whether the path continues; P then holds the
+// next ray.
+__device__ __forceinline__ bool shade_bounce(const SceneView& sc, const GpuUniforms& u, const float4 hit, uint32_t inst,
+                                    PathRegs& P, bool& shadow, ShadowRay& sr, bool& guide, float4& gal,
+                                    float4& gnr) {
+    shadow = false;
+    guide = false;
+    const V3 origin = P.origin, dir = P.dir;
+    const uint32_t depth = P.depth;
+    const uint32_t mode = u.sampling_mode > 2 ? 2 : u.sampling_mode;
+    const uint32_t total = (uint32_t)sc.num_lights + (sc.env.present() ? 1u : 0u);
+    const uint32_t id = __float_as_uint(hit.w);
+
+    if (id == MISS) {
+        if (sc.env.present()) {
+            float w = 1.0f;
+            if (depth > 0) {
+                if (mode == LUCIDA_SAMPLING_NEE) w = 0.0f;
+                else if (mode == LUCIDA_SAMPLING_MIS)
+                    w = power_heuristic(P.prev_bsdf_pdf, env_pdf(sc.env, dir) / (float)max(total, 1u));
+            }
+            if (w > 0.0f) P.radiance = P.radiance + P.throughput * env_radiance(sc.env, dir) * w;
+        } else {
+            P.radiance = P.radiance + P.throughput * ld3(u.background);
+        }
+        return false;
+    }
+    // Surface at the hit: shade_triangle / shade_analytic (scene.rs:405-560).
+    const float t = hit.x;
+    V3 position = origin + t * dir;
+    V3 normal, gnormal;
+    bool front;
+    uint32_t material;
+    float light_area;
+    if (id & ANALYTIC) {
+        const GpuPrimitive& pr = sc.prims[id & ~ANALYTIC];
+        const bool quad = pr.kind == LUCIDA_PRIM_KIND_QUAD;
+        light_area = quad ? length(cross(ld3(pr.edge_u), ld3(pr.edge_v))) : 0.0f;
+        V3 gn = quad ? ld3(pr.normal) : (position - ld3(pr.position)) / pr.radius;
+        front = dot(gn, dir) < 0.0f;
+        gnormal = front ? gn : -gn;
+        normal = gnormal;
+        material = pr.material;
+    } else {
+        const GpuTriangle tri = sc.tris[id];
+        const V3 p0 = f4xyz(sc.pos[tri.i0]), p1 = f4xyz(sc.pos[tri.i1]), p2 = f4xyz(sc.pos[tri.i2]);
+        const float w = 1.0f - hit.y - hit.z;
+        V3 sh = normalize_or(w * ld3(sc.attrs[tri.i0].normal) + hit.y * ld3(sc.attrs[tri.i1].normal) +
+                                 hit.z * ld3(sc.attrs[tri.i2].normal),
+                             v3(0, 0, 1));
+        V3 cr = cross(p1 - p0, p2 - p0);
+        V3 g = normalize_or(cr, v3(0, 0, 1));
+        material = tri.material;
+        if (sc.num_instances) {
+            // shade_instanced_triangle (scene.rs:375): orient in object
+            // space, bring both normals to world space, re-decide the face
+            // there, apply the material override; instanced triangles are
+            // not lights.
+            const InstanceXf I = load_instance(sc.insts, inst);
+            const float oflip = dot(g, xf_vector(I, dir)) < 0.0f ? 1.0f : -1.0f;
+            const V3 n_w = xf_normal(I, sh * oflip), g_w = xf_normal(I, g * oflip);
+            front = dot(g_w, dir) < 0.0f;
+            const float flip = front ? 1.0f : -1.0f;
+            normal = n_w * flip;
+            gnormal = g_w * flip;
+            if (I.material != 0xFFFFFFFFu) material = I.material;
+            light_area = 0.0f;
+        } else {
+            front = dot(g, dir) < 0.0f;
+            const float flip = front ? 1.0f : -1.0f;
+            normal = sh * flip;
+            gnormal = g * flip;
+            light_area = 0.5f * length(cr);
+        }
+    }
+    const GpuMaterial m = sc.materials[material];
+    const V3 emissive = ld3(m.emissive);
+    if (front && max3(emissive) > 0.0f) {
+        float w = 1.0f;
+        if (depth > 0) {
+            if (mode == LUCIDA_SAMPLING_NEE) w = 0.0f;
+            else if (mode == LUCIDA_SAMPLING_MIS) {
+                // light_pdf_from_geometry (light.rs:241) * n / total
+                float pdf_light = 0.0f;
+                const uint32_t n = sc.num_lights;
+                if (n != 0 && light_area > 0.0f) {
+                    V3 tl = position - P.prev_position;
+                    float dist_sq = dot(tl, tl);
+                    if (dist_sq > 0.0f) {
+                        V3 wl = tl / sqrtf(dist_sq);
+                        float cos_l = dot(gnormal, -wl);
+                        if (cos_l > 1.0e-6f) pdf_light = (1.0f / ((float)n * light_area)) * dist_sq / cos_l;
+                    }
+                }
+                pdf_light = pdf_light * ((float)n / (float)max(total, 1u));
+                w = power_heuristic(P.prev_bsdf_pdf, pdf_light);
+            }
+        }
+        if (w > 0.0f) P.radiance = P.radiance + P.throughput * emissive * w;
+    }
+    const Surface surf = surface_from_material(m, front, sc.ggx);
+    if (depth == 0) {
+        V3 ga = vmax(surf.diffuse_albedo + surf.f0 + emissive, v3s(1.0e-3f));
+        guide = true;
+        gal = make_float4(ga.x, ga.y, ga.z, t);
+        gnr = make_float4(normal.x, normal.y, normal.z, 0.0f);
+    }
+    const V3 wo = to_local(-dir, normal);
+    if (!(wo.z > 0.0f)) return false;
+    Sampler& s = P.s;
+    if (mode != LUCIDA_SAMPLING_BSDF && depth + 1 < u.max_depth && total > 0) {
+        // direct_light (integrator.rs:418): the contribution is resolved
+        // here; the shadow ray only answers visibility.
+        const float u_select = s.next_f32();
+        float ua_x, ua_y;
+        s.next_vec2(&ua_x, &ua_y);
+        const uint32_t n_lights = sc.num_lights;
+        const uint32_t pick = min((uint32_t)(u_select * (float)total), total - 1u);
+        V3 wi_world{0, 1, 0}, rin{0, 0, 0};
+        float distance = 0.0f, pdf = 0.0f;
+        bool valid = false;
+        if (pick < n_lights) {
+            const float u_light = ((float)pick + 0.5f) / (float)n_lights;
+            const uint32_t li = min((uint32_t)(u_light * (float)n_lights), n_lights - 1u);
+            const GpuLight L = sc.lights[li];
+            V3 lo = ld3(L.origin), eu = ld3(L.edge_u), ev = ld3(L.edge_v);
+            V3 pt;
+            if (L.kind == LUCIDA_LIGHT_KIND_TRIANGLE) {
+                float su = sqrtf(ua_x);
+                pt = lo + (1.0f - su) * eu + (ua_y * su) * ev;
+            } else {
+                pt = lo + ua_x * eu + ua_y * ev;
+            }
+            V3 tl = pt - position;
+            float dist_sq = dot(tl, tl);
+            if (dist_sq > 0.0f) {
+                float dist = sqrtf(dist_sq);
+                V3 wl = tl / dist;
+                float cos_l = dot(ld3(L.normal), -wl);
+                if (cos_l > 1.0e-6f) {
+                    float lp = (1.0f / ((float)n_lights * L.area)) * dist_sq / cos_l;
+                    if (isfinite(lp) && lp > 0.0f) {
+                        wi_world = wl;
+                        rin = ld3(sc.materials[L.material].emissive);
+                        distance = dist;
+                        pdf = lp * ((float)n_lights / (float)total);
+                        valid = true;
+                    }
+                }
+            }
+        } else {
+            V3 ed, er;
+            float ep = env_sample(sc.env, ua_x, ua_y, &ed, &er);
+            if (ep > 0.0f) {
+                wi_world = ed;
+                rin = er;
+                distance = INFINITY;
+                pdf = ep / (float)total;
+                valid = true;
+            }
+        }
+        if (valid && pdf > 0.0f) {
+            V3 wi = to_local(wi_world, normal);
+            if (wi.z != 0.0f && (wi.z > 0.0f || surf.transmission > 0.0f)) {
+                V3 f = surface_eval(surf, wo, wi);
+                if (max3(f) > 0.0f) {
+                    float mis = mode == LUCIDA_SAMPLING_MIS ? power_heuristic(pdf, surface_pdf(surf, wo, wi)) : 1.0f;
+                    if (mis > 0.0f) {
+                        sr.o = offset_ray_origin(position, wi.z < 0.0f ? -gnormal : gnormal);
+                        sr.d = wi_world;
+                        sr.t_max = distance * (1.0f - 1.0e-3f);
+                        sr.c = P.throughput * (f * fabsf(wi.z) * rin * (mis / pdf));
+                        shadow = true;
+                    }
+                }
+            }
+        }
+    }
+    const float u_lobe = s.next_f32();
+    float bx, by;
+    s.next_vec2(&bx, &by);
+    V3 wi, weight;
+    const float bpdf = surface_sample(surf, wo, u_lobe, bx, by, &wi, &weight);
+    if (!(bpdf > 0.0f)) return false;
+    V3 nt = P.throughput * weight;
+    if (!(max3(nt) > 0.0f && depth + 1 < u.max_depth)) return false;
+    P.origin = offset_ray_origin(position, wi.z < 0.0f ? -gnormal : gnormal);
+    P.dir = to_world(wi, normal);
+    P.throughput = nt;
+    P.prev_bsdf_pdf = bpdf;
+    P.prev_position = position;
+    P.depth = depth + 1;
+    return true;
+}
+
 // With `key_out`, each appended extension-queue entry gets its ray's sort key
 // (queue_key) at the same slot, and each shadow ray its key and slot in
 // skey / sidx.
-// ---------------------------------------------------------------------------
 template <bool BATCH>
 __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUniforms* __restrict__ U,
                                               const BatchJob* __restrict__ jobs, Planes p,
@@ -968,8 +1192,7 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
     const uint32_t count = *count_ptr;
     bool alive = false, shadow = false;
     uint32_t pid = 0, key = 0, skey = 0;
-    V3 sr_o{0, 0, 0}, sr_d{0, 0, 0}, sr_c{0, 0, 0};
-    float sr_t = 0.0f;
+    ShadowRay sr{};
     if (i < count) {
       pid = queue_in[i];
       const uint32_t jid = BATCH ? p.job[pid] : 0u;
@@ -980,208 +1203,37 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
         const GpuUniforms& u = BATCH ? jobs[j].u : *U;
         const uint32_t local = pid - (BATCH ? jobs[j].path_base : 0u);  // index within its own job
         const float4 hit = p.hit[i];
-        float4 of = p.org[pid], df = p.dir[pid], tf = p.thr[pid], pf = p.prv[pid], rf = p.rad[pid];
-        const uint32_t depth = __float_as_uint(of.w);
-        const V3 origin = f4xyz(of), dir = f4xyz(df);
-        V3 throughput = f4xyz(tf), radiance = f4xyz(rf), prev_position = f4xyz(pf);
-        const float prev_bsdf_pdf = tf.w;
+        const float4 of = p.org[pid], df = p.dir[pid], tf = p.thr[pid], pf = p.prv[pid], rf = p.rad[pid];
         const Band band = BATCH ? Band{0u, jobs[j].pixels} : band0;
-        Sampler s;
-        s.sobol = u.sampler_kind == LUCIDA_SAMPLER_SOBOL;
-        s.state = __float_as_uint(df.w);
-        s.index = u.sample_offset + local / band.count;
-        s.seed = hash_u32(band.first + local % band.count + u.frame_seed * 0x9e3779b9u);
-        s.dim = __float_as_uint(pf.w);
-        const uint32_t mode = u.sampling_mode > 2 ? 2 : u.sampling_mode;
-        const uint32_t total = (uint32_t)sc.num_lights + (sc.env.present() ? 1u : 0u);
-        const uint32_t id = __float_as_uint(hit.w);
-
-        if (id == MISS) {
-            if (sc.env.present()) {
-                float w = 1.0f;
-                if (depth > 0) {
-                    if (mode == LUCIDA_SAMPLING_NEE) w = 0.0f;
-                    else if (mode == LUCIDA_SAMPLING_MIS)
-                        w = power_heuristic(prev_bsdf_pdf, env_pdf(sc.env, dir) / (float)max(total, 1u));
-                }
-                if (w > 0.0f) radiance = radiance + throughput * env_radiance(sc.env, dir) * w;
-            } else {
-                radiance = radiance + throughput * ld3(u.background);
-            }
-            p.rad[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
-        } else {
-            // Surface at the hit: shade_triangle / shade_analytic (scene.rs:405-560).
-            const float t = hit.x;
-            V3 position = origin + t * dir;
-            V3 normal, gnormal;
-            bool front;
-            uint32_t material;
-            float light_area;
-            if (id & ANALYTIC) {
-                const GpuPrimitive& pr = sc.prims[id & ~ANALYTIC];
-                const bool quad = pr.kind == LUCIDA_PRIM_KIND_QUAD;
-                light_area = quad ? length(cross(ld3(pr.edge_u), ld3(pr.edge_v))) : 0.0f;
-                V3 gn = quad ? ld3(pr.normal) : (position - ld3(pr.position)) / pr.radius;
-                front = dot(gn, dir) < 0.0f;
-                gnormal = front ? gn : -gn;
-                normal = gnormal;
-                material = pr.material;
-            } else {
-                const GpuTriangle tri = sc.tris[id];
-                const V3 p0 = f4xyz(sc.pos[tri.i0]), p1 = f4xyz(sc.pos[tri.i1]), p2 = f4xyz(sc.pos[tri.i2]);
-                const float w = 1.0f - hit.y - hit.z;
-                V3 sh = normalize_or(w * ld3(sc.attrs[tri.i0].normal) + hit.y * ld3(sc.attrs[tri.i1].normal) +
-                                         hit.z * ld3(sc.attrs[tri.i2].normal),
-                                     v3(0, 0, 1));
-                V3 cr = cross(p1 - p0, p2 - p0);
-                V3 g = normalize_or(cr, v3(0, 0, 1));
-                material = tri.material;
-                if (sc.num_instances) {
-                    // shade_instanced_triangle (scene.rs:375): orient in object
-                    // space, bring both normals to world space, re-decide the
-                    // face there, apply the material override; instanced
-                    // triangles are not lights.
-                    const InstanceXf I = load_instance(sc.insts, p.hin[i]);
-                    const float oflip = dot(g, xf_vector(I, dir)) < 0.0f ? 1.0f : -1.0f;
-                    const V3 n_w = xf_normal(I, sh * oflip), g_w = xf_normal(I, g * oflip);
-                    front = dot(g_w, dir) < 0.0f;
-                    const float flip = front ? 1.0f : -1.0f;
-                    normal = n_w * flip;
-                    gnormal = g_w * flip;
-                    if (I.material != 0xFFFFFFFFu) material = I.material;
-                    light_area = 0.0f;
-                } else {
-                    front = dot(g, dir) < 0.0f;
-                    const float flip = front ? 1.0f : -1.0f;
-                    normal = sh * flip;
-                    gnormal = g * flip;
-                    light_area = 0.5f * length(cr);
-                }
-            }
-            const GpuMaterial m = sc.materials[material];
-            const V3 emissive = ld3(m.emissive);
-            if (front && max3(emissive) > 0.0f) {
-                float w = 1.0f;
-                if (depth > 0) {
-                    if (mode == LUCIDA_SAMPLING_NEE) w = 0.0f;
-                    else if (mode == LUCIDA_SAMPLING_MIS) {
-                        // light_pdf_from_geometry (light.rs:241) * n / total
-                        float pdf_light = 0.0f;
-                        const uint32_t n = sc.num_lights;
-                        if (n != 0 && light_area > 0.0f) {
-                            V3 tl = position - prev_position;
-                            float dist_sq = dot(tl, tl);
-                            if (dist_sq > 0.0f) {
-                                V3 wl = tl / sqrtf(dist_sq);
-                                float cos_l = dot(gnormal, -wl);
-                                if (cos_l > 1.0e-6f) pdf_light = (1.0f / ((float)n * light_area)) * dist_sq / cos_l;
-                            }
-                        }
-                        pdf_light = pdf_light * ((float)n / (float)max(total, 1u));
-                        w = power_heuristic(prev_bsdf_pdf, pdf_light);
-                    }
-                }
-                if (w > 0.0f) radiance = radiance + throughput * emissive * w;
-            }
-            const Surface surf = surface_from_material(m, front, sc.ggx);
-            if (depth == 0) {
-                V3 ga = vmax(surf.diffuse_albedo + surf.f0 + emissive, v3s(1.0e-3f));
-                p.gal[pid] = make_float4(ga.x, ga.y, ga.z, t);
-                p.gnr[pid] = make_float4(normal.x, normal.y, normal.z, 0.0f);
-            }
-            const V3 wo = to_local(-dir, normal);
-            if (wo.z > 0.0f) {
-                if (mode != LUCIDA_SAMPLING_BSDF && depth + 1 < u.max_depth && total > 0) {
-                    // direct_light (integrator.rs:418): the contribution is
-                    // resolved here; CONNECT only answers visibility.
-                    const float u_select = s.next_f32();
-                    float ua_x, ua_y;
-                    s.next_vec2(&ua_x, &ua_y);
-                    const uint32_t n_lights = sc.num_lights;
-                    const uint32_t pick = min((uint32_t)(u_select * (float)total), total - 1u);
-                    V3 wi_world{0, 1, 0}, rin{0, 0, 0};
-                    float distance = 0.0f, pdf = 0.0f;
-                    bool valid = false;
-                    if (pick < n_lights) {
-                        const float u_light = ((float)pick + 0.5f) / (float)n_lights;
-                        const uint32_t li = min((uint32_t)(u_light * (float)n_lights), n_lights - 1u);
-                        const GpuLight L = sc.lights[li];
-                        V3 lo = ld3(L.origin), eu = ld3(L.edge_u), ev = ld3(L.edge_v);
-                        V3 pt;
-                        if (L.kind == LUCIDA_LIGHT_KIND_TRIANGLE) {
-                            float su = sqrtf(ua_x);
-                            pt = lo + (1.0f - su) * eu + (ua_y * su) * ev;
-                        } else {
-                            pt = lo + ua_x * eu + ua_y * ev;
-                        }
-                        V3 tl = pt - position;
-                        float dist_sq = dot(tl, tl);
-                        if (dist_sq > 0.0f) {
-                            float dist = sqrtf(dist_sq);
-                            V3 wl = tl / dist;
-                            float cos_l = dot(ld3(L.normal), -wl);
-                            if (cos_l > 1.0e-6f) {
-                                float lp = (1.0f / ((float)n_lights * L.area)) * dist_sq / cos_l;
-                                if (isfinite(lp) && lp > 0.0f) {
-                                    wi_world = wl;
-                                    rin = ld3(sc.materials[L.material].emissive);
-                                    distance = dist;
-                                    pdf = lp * ((float)n_lights / (float)total);
-                                    valid = true;
-                                }
-                            }
-                        }
-                    } else {
-                        V3 ed, er;
-                        float ep = env_sample(sc.env, ua_x, ua_y, &ed, &er);
-                        if (ep > 0.0f) {
-                            wi_world = ed;
-                            rin = er;
-                            distance = INFINITY;
-                            pdf = ep / (float)total;
-                            valid = true;
-                        }
-                    }
-                    if (valid && pdf > 0.0f) {
-                        V3 wi = to_local(wi_world, normal);
-                        if (wi.z != 0.0f && (wi.z > 0.0f || surf.transmission > 0.0f)) {
-                            V3 f = surface_eval(surf, wo, wi);
-                            if (max3(f) > 0.0f) {
-                                float mis = mode == LUCIDA_SAMPLING_MIS
-                                                ? power_heuristic(pdf, surface_pdf(surf, wo, wi))
-                                                : 1.0f;
-                                if (mis > 0.0f) {
-                                    sr_o = offset_ray_origin(position, wi.z < 0.0f ? -gnormal : gnormal);
-                                    sr_d = wi_world;
-                                    sr_t = distance * (1.0f - 1.0e-3f);
-                                    sr_c = throughput * (f * fabsf(wi.z) * rin * (mis / pdf));
-                                    shadow = true;
-                                    if (key_out) skey = queue_key(sc, BATCH ? (int)j : -1, sr_o, sr_d);
-                                }
-                            }
-                        }
-                    }
-                }
-                const float u_lobe = s.next_f32();
-                float bx, by;
-                s.next_vec2(&bx, &by);
-                V3 wi, weight;
-                const float bpdf = surface_sample(surf, wo, u_lobe, bx, by, &wi, &weight);
-                if (bpdf > 0.0f) {
-                    V3 nt = throughput * weight;
-                    if (max3(nt) > 0.0f && depth + 1 < u.max_depth) {
-                        alive = true;
-                        V3 no = offset_ray_origin(position, wi.z < 0.0f ? -gnormal : gnormal);
-                        V3 nd = to_world(wi, normal);
-                        p.org[pid] = make_float4(no.x, no.y, no.z, __uint_as_float(depth + 1));
-                        p.dir[pid] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(s.state));
-                        p.thr[pid] = make_float4(nt.x, nt.y, nt.z, bpdf);
-                        p.prv[pid] = make_float4(position.x, position.y, position.z, __uint_as_float(s.dim));
-                        if (key_out) key = queue_key(sc, BATCH ? (int)j : -1, no, nd);
-                    }
-                }
-            }
-            p.rad[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+        PathRegs P;
+        P.origin = f4xyz(of);
+        P.dir = f4xyz(df);
+        P.throughput = f4xyz(tf);
+        P.radiance = f4xyz(rf);
+        P.prev_position = f4xyz(pf);
+        P.prev_bsdf_pdf = tf.w;
+        P.depth = __float_as_uint(of.w);
+        P.s.sobol = u.sampler_kind == LUCIDA_SAMPLER_SOBOL;
+        P.s.state = __float_as_uint(df.w);
+        P.s.index = u.sample_offset + local / band.count;
+        P.s.seed = hash_u32(band.first + local % band.count + u.frame_seed * 0x9e3779b9u);
+        P.s.dim = __float_as_uint(pf.w);
+        bool guide;
+        float4 gal, gnr;
+        alive = shade_bounce(sc, u, hit, (sc.num_instances && __float_as_uint(hit.w) != MISS) ? p.hin[i] : 0u, P,
+                             shadow, sr, guide, gal, gnr);
+        if (guide) {
+            p.gal[pid] = gal;
+            p.gnr[pid] = gnr;
+        }
+        p.rad[pid] = make_float4(P.radiance.x, P.radiance.y, P.radiance.z, 0.0f);
+        if (shadow && key_out) skey = queue_key(sc, BATCH ? (int)j : -1, sr.o, sr.d);
+        if (alive) {
+            p.org[pid] = make_float4(P.origin.x, P.origin.y, P.origin.z, __uint_as_float(P.depth));
+            p.dir[pid] = make_float4(P.dir.x, P.dir.y, P.dir.z, __uint_as_float(P.s.state));
+            p.thr[pid] = make_float4(P.throughput.x, P.throughput.y, P.throughput.z, P.prev_bsdf_pdf);
+            p.prv[pid] = make_float4(P.prev_position.x, P.prev_position.y, P.prev_position.z, __uint_as_float(P.s.dim));
+            if (key_out) key = queue_key(sc, BATCH ? (int)j : -1, P.origin, P.dir);
         }
         break;
       }
@@ -1190,9 +1242,9 @@ __global__ __launch_bounds__(SHADE_WG) void k_shade(SceneView sc0, const GpuUnif
     __shared__ uint32_t append_lds[SHADE_WG / 64 + 1];
     const uint32_t sslot = block_append<SHADE_WG / 64>(shadow, shadow_count, append_lds);
     if (shadow) {
-        p.sro[sslot] = make_float4(sr_o.x, sr_o.y, sr_o.z, sr_t);
-        p.srd[sslot] = make_float4(sr_d.x, sr_d.y, sr_d.z, __uint_as_float(pid));
-        p.src[sslot] = make_float4(sr_c.x, sr_c.y, sr_c.z, 0.0f);
+        p.sro[sslot] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.t_max);
+        p.srd[sslot] = make_float4(sr.d.x, sr.d.y, sr.d.z, __uint_as_float(pid));
+        p.src[sslot] = make_float4(sr.c.x, sr.c.y, sr.c.z, 0.0f);
         if (key_out) {
             p.skey[sslot] = skey;
             p.sidx[sslot] = sslot;
@@ -1316,6 +1368,139 @@ __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* _
     }
     hits[i] = out;
     insts[i] = inst;
+}
+
+// ---------------------------------------------------------------------------
+// MEGAKERNEL (shaders/trace/megakernel.wgsl:61-216): the whole path in one
+// thread. A thread owns one pixel of the band and traces its samples one
+// after another, each through every bounce: the closest-hit walk, the
+// bounce's shading (shade_bounce, the wavefront SHADE's own code), the shadow
+// ray's occlusion walk, then the next ray. The samples' sums are added to the
+// film in sample order, as RESOLVE adds a batch, so the film equals the
+// wavefront render's. The comparison architecture: every lane carries its
+// path to the end, so a wave runs as long as its longest path and its lanes
+// diverge through every stage, where the wavefront regroups live paths
+// between stages.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(TRACE_WG) void k_megakernel(SceneView sc, const GpuUniforms* __restrict__ U,
+                                                         GpuAccum* __restrict__ accum, Band band,
+                                                         uint32_t* __restrict__ spill, uint32_t* __restrict__ counters,
+                                                         uint32_t stats_on) {
+    __shared__ uint32_t stack_mem[LDS_STACK * TRACE_WG];
+    const uint32_t px = blockIdx.x * TRACE_WG + threadIdx.x;
+    uint32_t visits = 0, tests = 0, overflow = 0, rays = 0, shadow_rays = 0;
+    if (px < band.count) {
+        const GpuUniforms& u = *U;
+        RayStack st = make_stack(stack_mem, spill, MODE == TRAV_BINARY ? STACK : WIDE_STACK);
+        const uint32_t pixel = band.first + px;
+        V3 sum{0, 0, 0}, sq{0, 0, 0}, al{0, 0, 0}, nr{0, 0, 0};
+        float dp = 0.0f;
+        for (uint32_t j = 0; j < u.samples_per_launch; ++j) {
+            // GENERATE
+            PathRegs P;
+            V3 origin;
+            camera_ray(u, pixel, u.sample_offset + j, &P.s, &origin, &P.dir);
+            P.origin = origin;
+            P.throughput = v3(1.0f, 1.0f, 1.0f);
+            P.radiance = v3(0.0f, 0.0f, 0.0f);
+            P.prev_position = origin;
+            P.prev_bsdf_pdf = 0.0f;
+            P.depth = 0;
+            float4 g_al = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g_nr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (;;) {
+                // EXTEND: analytic primitives by brute force, then the tree
+                ++rays;
+                const V3 o = P.origin, d = P.dir;
+                float best_t = T_MAX;
+                uint32_t best = MISS;
+                for (uint32_t k = 0; k < sc.num_prims; ++k) {
+                    const GpuPrimitive& pr = sc.prims[k];
+                    float t;
+                    const bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD
+                                       ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, best_t, &t)
+                                       : intersect_sphere(pr, o, d, T_MIN, best_t, &t);
+                    if (h) {
+                        best_t = t;
+                        best = k;
+                    }
+                }
+                float t = 0.0f, hu = 0.0f, hv = 0.0f;
+                uint32_t tri = 0, inst = 0;
+                const bool two_level = MODE == TRAV_INSTANCED && sc.num_instances;
+                bool th;
+                if (MODE == TRAV_WIDE)
+                    th = traverse_wide<false>(sc, st, o, d, T_MIN, best_t, &t, &hu, &hv, &tri, &visits, &tests,
+                                              &overflow);
+                else if (two_level)
+                    th = traverse_instances<false>(sc, st, o, d, T_MIN, best_t, &t, &hu, &hv, &tri, &inst, &visits,
+                                                   &tests, &overflow);
+                else
+                    th = traverse<false>(sc, st, 0, o, d, T_MIN, best_t, &t, &hu, &hv, &tri, &visits, &tests,
+                                         &overflow);
+                float4 hit;
+                if (th) hit = make_float4(t, hu, hv, __uint_as_float(tri));
+                else if (best != MISS) hit = make_float4(best_t, 0.0f, 0.0f, __uint_as_float(ANALYTIC | best));
+                else hit = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(MISS));
+                // SHADE
+                bool shadow, guide;
+                ShadowRay sr;
+                float4 gal, gnr;
+                const bool alive = shade_bounce(sc, u, hit, two_level ? inst : 0u, P, shadow, sr, guide, gal, gnr);
+                if (guide) {
+                    g_al = gal;
+                    g_nr = gnr;
+                }
+                // CONNECT
+                if (shadow) {
+                    ++shadow_rays;
+                    bool blocked = false;
+                    for (uint32_t k = 0; k < sc.num_prims && !blocked; ++k) {
+                        const GpuPrimitive& pr = sc.prims[k];
+                        float tt;
+                        blocked = pr.kind == LUCIDA_PRIM_KIND_QUAD
+                                      ? intersect_quad(pr, f4xyz(sc.prim_w[k]), sr.o, sr.d, T_MIN, sr.t_max, &tt)
+                                      : intersect_sphere(pr, sr.o, sr.d, T_MIN, sr.t_max, &tt);
+                    }
+                    if (!blocked) {
+                        float tt, uu, vv;
+                        uint32_t tri2, inst2;
+                        if (MODE == TRAV_WIDE)
+                            blocked = traverse_wide<true>(sc, st, sr.o, sr.d, T_MIN, sr.t_max, &tt, &uu, &vv, &tri2,
+                                                          &visits, &tests, &overflow);
+                        else if (two_level)
+                            blocked = traverse_instances<true>(sc, st, sr.o, sr.d, T_MIN, sr.t_max, &tt, &uu, &vv,
+                                                               &tri2, &inst2, &visits, &tests, &overflow);
+                        else
+                            blocked = traverse<true>(sc, st, 0, sr.o, sr.d, T_MIN, sr.t_max, &tt, &uu, &vv, &tri2,
+                                                     &visits, &tests, &overflow);
+                    }
+                    if (!blocked) P.radiance = v3(P.radiance.x + sr.c.x, P.radiance.y + sr.c.y, P.radiance.z + sr.c.z);
+                }
+                if (!alive) break;
+            }
+            // RESOLVE's per-sample sums, in sample order
+            sum = sum + P.radiance;
+            sq = sq + P.radiance * P.radiance;
+            al = al + f4xyz(g_al);
+            nr = nr + f4xyz(g_nr);
+            dp = dp + g_al.w;
+        }
+        float4* a = reinterpret_cast<float4*>(accum + pixel);
+        float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+        a0.x += sum.x; a0.y += sum.y; a0.z += sum.z; a0.w += (float)u.samples_per_launch;
+        a1.x += al.x; a1.y += al.y; a1.z += al.z; a1.w += dp;
+        a2.x += nr.x; a2.y += nr.y; a2.z += nr.z;
+        a3.x += sq.x; a3.y += sq.y; a3.z += sq.z;
+        a[0] = a0; a[1] = a1; a[2] = a2; a[3] = a3;
+    }
+    // ray counts into bounce 0's slots (the fold sums every bounce's slot)
+    const uint64_t wr = wave_sum_u32(rays), ws = wave_sum_u32(shadow_rays);
+    if ((threadIdx.x & 63) == 0) {
+        if (wr) atomicAdd(&counters[0], (uint32_t)wr);
+        if (ws) atomicAdd(&counters[SHADOW_AT], (uint32_t)ws);
+    }
+    flush_traversal_stats(counters, visits, tests, overflow, stats_on, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1865,6 +2050,90 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     return LUCIDA_OK;
 }
 
+
+lucida_status lucida_render_megakernel(lucida_ctx* ctx, const lucida_scene* scene, const GpuUniforms* uniforms,
+                                       uint32_t samples, lucida_film* film, const lucida_wavefront_config* config,
+                                       lucida_render_stats* stats) {
+    if (!ctx || !scene || !uniforms || !film)
+        return set_error(LUCIDA_INVALID_ARG, "lucida_render_megakernel: null argument");
+    clear_error();
+    if (uniforms->width != film->width || uniforms->height != film->height)
+        return set_error(LUCIDA_INVALID_ARG, "uniforms width/height do not match the film");
+    if (uniforms->max_depth > MAX_BOUNCES) return set_error(LUCIDA_INVALID_ARG, "max_depth above 64");
+    if (scene->num_bvh_nodes >= (1u << 26) || scene->num_triangles + scene->num_primitives + scene->num_instances >= (1u << 26))
+        return set_error(LUCIDA_INVALID_ARG, "scene too large: node and primitive indices are packed in 26 bits");
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    if (samples == 0 || uniforms->max_depth == 0) return LUCIDA_OK;
+    lucida_status st = ensure_ggx_table(ctx);
+    if (st != LUCIDA_OK) return st;
+    lucida_wavefront_config cfg{};
+    if (config) cfg = *config;
+    const size_t pixels = (size_t)film->width * film->height;
+    // The wavefront's batches (about 2^23 paths, samples_per_batch if set):
+    // each batch's per-pixel sums are added to the film as one, so the film
+    // is summed exactly as lucida_render_wavefront sums it.
+    uint32_t spb = cfg.samples_per_batch ? cfg.samples_per_batch
+                                         : (uint32_t)std::max<size_t>(1, ((size_t)1 << 23) / pixels);
+    spb = std::min(spb, samples);
+    const uint32_t batches = (samples + spb - 1) / spb;
+    const SceneView sc = make_view(ctx, scene);
+    const int mode = scene->num_instances ? TRAV_INSTANCED
+                     : (scene->wide && cfg.bvh_width == 4) ? TRAV_WIDE
+                                                           : TRAV_BINARY;
+    hipStream_t s = ctx->stream;
+    LUCIDA_HIP(ctx->mk_spill.ensure((pixels + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->mk_state.ensure(COUNTER_BYTES + (size_t)batches * sizeof(GpuUniforms)), LUCIDA_OOM);
+    uint32_t* counters = ctx->mk_state.as<uint32_t>();
+    GpuUniforms* dU = reinterpret_cast<GpuUniforms*>(ctx->mk_state.as<char>() + COUNTER_BYTES);
+    std::vector<GpuUniforms> host_u(batches, *uniforms);
+    for (uint32_t b = 0; b < batches; ++b) {
+        host_u[b].sample_offset = uniforms->sample_offset + b * spb;
+        host_u[b].samples_per_launch = std::min(spb, samples - b * spb);
+    }
+    LUCIDA_HIP(hipMemcpyAsync(dU, host_u.data(), batches * sizeof(GpuUniforms), hipMemcpyHostToDevice, s),
+               LUCIDA_VALIDATION);
+    const uint32_t stats_on = (stats && cfg.count_traversal) ? 1u : 0u;
+    const Band whole{0u, (uint32_t)pixels};
+    LUCIDA_HIP(hipEventRecord(ctx->ev0, s), LUCIDA_VALIDATION);
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, counters, (uint32_t)TOTALS_AT + 24u);
+    for (uint32_t b = 0; b < batches; ++b) {
+        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, counters, (uint32_t)TOTALS_AT);
+        const dim3 grid(grid_for(pixels, TRACE_WG)), block(TRACE_WG);
+        if (mode == TRAV_INSTANCED)
+            hipLaunchKernelGGL(k_megakernel<TRAV_INSTANCED>, grid, block, 0, s, sc, dU + b, film->accum, whole,
+                               ctx->mk_spill.as<uint32_t>(), counters, stats_on);
+        else if (mode == TRAV_WIDE)
+            hipLaunchKernelGGL(k_megakernel<TRAV_WIDE>, grid, block, 0, s, sc, dU + b, film->accum, whole,
+                               ctx->mk_spill.as<uint32_t>(), counters, stats_on);
+        else
+            hipLaunchKernelGGL(k_megakernel<TRAV_BINARY>, grid, block, 0, s, sc, dU + b, film->accum, whole,
+                               ctx->mk_spill.as<uint32_t>(), counters, stats_on);
+        hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(64), 0, s, counters, 1u);
+        LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    }
+    LUCIDA_HIP(hipEventRecord(ctx->ev1, s), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipEventSynchronize(ctx->ev1), LUCIDA_VALIDATION);
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_VALIDATION);
+    if (stats) {
+        uint64_t t[10];
+        LUCIDA_HIP(hipMemcpy(t, counters + TOTALS_AT, sizeof t, hipMemcpyDeviceToHost), LUCIDA_READBACK);
+        float ms = 0.0f;
+        LUCIDA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1), LUCIDA_VALIDATION);
+        stats->extension_rays = t[0];
+        stats->shadow_rays = t[1];
+        stats->extend_node_visits = t[2];
+        stats->extend_triangle_tests = t[3];
+        stats->node_visits = t[2] + t[4];
+        stats->triangle_tests = t[3] + t[5];
+        stats->stack_overflows = t[6];
+        stats->extend_lane_slots = t[7];
+        stats->connect_lane_slots = t[8];
+        stats->elapsed_ms = ms;
+        stats->kernel_launches[1] = batches;
+        stats->kernel_ms[1] = ms;
+    }
+    return LUCIDA_OK;
+}
 
 lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, const float* rays, size_t n, int flags,
                                 float* hits, uint32_t* instances) {

@@ -70,6 +70,7 @@ def lib() -> C.CDLL:
             "oracle_env_sample": ([U32, U32, P, P, P, F, F, F, P], F),
             "oracle_env_direction_from_uv": ([F, F, P], None),
             "oracle_env_uv_from_direction": ([P, P], None),
+            "oracle_convergence": ([P, S], C.c_double),
         }
         for name, (args, res) in sigs.items():
             try:
@@ -358,3 +359,11 @@ class EnvMap:
         out = np.empty(2, np.float32)
         lib().oracle_env_uv_from_direction(_p(d), _p(out))
         return out
+
+
+def convergence(accum: np.ndarray) -> float:
+    """The convergence meter (shaders/stats/convergence.wgsl) over GpuAccum
+    records (a structured array or a float array of 16 per pixel)."""
+    a = np.ascontiguousarray(accum).view(np.float32).reshape(-1, 16)
+    a = np.ascontiguousarray(a)
+    return lib().oracle_convergence(_p(a), len(a))

@@ -1480,6 +1480,27 @@ void oracle_env_uv_from_direction(const float d[3], float out[2]) {
     out[1] = uv.y;
 }
 
+// convergence.wgsl (the reduce / finish passes of crates/gpu/src/stats.rs)
+// restated per pixel in f32, summed in double: the mean over pixels with
+// N >= 2 and a mean luminance above 1e-3 of sqrt(luminance(var) / N) /
+// luminance(mean), var = radiance_sq / N - mean^2 clamped at zero; -1 when
+// no pixel qualifies. accum holds 16 floats (one GpuAccum) per pixel.
+double oracle_convergence(const float* accum, size_t pixels) {
+    double err = 0.0, cnt = 0.0;
+    for (size_t i = 0; i < pixels; ++i) {
+        const float* a = accum + 16 * i;
+        const float n = a[3];
+        if (!(n >= 2.0f)) continue;
+        const Vec3 mean = Vec3(a[0], a[1], a[2]) / n;
+        const float mean_l = luminance(mean);
+        if (!(mean_l > 1.0e-3f)) continue;
+        const Vec3 var = vmax(Vec3(a[12], a[13], a[14]) / n - mean * mean, Vec3());
+        err += (double)(std::sqrt(luminance(var) / n) / mean_l);
+        cnt += 1.0;
+    }
+    return cnt > 0.0 ? err / cnt : -1.0;
+}
+
 // Directional albedo of a material's full surface BSDF at cos_o, estimated
 // from n one-sample draws of surface::sample (bsdf.rs:1571) with a PCG stream.
 // A white, energy-compensated lobe integrates to 1 (the white furnace).

@@ -69,6 +69,10 @@ float oracle_env_sample(uint32_t w, uint32_t h, const float* pixels, const float
 void oracle_env_direction_from_uv(float u, float v, float out[3]);
 void oracle_env_uv_from_direction(const float d[3], float out[2]);
 
+/* The convergence meter (shaders/stats/convergence.wgsl) over a film of
+ * GpuAccum records (16 floats per pixel). */
+double oracle_convergence(const float* accum, size_t pixels);
+
 /* image::compare (image.rs:178): out = {rmse, max_abs, mean_rel, mean_a,
  * mean_b, mean_abs}. */
 void oracle_compare(const float* a, const float* b, size_t pixels, double* out6);
