@@ -100,33 +100,36 @@ def lbvh_build_rate(L, gpu, blob, runs: int = 7) -> dict:
 
 
 def cpu_baseline(blob, u, target_s: float, runs: int = 7) -> dict:
-    """The CPU oracle (a C++ restatement of the reference tracer) on a band of
-    rows of the same workload, threads = the cores this process may use: one
-    warm-up, then the median of `runs` timed renders of the band
-    (crates/cli/src/bin/bench.rs:192-212), the band sized so all runs take
-    about target_s."""
+    """The CPU oracle (a C++ restatement of the reference tracer) on a sample
+    of rows of the same workload, spread evenly over the image, with one
+    thread per core this process may use (each row a work item, so every
+    thread is busy): one warm-up, then the median of `runs` timed renders of
+    the sample (crates/cli/src/bin/bench.rs:192-212), its size chosen so all
+    runs take about target_s."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
     threads = len(os.sched_getaffinity(0))
     threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
     desc = blob.desc()
-    rows = 2
+    spread = lambda n: ((np.arange(n) + 0.5) * u.height / n).astype(np.uint32)
+    n = threads
     t0 = time.perf_counter()
-    pyoracle.render(desc, u, threads=threads, rows=(0, rows), guides=False)  # warm-up and sizing
+    pyoracle.render_rows(desc, u, spread(n), threads=threads)  # warm-up and sizing
     dt = time.perf_counter() - t0
-    rows = int(max(2, min(u.height, rows * target_s / runs / max(dt, 1e-3))))
-    r0 = (u.height - rows) // 2
+    n = int(max(threads, min(u.height, n * target_s / runs / max(dt, 1e-3))))
+    n = max(threads, n // threads * threads)  # whole rounds of rows per thread
+    rows = spread(n)
     times = []
     for _ in range(runs):
         t0 = time.perf_counter()
-        pyoracle.render(desc, u, threads=threads, rows=(r0, r0 + rows), guides=False)
+        pyoracle.render_rows(desc, u, rows, threads=threads)
         times.append(time.perf_counter() - t0)
     dt = float(np.median(times))
-    paths = rows * u.width * u.samples_per_launch
+    paths = n * u.width * u.samples_per_launch
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"rows {r0}..{r0 + rows} of the {u.width}x{u.height} image at {u.samples_per_launch} spp, "
-                      f"{u.max_depth} bounces ({paths} paths): median of {runs} runs after a warm-up, "
-                      f"{dt:.2f} s each"}
+            "sample": f"{n} rows spread evenly over the {u.width}x{u.height} image (every {u.height / n:.1f}th), "
+                      f"{u.samples_per_launch} spp, {u.max_depth} bounces ({paths} paths), one thread per core: "
+                      f"median of {runs} runs after a warm-up, {dt:.2f} s each"}
 
 
 def other_configs(L, gpu) -> dict:
@@ -159,6 +162,34 @@ def other_configs(L, gpu) -> dict:
                     "extension_rays": st.extension_rays, "shadow_rays": st.shadow_rays,
                     "scene_upload_and_build_ms": round(build_ms, 1), "triangles": counts.get("triangles"),
                     "instances": counts.get("instances")}
+        del film, scene
+    return out
+
+
+def architectures(L, gpu, runs: int = 3) -> dict:
+    """The reference's architecture comparison (README.md:703-714: wavefront
+    speed over megakernel speed at 512^2, 8 spp, depth 8): both renders of
+    each scene timed, median of `runs` after a warm-up. The films are
+    bit-identical (tests/test_megakernel_gpu.py)."""
+    out = {}
+    for name in ("furnace-test", "cornell-box", "cornell-mesh", "bvh-stress"):
+        blob = L.Blob(name)
+        scene = L.Scene(gpu, blob, build_lbvh=True)
+        u = blob.uniforms(512, 512, samples=8, max_depth=8)
+        film = L.Film(gpu, 512, 512)
+        ms = {}
+        for arch in L.ARCHITECTURES:
+            L.render_with(gpu, scene, u, 8, film, arch)
+            gpu.check_errors()
+            t = []
+            for _ in range(runs):
+                film.clear()
+                t0 = time.perf_counter()
+                L.render_with(gpu, scene, u, 8, film, arch)
+                t.append((time.perf_counter() - t0) * 1e3)
+            ms[arch] = float(np.median(t))
+        out[name] = {"megakernel_ms": round(ms["megakernel"], 3), "wavefront_ms": round(ms["wavefront"], 3),
+                     "wavefront_speed_over_megakernel": round(ms["megakernel"] / ms["wavefront"], 3)}
         del film, scene
     return out
 
@@ -539,8 +570,10 @@ def main() -> int:
                        "achieved_gbs": round(atrous_gbs, 1), "frac_of_8tbs": round(atrous_gbs / HBM_PEAK_GBS, 4)},
         },
     }
+    out["extra"]["convergence_meter"] = round(film.convergence(), 6)
     if not a.no_configs:
         out["extra"]["other_configs"] = other_configs(L, gpu)
+        out["extra"]["architectures_512x512_8spp"] = architectures(L, gpu)
     if not a.no_relmse:
         net = None
         if a.net_steps > 0:

@@ -71,6 +71,7 @@ def lib() -> C.CDLL:
             "oracle_env_direction_from_uv": ([F, F, P], None),
             "oracle_env_uv_from_direction": ([P, P], None),
             "oracle_convergence": ([P, S], C.c_double),
+            "oracle_render_rows": ([P, P, P, U32, P, U32, P, P], None),
         }
         for name, (args, res) in sigs.items():
             try:
@@ -220,6 +221,18 @@ def render(desc, uniforms, background=(0.0, 0.0, 0.0), threads: int = 0, rows=No
     counts = RenderCounts()
     lib().oracle_render(C.byref(desc), C.byref(uniforms), _p(bg), threads, r0, r1, _p(rgb), _p(g), C.byref(counts))
     return rgb, g, counts
+
+
+def render_rows(desc, uniforms, rows, background=(0.0, 0.0, 0.0), threads: int = 0):
+    """The render restricted to the listed image rows, one row per work item
+    of the thread pool: (rgb (len(rows), w, 3), counts)."""
+    rows = np.ascontiguousarray(rows, np.uint32)
+    rgb = np.zeros((len(rows), uniforms.width, 3), np.float32)
+    bg = np.asarray(background, np.float32)
+    counts = RenderCounts()
+    lib().oracle_render_rows(C.byref(desc), C.byref(uniforms), _p(bg), threads, _p(rows), len(rows), _p(rgb),
+                             C.byref(counts))
+    return rgb, counts
 
 
 def denoise(film, albedo, normal, depth, params) -> np.ndarray:

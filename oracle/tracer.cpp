@@ -1207,6 +1207,39 @@ void render(const lucida_scene_desc& d, const GpuUniforms& u, const float backgr
     for (auto& t : pool) t.join();
 }
 
+// The rows listed (each row one work item for the threads), written
+// compactly: output row k is image row rows[k]. For a bounded CPU sample of
+// an image that keeps every thread busy on rows spread over the frame.
+void render_rows(const lucida_scene_desc& d, const GpuUniforms& u, const float background[3], uint32_t threads,
+                 const uint32_t* rows, uint32_t n_rows, float* rgb, RenderCounts* counts) {
+    const OScene sc = make_scene(d, background);
+    ggx_table();
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint32_t> next{0};
+    std::mutex mu;
+    auto worker = [&]() {
+        RenderCounts local{};
+        for (;;) {
+            uint32_t k = next.fetch_add(1);
+            if (k >= n_rows) break;
+            const uint32_t y = rows[k];
+            if (y >= u.height) continue;
+            for (uint32_t x = 0; x < u.width; ++x)
+                render_pixel(sc, u, x, y, rgb + 3 * ((size_t)k * u.width + x), nullptr, counts ? &local : nullptr);
+        }
+        if (counts) {
+            std::lock_guard<std::mutex> g(mu);
+            counts->extension_rays += local.extension_rays;
+            counts->shadow_rays += local.shadow_rays;
+            counts->node_visits += local.node_visits;
+            counts->triangle_tests += local.triangle_tests;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < std::min(threads, n_rows); ++t) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+}
+
 // --- denoise.rs ---------------------------------------------------------------------------
 
 void denoise(const float* film, const float* albedo, const float* normal, const float* depth, uint32_t width,
@@ -1309,6 +1342,11 @@ void oracle_offset_ray_origin(const float p[3], const float n[3], float out[3]) 
 void oracle_render(const lucida_scene_desc* d, const GpuUniforms* u, const float background[3], uint32_t threads,
                    uint32_t row_begin, uint32_t row_end, float* rgb, float* guides, RenderCounts* counts) {
     render(*d, *u, background, threads, row_begin, row_end, rgb, guides, counts);
+}
+
+void oracle_render_rows(const lucida_scene_desc* d, const GpuUniforms* u, const float background[3], uint32_t threads,
+                        const uint32_t* rows, uint32_t n_rows, float* rgb, RenderCounts* counts) {
+    oracle::render_rows(*d, *u, background, threads, rows, n_rows, rgb, counts);
 }
 
 void oracle_scene_cast(const lucida_scene_desc* d, const float* rays, size_t n, int any_hit, float* hits,

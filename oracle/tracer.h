@@ -35,6 +35,12 @@ void oracle_offset_ray_origin(const float p[3], const float n[3], float out[3]);
 void oracle_render(const lucida_scene_desc* d, const GpuUniforms* u, const float background[3], uint32_t threads,
                    uint32_t row_begin, uint32_t row_end, float* rgb, float* guides, RenderCounts* counts);
 
+/* The same render over an explicit list of image rows (each row one work
+ * item of the thread pool); rgb receives row k of the output for image row
+ * rows[k] (3 floats per pixel). */
+void oracle_render_rows(const lucida_scene_desc* d, const GpuUniforms* u, const float background[3], uint32_t threads,
+                        const uint32_t* rows, uint32_t n_rows, float* rgb, RenderCounts* counts);
+
 /* One ray per 8 floats (origin.xyz, unused, direction.xyz, t_max) against the
  * scene without shading: hits receive (t, u, v, id bits) per ray, id =
  * triangle, 0x80000000 | analytic primitive, or 0xFFFFFFFF for a miss;

@@ -312,3 +312,14 @@ def test_denoise_zero_iterations_is_identity(oracle):
     alb = (0.2 + rng.random((h, w, 3))).astype(np.float32)
     out = run_denoise(oracle, film, alb, np.tile([0, 0, 1.0], (h, w, 1)), np.full((h, w), 1.0), iterations=0)
     np.testing.assert_allclose(out, film, rtol=1e-6)
+
+
+def test_render_rows_equals_the_banded_render(oracle):
+    """The row-list render (the CPU baseline's sampler) traces the same
+    pixels as the banded render: rows in any order, one per work item."""
+    b = L.Blob("box-c1")
+    u = b.uniforms(48, 40, samples=4, max_depth=6)
+    full, _, _ = oracle.render(b.desc(), u, guides=False)
+    rows = np.array([39, 0, 17, 18, 5], np.uint32)
+    part, _ = oracle.render_rows(b.desc(), u, rows, threads=3)
+    assert np.array_equal(part, full[rows])
