@@ -98,7 +98,7 @@ struct lucida_scene {
     GpuVertexAttr* vertex_attrs = nullptr;
     GpuTriangle* triangles = nullptr;
     float4* tri_edges = nullptr;  // per triangle: p0, p1 - p0, p2 - p0 (the intersection's own operands)
-    float4* prim_w = nullptr;     // per analytic primitive: a quad's w = n / |n|^2 (quad_w)
+    float4* prim_w = nullptr;     // per analytic primitive a quad's w = n / |n|^2 (quad_w), then their box (2)
     void* wide = nullptr;         // 4-wide BVH collapsed from bvh_nodes (128-byte nodes), or null
     size_t num_wide = 0;
     GpuBvhNode* bvh_nodes = nullptr;
@@ -145,7 +145,8 @@ lucida_status sort_pairs_counted(hipStream_t stream, uint32_t* keys, uint32_t* v
 // Builds scene->wide (the 4-wide BVH traversed by default) from its binary
 // BVH; a no-op for instanced scenes and scenes without a BVH.
 lucida_status build_wide_bvh(lucida_ctx* ctx, lucida_scene* scene);
-// Fills scene->prim_w from its analytic primitives (none: a no-op).
+// Fills scene->prim_w (quad planes and the primitives' box) from its analytic
+// primitives (none: a no-op).
 lucida_status build_prim_planes(lucida_ctx* ctx, lucida_scene* scene);
 // Fills scene->tri_edges from its (final, leaf-ordered) triangles.
 lucida_status build_tri_edges(lucida_ctx* ctx, lucida_scene* scene);

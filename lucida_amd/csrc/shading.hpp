@@ -536,6 +536,9 @@ __device__ inline bool intersect_quad(const GpuPrimitive& q, V3 w, V3 o, V3 d, f
     if (fabsf(denom) < 1.0e-8f) return false;
     const float num = dot(normal, origin) - dot(normal, o);
     if (t_min > 0.0f && (num == 0.0f || signbit(num) != signbit(denom))) return false;
+    // num * rcp(denom) is within a few ulp of num / denom: a plane far
+    // beyond t_max needs no correctly rounded division
+    if (num * __builtin_amdgcn_rcpf(denom) > t_max * 1.0001f) return false;
     float t = num / denom;
     if (t < t_min || t > t_max) return false;
     V3 p = o + t * d;

@@ -455,20 +455,22 @@ __device__ inline RayStack make_stack(uint32_t* lds_base, uint32_t* spill_base, 
 // triangles of the tree rooted at `root` (intersect_bvh_from, bvh.wgsl:137:
 // the root's own box is not tested). Returns the hit flag; closest-hit fills
 // t/u/v/tri. Entries the walk pushes sit above the stack's current top.
-// Closest hit (any_hit = false) or occlusion (any_hit = true) against the
-// triangles of the tree rooted at `root` (intersect_bvh_from, bvh.wgsl:137:
-// the root's own box is not tested). Returns the hit flag; closest-hit fills
-// t/u/v/tri. Entries the walk pushes sit above the stack's current top.
 //
-// The walk visits nodes in the reference's order. For closest hit it runs as
-// two loops: inner nodes until the lane stands on a leaf, then the leaf. A
-// wave executes an iteration of a loop while any lane needs it, so
-// separating the two kinds of visit keeps a wave from paying for both in
-// every step (the kernels are bound by vector-instruction issue, not
-// memory). Occlusion rays mostly end at their first hit leaf, where one loop
-// that takes either kind of visit per step measured faster. A tree visits
-// each node at most once per ray; the cap only guarantees termination on a
-// malformed (cyclic) node array.
+// Children are taken nearest first, as the reference takes them, and the
+// walk is bound by how many steps a wave executes over its diverged lanes
+// (vector-instruction issue), not by memory. Closest hit runs as two loops,
+// inner nodes then leaves, and a lane that reaches a leaf postpones it and
+// keeps walking inner nodes while some other lane of its wave is still
+// looking for a leaf (speculative traversal, Aila and Laine, "Understanding
+// the Efficiency of Ray Traversal on GPUs", HPG 2009): the wave's leaf steps
+// then serve most of its lanes at once. Leaves are still intersected in the
+// walk's order, so the hit is the one the reference's walk returns; the
+// inner nodes visited past a postponed leaf are tested against the closest
+// hit before that leaf's triangles, so the walk may visit nodes the
+// reference's walk culls (never fewer). Occlusion rays mostly end at their
+// first hit leaf, where one loop that takes either kind of visit per step
+// measured faster. A tree visits each node at most once per ray; the cap
+// only guarantees termination on a malformed (cyclic) node array.
 template <bool ANY_HIT>
 __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root_index, V3 o, V3 d, float t_min,
                                 float t_max, float* t_out, float* u_out, float* v_out, uint32_t* tri_out,
@@ -534,45 +536,56 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
         st.truncate(base);
         return false;
     }
-    bool more = true;
-    while (more) {
-        // inner nodes, until a leaf
-        while (more && (cur & COUNT_MASK) == 0) {
+    constexpr uint32_t NONE = 0xFFFFFFFFu;  // never a packed child: node indices are below 2^26
+    uint32_t leaf = NONE;                   // the postponed leaf
+    for (;;) {
+        // inner nodes, until every lane that still walks holds a leaf
+        for (;;) {
+            if (cur != NONE && (cur & COUNT_MASK) != 0 && leaf == NONE) {
+                leaf = cur;
+                cur = st.sp == base ? NONE : st.pop();
+            }
+            const bool inner = cur != NONE && (cur & COUNT_MASK) == 0;
+            if (__ballot(inner) == 0 || __ballot(inner && leaf == NONE) == 0) break;
+            if (inner) {
+                *visits += 1;
+                bool ha, hb;
+                float ea, eb;
+                uint32_t ca, cb;
+                test_pair(sc.nodes, cur >> COUNT_BITS, o, inv, t_min, closest, &ha, &hb, &ea, &eb, &ca, &cb);
+                if (ha && hb) {
+                    const bool a_near = ea <= eb;
+                    if (!st.push(a_near ? cb : ca)) *overflow += 1;
+                    cur = a_near ? ca : cb;
+                } else if (ha || hb) {
+                    cur = ha ? ca : cb;
+                } else {
+                    cur = st.sp == base ? NONE : st.pop();
+                }
+                if (++iter >= cap) {
+                    cur = NONE;
+                    leaf = NONE;
+                }
+            }
+        }
+        // the postponed leaves
+        if (leaf != NONE) {
             *visits += 1;
-            bool ha, hb;
-            float ea, eb;
-            uint32_t ca, cb;
-            test_pair(sc.nodes, cur >> COUNT_BITS, o, inv, t_min, closest, &ha, &hb, &ea, &eb, &ca, &cb);
-            if (ha && hb) {
-                const bool a_near = ea <= eb;
-                if (!st.push(a_near ? cb : ca)) *overflow += 1;
-                cur = a_near ? ca : cb;
-            } else if (ha || hb) {
-                cur = ha ? ca : cb;
-            } else if (st.sp == base) {
-                more = false;
-            } else {
-                cur = st.pop();
+            const uint32_t first = leaf >> COUNT_BITS, count = leaf & COUNT_MASK;
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t ti = first + k;
+                *tests += 1;
+                float t, u, v;
+                if (tri_test(sc.edges, ti, o, d, t_min, closest, &t, &u, &v)) {
+                    found = true;
+                    closest = t;
+                    *t_out = t; *u_out = u; *v_out = v; *tri_out = ti;
+                }
             }
-            more = more && ++iter < cap;
+            leaf = NONE;
+            if (++iter >= cap) cur = NONE;
         }
-        if (!more) break;
-        // the leaf
-        *visits += 1;
-        const uint32_t first = cur >> COUNT_BITS, count = cur & COUNT_MASK;
-        for (uint32_t k = 0; k < count; ++k) {
-            const uint32_t ti = first + k;
-            *tests += 1;
-            float t, u, v;
-            if (tri_test(sc.edges, ti, o, d, t_min, closest, &t, &u, &v)) {
-                found = true;
-                closest = t;
-                *t_out = t; *u_out = u; *v_out = v; *tri_out = ti;
-            }
-        }
-        if (st.sp == base) break;
-        cur = st.pop();
-        more = ++iter < cap;
+        if (__ballot(cur != NONE) == 0) break;
     }
     st.truncate(base);
     return found;
@@ -852,6 +865,28 @@ __device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 
     return found;
 }
 
+// The closest of the analytic primitives (scene.wgsl: each tested in turn
+// before the tree; on equal t the later one wins), into best_t / best. Their
+// padded bounding box (k_prim_box, after the primitives' quad_w planes) is
+// tested first: a ray that misses it misses every primitive, so the loop is
+// skipped without changing the result.
+__device__ inline void analytic_closest(const SceneView& sc, V3 o, V3 d, float* best_t, uint32_t* best) {
+    if (sc.num_prims == 0) return;
+    const V3 inv = safe_inv_dir(d);
+    float entry;
+    if (!slab(sc.prim_w[sc.num_prims], sc.prim_w[sc.num_prims + 1], o, inv, T_MIN, *best_t, &entry)) return;
+    for (uint32_t k = 0; k < sc.num_prims; ++k) {
+        const GpuPrimitive& pr = sc.prims[k];
+        float t;
+        const bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, *best_t, &t)
+                                                        : intersect_sphere(pr, o, d, T_MIN, *best_t, &t);
+        if (h) {
+            *best_t = t;
+            *best = k;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // EXTEND (extend.wgsl:124, scene.rs:452): analytic primitives by brute force,
 // then the BVH with t_max = the best analytic hit. Writes (t, u, v, id):
@@ -879,16 +914,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJ
         const V3 o = f4xyz(of), d = f4xyz(df);
         float best_t = T_MAX;
         uint32_t best = MISS;
-        for (uint32_t k = 0; k < sc.num_prims; ++k) {
-            const GpuPrimitive& pr = sc.prims[k];
-            float t;
-            bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, best_t, &t)
-                                                      : intersect_sphere(pr, o, d, T_MIN, best_t, &t);
-            if (h) {
-                best_t = t;
-                best = k;
-            }
-        }
+        analytic_closest(sc, o, d, &best_t, &best);
         float t = 0.0f, u = 0.0f, v = 0.0f;
         uint32_t tri = 0, inst = 0;
         // Instanced scenes take the two-level path instead of the
@@ -1414,17 +1440,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_megakernel(SceneView sc, const Gpu
                 const V3 o = P.origin, d = P.dir;
                 float best_t = T_MAX;
                 uint32_t best = MISS;
-                for (uint32_t k = 0; k < sc.num_prims; ++k) {
-                    const GpuPrimitive& pr = sc.prims[k];
-                    float t;
-                    const bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD
-                                       ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, best_t, &t)
-                                       : intersect_sphere(pr, o, d, T_MIN, best_t, &t);
-                    if (h) {
-                        best_t = t;
-                        best = k;
-                    }
-                }
+                analytic_closest(sc, o, d, &best_t, &best);
                 float t = 0.0f, hu = 0.0f, hv = 0.0f;
                 uint32_t tri = 0, inst = 0;
                 const bool two_level = MODE == TRAV_INSTANCED && sc.num_instances;
@@ -1625,6 +1641,54 @@ __global__ void k_prim_planes(const GpuPrimitive* __restrict__ prims, size_t n, 
     w[i] = make_float4(q.x, q.y, q.z, 0.0f);
 }
 
+// The box around every analytic primitive (a quad by its four corners, a
+// sphere by center +- radius) at w[n] (low) and w[n + 1] (high), padded by
+// 1e-3 of its largest coordinate magnitude (plus 1e-6): far beyond the
+// rounding of the primitive tests and of the slab test, so a ray that any
+// primitive test accepts always hits the box (a flat set of quads gets a
+// thickness).
+__global__ __launch_bounds__(256) void k_prim_box(const GpuPrimitive* __restrict__ prims, size_t n,
+                                                  float4* __restrict__ w) {
+    __shared__ float red[6][256];
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (size_t i = threadIdx.x; i < n; i += 256) {
+        const GpuPrimitive& q = prims[i];
+        for (int a = 0; a < 3; ++a) {
+            float l, h;
+            if (q.kind == LUCIDA_PRIM_KIND_QUAD) {
+                const float p = q.position[a], u = q.edge_u[a], v = q.edge_v[a];
+                l = fminf(fminf(p, p + u), fminf(p + v, p + u + v));
+                h = fmaxf(fmaxf(p, p + u), fmaxf(p + v, p + u + v));
+            } else {
+                l = q.position[a] - fabsf(q.radius);
+                h = q.position[a] + fabsf(q.radius);
+            }
+            lo[a] = fminf(lo[a], l);
+            hi[a] = fmaxf(hi[a], h);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        red[a][threadIdx.x] = lo[a];
+        red[3 + a][threadIdx.x] = hi[a];
+    }
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1) {
+        if ((int)threadIdx.x < half)
+            for (int a = 0; a < 3; ++a) {
+                red[a][threadIdx.x] = fminf(red[a][threadIdx.x], red[a][threadIdx.x + half]);
+                red[3 + a][threadIdx.x] = fmaxf(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + half]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int a = 0; a < 6; ++a) m = fmaxf(m, fabsf(red[a][0]));
+        const float pad = 1.0e-3f * m + 1.0e-6f;
+        w[n] = make_float4(red[0][0] - pad, red[1][0] - pad, red[2][0] - pad, 0.0f);
+        w[n + 1] = make_float4(red[3][0] + pad, red[4][0] + pad, red[5][0] + pad, 0.0f);
+    }
+}
+
 // Per triangle (in the scene's leaf order): p0, p1 - p0, p2 - p0.
 __global__ void k_tri_edges(const GpuTriangle* __restrict__ tris, const float4* __restrict__ pos, size_t n,
                             float4* __restrict__ edges) {
@@ -1644,9 +1708,11 @@ __global__ void k_tri_edges(const GpuTriangle* __restrict__ tris, const float4* 
 lucida_status build_prim_planes(lucida_ctx* ctx, lucida_scene* scene) {
     const size_t n = scene->num_primitives;
     if (n == 0) return LUCIDA_OK;
-    if (!scene->prim_w) LUCIDA_HIP(hipMalloc(&scene->prim_w, n * sizeof(float4)), LUCIDA_OOM);
+    // per primitive its quad_w, then the primitives' box (k_prim_box)
+    if (!scene->prim_w) LUCIDA_HIP(hipMalloc(&scene->prim_w, (n + 2) * sizeof(float4)), LUCIDA_OOM);
     hipLaunchKernelGGL(k_prim_planes, dim3(grid_for(n, 64)), dim3(64), 0, ctx->stream, scene->primitives, n,
                        scene->prim_w);
+    hipLaunchKernelGGL(k_prim_box, dim3(1), dim3(256), 0, ctx->stream, scene->primitives, n, scene->prim_w);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
     return LUCIDA_OK;
 }

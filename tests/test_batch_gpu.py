@@ -97,8 +97,11 @@ def test_batch_ray_order_does_not_change_films(gpu, scenes):
     (kept, sk), (sorted_, ss) = out
     for k, (a, b) in enumerate(zip(kept, sorted_)):
         np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=f"job {k} {JOBS[k]}")
-    for f in ("extension_rays", "shadow_rays", "node_visits", "triangle_tests"):
+    for f in ("extension_rays", "shadow_rays"):
         assert getattr(sk, f) == getattr(ss, f), f
+    # speculative visits (past a postponed leaf) depend on which rays share a wave
+    for f in ("node_visits", "triangle_tests"):
+        assert abs(getattr(sk, f) - getattr(ss, f)) <= 0.05 * getattr(ss, f), f
 
 
 def test_batch_accumulates_like_progressive_renders(gpu, scenes):

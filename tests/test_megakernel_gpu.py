@@ -47,7 +47,10 @@ def test_megakernel_film_equals_wavefront_film(gpu, oracle, name, w, h, spp, dep
     a, b = fa.download(), fb.download()
     np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
     assert (sa.extension_rays, sa.shadow_rays) == (sb.extension_rays, sb.shadow_rays)
-    assert (sa.node_visits, sa.triangle_tests) == (sb.node_visits, sb.triangle_tests)
+    # the walks find the same hits, but a lane's speculative visits (past a
+    # postponed leaf) depend on which rays share its wave
+    for f in ("node_visits", "triangle_tests"):
+        assert abs(getattr(sa, f) - getattr(sb, f)) <= 0.05 * getattr(sb, f), f
     assert sa.stack_overflows == 0
     assert fa.convergence() == fb.convergence()
     d, keep = scene.host_desc(blob)

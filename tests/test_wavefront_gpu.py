@@ -139,9 +139,14 @@ def test_config2_geometry_matches_oracle(gpu, oracle):
     # flip at silhouettes
     assert abs(st.extension_rays - counts.extension_rays) <= 2e-3 * counts.extension_rays
     assert abs(st.shadow_rays - counts.shadow_rays) <= 2e-3 * counts.shadow_rays
-    # the oracle counts closest-hit traversal only (Scene::intersect_counting)
-    assert abs(st.extend_node_visits - counts.node_visits) <= 5e-3 * counts.node_visits
-    assert abs(st.extend_triangle_tests - counts.triangle_tests) <= 5e-3 * counts.triangle_tests
+    # the oracle counts closest-hit traversal only (Scene::intersect_counting);
+    # the GPU walk visits every node the reference's walk visits and, past a
+    # postponed leaf, some that the walk would cull: at least the oracle's
+    # counts (less the paths that flip at silhouettes), a few percent above
+    say(f"closest-hit visits / oracle {st.extend_node_visits / counts.node_visits:.4f}, "
+        f"tests / oracle {st.extend_triangle_tests / counts.triangle_tests:.4f}")
+    assert (1 - 5e-3) * counts.node_visits <= st.extend_node_visits <= 1.05 * counts.node_visits
+    assert (1 - 5e-3) * counts.triangle_tests <= st.extend_triangle_tests <= 1.05 * counts.triangle_tests
 
 
 def test_progressive_equals_one_shot(gpu, oracle):
@@ -173,7 +178,9 @@ def test_batch_size_does_not_change_the_image(gpu, oracle):
 @pytest.mark.parametrize("name,width", [("sphere-c2", 0), ("sphere-c2", 4), ("instances-c3", 0)])
 def test_ray_order_does_not_change_the_image(gpu, oracle, name, width):
     """Sorting the extension queues only regroups rays into waves: every path
-    computes the same values, so films and counters are bit-identical."""
+    computes the same values, so films and ray counts are bit-identical (node
+    visits and triangle tests differ a little: a lane's speculative visits
+    past a postponed leaf depend on which rays share its wave)."""
     if name.startswith("instances"):
         blob = L.Blob(name)
         scene = L.Scene(gpu, blob)
@@ -187,8 +194,10 @@ def test_ray_order_does_not_change_the_image(gpu, oracle, name, width):
         runs.append(gpu_render(gpu, scene, u, 6, cfg))
     (a, sa), (b, sb) = runs
     np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
-    for f in ("extension_rays", "shadow_rays", "node_visits", "triangle_tests", "stack_overflows"):
+    for f in ("extension_rays", "shadow_rays", "stack_overflows"):
         assert getattr(sa, f) == getattr(sb, f), f
+    for f in ("node_visits", "triangle_tests"):
+        assert abs(getattr(sa, f) - getattr(sb, f)) <= 0.05 * getattr(sb, f), f
     assert sa.queue_sorts == 0 and sb.queue_sorts == 8  # one batch of 6 spp: the queues are sorted after every SHADE
     eff = lambda st: (st.extend_node_visits / st.extend_lane_slots,
                       (st.node_visits - st.extend_node_visits) / st.connect_lane_slots)
