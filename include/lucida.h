@@ -288,10 +288,13 @@ lucida_status lucida_copy_device(lucida_ctx* ctx, void* dst, const void* src, si
 /* LBVH (crates/gpu/src/lbvh.rs)                                            */
 /* ------------------------------------------------------------------------ */
 
-/* Chooses how the LBVH fit orders its bottom-up unions: 0 (default) one launch
- * per tree level, deepest first; 1 the atomic bottom-up walk (the second
- * arrival at each node continues), which needs agent-scope acquire/release
- * between XCDs. Both produce identical boxes (crates/core/src/lbvh.rs:490-525). */
+/* Chooses how the LBVH fit orders its bottom-up unions: 0 (default) two
+ * phases, the subtrees inside each block of 1024 sorted primitives by one
+ * workgroup each (the atomic walk with its state in LDS), then the nodes that
+ * span blocks by one workgroup; 1 the atomic bottom-up walk over the whole
+ * tree (the second arrival at each node continues), which needs agent-scope
+ * acquire/release between XCDs; 2 one launch per tree level, deepest first.
+ * All produce identical boxes (crates/core/src/lbvh.rs:490-525). */
 lucida_status lucida_ctx_set_lbvh_fit(lucida_ctx* ctx, int mode);
 
 /* LbvhBuilder::sort (crates/gpu/src/lbvh.rs:669): stable ascending sort of u32

@@ -384,11 +384,13 @@ class Gpu:
 
 class LbvhBuilder:
     """LbvhBuilder (crates/gpu/src/lbvh.rs:153-758). `fit` picks the fit's
-    ordering: "levels" (one launch per level) or "atomic" (bottom-up walk)."""
+    ordering: "blocks" (block-local subtrees, then the nodes spanning blocks),
+    "atomic" (bottom-up walk over the whole tree) or "levels" (one launch per
+    level)."""
 
-    FITS = {"levels": 0, "atomic": 1}
+    FITS = {"blocks": 0, "atomic": 1, "levels": 2}
 
-    def __init__(self, gpu: Gpu, fit: str = "levels"):
+    def __init__(self, gpu: Gpu, fit: str = "blocks"):
         self.gpu = gpu
         _check(lib().lucida_ctx_set_lbvh_fit(gpu._h, self.FITS[fit]))
 

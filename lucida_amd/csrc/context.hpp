@@ -77,7 +77,7 @@ struct lucida_ctx {
     // LBVH scratch, reused between builds.
     lucida::DevBuf lb_bounds, lb_partial, lb_codes, lb_vals, lb_tmpk, lb_tmpv, lb_hist, lb_karras, lb_parent,
         lb_nodebox, lb_subtree, lb_flags, lb_misc, lb_rank, lb_nodes, lb_tris, lb_pos, lb_prims, lb_level;
-    int lbvh_fit = 0;  // 0: level-synchronous fit, 1: atomic bottom-up fit
+    int lbvh_fit = 0;  // 0: two-phase fit (block-local, then the top), 1: atomic bottom-up, 2: level-synchronous
     // GGX directional-albedo table (energy compensation), 32 x 32 floats.
     lucida::DevBuf ggx_table;
     bool ggx_ready = false;

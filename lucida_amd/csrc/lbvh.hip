@@ -107,8 +107,15 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prepare(const GpuTriangle* __res
 // Folds the per-block centroid bounds into the scene's (one block). A launch
 // of its own: a last-block-done ticket would cost every block of k_prepare a
 // device-scope fence and an atomic on one address.
+// Also clears what the later stages count or mark, saving them launches:
+// the root's parent (`root_parent`, when given) and the two-phase fit's list
+// counters (`fit_counts`, eight words, when given).
 __global__ __launch_bounds__(PREP_BLOCK) void k_fold_bounds(const float* __restrict__ partial, uint32_t blocks,
-                                                            LucidaAabb* __restrict__ scene_box) {
+                                                            LucidaAabb* __restrict__ scene_box,
+                                                            uint32_t* __restrict__ root_parent,
+                                                            uint32_t* __restrict__ fit_counts) {
+    if (root_parent && threadIdx.x == 0) *root_parent = 0xFFFFFFFFu;
+    if (fit_counts && threadIdx.x < 8) fit_counts[threadIdx.x] = 0;
     __shared__ float red[PREP_BLOCK / 64][6];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
@@ -404,9 +411,25 @@ __device__ inline int lcp(const uint32_t* __restrict__ codes, int n, int i, int 
     return __clz(a ^ b);
 }
 
+// With `lists` (the two-phase fit), each node is also classified for it: a
+// node whose subtree holds more than TREELET leaves is a "top" node (listed,
+// with a slot), and each child of a top node with at most TREELET leaves is a
+// treelet root (listed); a whole tree of at most TREELET leaves is one
+// treelet. The children's subtree sizes follow from the split. counts[0] =
+// treelets, counts[1] = top nodes (zeroed before the launch).
+constexpr uint32_t TREELET = 1024;
+constexpr uint32_t TOP_LDS = 3072;  // top slots held in LDS (48 bytes each)
+
+struct FitLists {
+    uint32_t* counts;
+    uint32_t* treelets;
+    uint32_t* top;
+    uint32_t* slot_of;
+};
+
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ codes, uint32_t n,
                                                 uint2* __restrict__ karras, uint2* __restrict__ ranges,
-                                                uint32_t* __restrict__ parent) {
+                                                uint32_t* __restrict__ parent, FitLists lists) {
     const uint32_t gi = blockIdx.x * 256 + threadIdx.x;
     if (n < 2 || gi >= n - 1) return;
     const int N = (int)n, i = (int)gi;
@@ -434,6 +457,22 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     ranges[gi] = make_uint2((uint32_t)first, (uint32_t)last);
     parent[left] = gi;
     parent[right] = gi;
+    if (lists.counts) {
+        const uint32_t size = (uint32_t)(last - first + 1);
+        if (size <= TREELET) {
+            if (gi == 0) lists.treelets[atomicAdd(&lists.counts[0], 1u)] = 0;  // the whole tree is one treelet
+            return;
+        }
+        // one atomic for both lists: the top count in the high word, treelets in the low
+        const bool tl = (uint32_t)(split - first + 1) <= TREELET, tr = (uint32_t)(last - split) <= TREELET;
+        const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(lists.counts),
+                                                 (1ull << 32) | (unsigned long long)((tl ? 1u : 0u) + (tr ? 1u : 0u)));
+        const uint32_t slot = (uint32_t)(old >> 32), t0 = (uint32_t)old;
+        lists.top[slot] = gi;
+        lists.slot_of[gi] = slot;
+        if (tl) lists.treelets[t0] = left;
+        if (tr) lists.treelets[t0 + (tl ? 1u : 0u)] = right;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -586,6 +625,228 @@ __global__ __launch_bounds__(256) void k_fit_level(const uint32_t* __restrict__ 
     box_store(&node_box[node], lo, hi);
     subtree[node] = size;
     inner[node] = size > MAX_LEAF ? 1u + il + ir : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Two-phase fit: the same unions again, nearly all of them inside one
+// workgroup with their state in LDS. A "treelet" is a subtree of at most
+// TREELET leaves whose parent's subtree is larger; its leaves are
+// consecutive sorted primitives (a Karras node's range is its subtree's
+// leaves) and so are the indices of its internal nodes (a node is one end of
+// its range), so one workgroup fits a treelet by the atomic bottom-up walk
+// with boxes and arrival counts in LDS indexed from the treelet's first leaf.
+// The nodes above the treelets ("top" nodes, subtrees of more than TREELET
+// leaves: about 2n / TREELET) are fitted afterwards by one workgroup that
+// keeps their topology and arrival counts in LDS (device memory past
+// TOP_LDS of them) and climbs from the top nodes whose children are both
+// treelet roots. Subtree sizes come from the ranges, so both lists are known
+// before any box is.
+// ---------------------------------------------------------------------------
+
+__device__ inline uint32_t range_size(const uint2* __restrict__ ranges, uint32_t leaf_base, uint32_t node) {
+    return node >= leaf_base ? 1u : ranges[node].y - ranges[node].x + 1u;
+}
+
+// One treelet per iteration of the block's loop (the grid strides over the
+// treelet list). The treelet's topology is read once, coalesced, into LDS:
+// for the node at index f + k (k < m) its parent's and children's slots
+// (children tagged as leaves by LEAF_TAG); slot k of the leaf arrays is leaf
+// f + k. Then each leaf's thread climbs with LDS reads only, arrival bits
+// flipped by atomic XOR: the second arrival at a node sees its bit set.
+constexpr uint32_t LEAF_TAG = 0x8000u;
+
+__global__ __launch_bounds__(TREELET) void k_fit_treelets(const LucidaAabb* __restrict__ tri_bounds,
+                                                          const uint32_t* __restrict__ order, uint32_t n,
+                                                          const uint2* __restrict__ karras,
+                                                          const uint2* __restrict__ ranges,
+                                                          const uint32_t* __restrict__ parent,
+                                                          const uint32_t* __restrict__ treelets,
+                                                          const uint32_t* __restrict__ counts,
+                                                          LucidaAabb* __restrict__ node_box,
+                                                          uint32_t* __restrict__ subtree,
+                                                          uint32_t* __restrict__ inner) {
+    __shared__ float leaf_lo[TREELET][3], leaf_hi[TREELET][3];
+    __shared__ float node_lo[TREELET][3], node_hi[TREELET][3];
+    __shared__ uint32_t size_emit[TREELET];  // subtree size (low 16 bits), emitted internal count (high 16)
+    __shared__ uint32_t kids[TREELET];       // child slots, low 16 bits left, high 16 right
+    __shared__ uint32_t up[TREELET];         // parent slot of internal node f + k
+    __shared__ uint32_t arrived[TREELET / 32];
+    const uint32_t leaf_base = n - 1;
+    const uint32_t count = counts[0];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t j = blockIdx.x; j < count; j += gridDim.x) {
+        const uint32_t root = treelets[j];
+        uint32_t f, m;
+        if (root >= leaf_base) {
+            f = root - leaf_base;
+            m = 1;
+        } else {
+            f = ranges[root].x;
+            m = ranges[root].y - f + 1u;
+        }
+        if (t < TREELET / 32) arrived[t] = 0;
+        float lo[3], hi[3];
+        uint32_t leaf_up = 0;
+        if (t < m) {
+            const LucidaAabb b = tri_bounds[order[f + t]];
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = b.lo[a];
+                hi[a] = b.hi[a];
+                leaf_lo[t][a] = lo[a];
+                leaf_hi[t][a] = hi[a];
+            }
+            node_box[leaf_base + f + t] = b;
+            leaf_up = parent[leaf_base + f + t] - f;
+            const uint32_t i = f + t;  // an internal index inside the range (one of them may lie outside the treelet: never visited)
+            if (i < leaf_base) {
+                const uint2 c = karras[i];
+                const uint32_t l = c.x >= leaf_base ? ((c.x - leaf_base - f) | LEAF_TAG) : c.x - f;
+                const uint32_t r = c.y >= leaf_base ? ((c.y - leaf_base - f) | LEAF_TAG) : c.y - f;
+                kids[t] = (l & 0xFFFFu) | (r << 16);
+                up[t] = parent[i] - f;
+            }
+        }
+        __syncthreads();
+        if (t < m && root < leaf_base) {
+            uint32_t slot = leaf_up;  // the node being arrived at
+            for (;;) {
+                // this child's LDS entries are written: release them to the sibling's thread
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                const uint32_t bit = 1u << (slot & 31u);
+                if ((atomicXor(&arrived[slot >> 5], bit) & bit) == 0u) break;  // first arrival: the sibling goes on
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                const uint32_t k2 = kids[slot];
+                const uint32_t ch[2] = {k2 & 0xFFFFu, k2 >> 16};
+                float clo[2][3], chi[2][3];
+                uint32_t csize[2], cemit[2];
+                for (int q = 0; q < 2; ++q) {
+                    const bool leaf = (ch[q] & LEAF_TAG) != 0u;
+                    const uint32_t cs = ch[q] & ~LEAF_TAG;
+                    for (int a = 0; a < 3; ++a) {
+                        clo[q][a] = leaf ? leaf_lo[cs][a] : node_lo[cs][a];
+                        chi[q][a] = leaf ? leaf_hi[cs][a] : node_hi[cs][a];
+                    }
+                    csize[q] = leaf ? 1u : size_emit[cs] & 0xFFFFu;
+                    cemit[q] = leaf ? 0u : size_emit[cs] >> 16;
+                }
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = fminf(clo[0][a], clo[1][a]);
+                    hi[a] = fmaxf(chi[0][a], chi[1][a]);
+                }
+                const uint32_t size = csize[0] + csize[1];
+                const uint32_t emit = size > MAX_LEAF ? 1u + cemit[0] + cemit[1] : 0u;
+                for (int a = 0; a < 3; ++a) {
+                    node_lo[slot][a] = lo[a];
+                    node_hi[slot][a] = hi[a];
+                }
+                size_emit[slot] = size | (emit << 16);
+                const uint32_t node = f + slot;
+                box_store(&node_box[node], lo, hi);
+                subtree[node] = size;
+                inner[node] = emit;
+                if (node == root) break;
+                slot = up[slot];
+            }
+        }
+        __syncthreads();  // LDS is reused by the next treelet
+    }
+}
+
+// The top nodes, by one workgroup, with their state in LDS (device memory
+// past TOP_LDS slots; gfx950 gives a workgroup up to 160 KB of LDS). Per
+// top slot: a record (box, subtree size, emitted-internal count), the slots
+// of its top children (NONE for a child that is a treelet root, final since
+// k_fit_treelets), its parent's slot and an arrival count that starts at the
+// number of treelet-root children (also kept in the second byte, which marks
+// the nodes ready at the start); the count reaching 2 makes the node ready.
+// A node with one treelet-root child holds that child's record until it is
+// fitted; one with two is fitted while the state is read in. Unions take the
+// left child first, as every other fit does. The climbs read and write LDS
+// only, ordered by workgroup-scope release / acquire; the records go to the
+// node arrays at the end.
+struct TopRec {
+    float lo[3], hi[3];
+    uint32_t size, emit;
+};
+
+__device__ inline TopRec top_union(const TopRec& l, const TopRec& r) {
+    TopRec u;
+    for (int a = 0; a < 3; ++a) {
+        u.lo[a] = fminf(l.lo[a], r.lo[a]);
+        u.hi[a] = fmaxf(l.hi[a], r.hi[a]);
+    }
+    u.size = l.size + r.size;
+    u.emit = u.size > MAX_LEAF ? 1u + l.emit + r.emit : 0u;
+    return u;
+}
+
+__global__ __launch_bounds__(1024) void k_fit_top(uint32_t n, const uint2* __restrict__ karras,
+                                                  const uint2* __restrict__ ranges,
+                                                  const uint32_t* __restrict__ parent,
+                                                  const uint32_t* __restrict__ top,
+                                                  const uint32_t* __restrict__ slot_of,
+                                                  const uint32_t* __restrict__ counts, uint32_t* __restrict__ spill,
+                                                  LucidaAabb* node_box, uint32_t* subtree, uint32_t* inner) {
+    __shared__ TopRec rec_s[TOP_LDS];
+    __shared__ uint32_t kid0_s[TOP_LDS], kid1_s[TOP_LDS], up_s[TOP_LDS], arr_s[TOP_LDS];
+    const uint32_t leaf_base = n - 1;
+    const uint32_t T = counts[1];
+    TopRec* rec_g = reinterpret_cast<TopRec*>(spill);
+    uint32_t* kid0_g = spill + 8 * (size_t)T;
+    uint32_t* kid1_g = kid0_g + T;
+    uint32_t* up_g = kid1_g + T;
+    uint32_t* arr_g = up_g + T;
+    auto rec = [&](uint32_t s) -> TopRec& { return s < TOP_LDS ? rec_s[s] : rec_g[s]; };
+    auto at = [](uint32_t* lds, uint32_t* g, uint32_t s) -> uint32_t* { return s < TOP_LDS ? lds + s : g + s; };
+    // the record of a child that is a treelet root (or a leaf)
+    auto final_rec = [&](uint32_t x) {
+        TopRec r;
+        box_load(&node_box[x], r.lo, r.hi);
+        r.size = x >= leaf_base ? 1u : subtree[x];
+        r.emit = x >= leaf_base ? 0u : inner[x];
+        return r;
+    };
+    for (uint32_t s = threadIdx.x; s < T; s += blockDim.x) {
+        const uint32_t node = top[s];
+        const uint2 c = karras[node];
+        const uint32_t k0 = range_size(ranges, leaf_base, c.x) > TREELET ? slot_of[c.x] : NONE;
+        const uint32_t k1 = range_size(ranges, leaf_base, c.y) > TREELET ? slot_of[c.y] : NONE;
+        if (k0 == NONE && k1 == NONE) rec(s) = top_union(final_rec(c.x), final_rec(c.y));
+        else if (k0 == NONE) rec(s) = final_rec(c.x);
+        else if (k1 == NONE) rec(s) = final_rec(c.y);
+        const uint32_t ready = (k0 == NONE ? 1u : 0u) + (k1 == NONE ? 1u : 0u);
+        *at(kid0_s, kid0_g, s) = k0;
+        *at(kid1_s, kid1_g, s) = k1;
+        *at(up_s, up_g, s) = node == 0 ? NONE : slot_of[parent[node]];
+        *at(arr_s, arr_g, s) = ready | (ready << 8);  // arrivals (low byte) and the count at the start
+    }
+    __syncthreads();
+    for (uint32_t s0 = threadIdx.x; s0 < T; s0 += blockDim.x) {
+        if ((*at(arr_s, arr_g, s0) >> 8) != 2u) continue;  // not ready at the start: a top child climbs into it
+        uint32_t s = s0;  // fitted
+        for (;;) {
+            const uint32_t ps = *at(up_s, up_g, s);
+            if (ps == NONE) break;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if ((atomicAdd(at(arr_s, arr_g, ps), 1u) & 0xFFu) != 1u) break;  // the sibling finishes the parent
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            s = ps;
+            const uint32_t k0 = *at(kid0_s, kid0_g, s), k1 = *at(kid1_s, kid1_g, s);
+            const TopRec own = rec(s);  // a treelet-root child's record, if it has one
+            TopRec l = own, r = own;
+            if (k0 != NONE) l = rec(k0);
+            if (k1 != NONE) r = rec(k1);
+            rec(s) = top_union(l, r);
+        }
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < T; s += blockDim.x) {
+        const uint32_t node = top[s];
+        const TopRec& r = rec(s);
+        box_store(&node_box[node], r.lo, r.hi);
+        subtree[node] = r.size;
+        inner[node] = r.emit;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -854,6 +1115,38 @@ static lucida_status fit_levels(lucida_ctx* ctx, size_t n) {
     return LUCIDA_OK;
 }
 
+// The lists of the two-phase fit, in lb_level (k_karras fills them).
+static FitLists fit_lists(lucida_ctx* ctx, size_t n) {
+    const size_t internal = n - 1;
+    FitLists l;
+    l.counts = ctx->lb_level.as<uint32_t>();
+    l.treelets = l.counts + 8;
+    l.top = l.treelets + internal + 1;
+    l.slot_of = l.top + internal;
+    return l;
+}
+
+// Two-phase fit: k_fit_treelets fits the treelets k_karras listed,
+// k_fit_top the nodes above them.
+static lucida_status fit_treelets(lucida_ctx* ctx, size_t n) {
+    hipStream_t s = ctx->stream;
+    const size_t internal = n - 1;
+    const FitLists l = fit_lists(ctx, n);
+    uint32_t* spill = l.slot_of + internal;  // 12 words per top slot
+    uint32_t* subtree = ctx->lb_subtree.as<uint32_t>();
+    // at most 2n / TREELET + 1 treelets in a balanced tree; blocks stride over the rest
+    const uint32_t grid = std::min<uint32_t>(blocks_for(2 * n, TREELET) + 1, 4096);
+    hipLaunchKernelGGL(k_fit_treelets, dim3(grid), dim3(TREELET), 0, s, ctx->lb_bounds.as<LucidaAabb>(),
+                       ctx->lb_vals.as<uint32_t>(), (uint32_t)n, ctx->lb_karras.as<uint2>(),
+                       ctx->lb_rank.as<uint2>(), ctx->lb_parent.as<uint32_t>(), l.treelets, l.counts,
+                       ctx->lb_nodebox.as<LucidaAabb>(), subtree, subtree + internal);
+    hipLaunchKernelGGL(k_fit_top, dim3(1), dim3(1024), 0, s, (uint32_t)n, ctx->lb_karras.as<uint2>(),
+                       ctx->lb_rank.as<uint2>(), ctx->lb_parent.as<uint32_t>(), l.top, l.slot_of, l.counts, spill,
+                       ctx->lb_nodebox.as<LucidaAabb>(), subtree, subtree + internal);
+    LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    return LUCIDA_OK;
+}
+
 // The device stages of the build, up to and including the fit. Results stay in
 // the context's scratch (lb_codes = sorted codes, lb_vals = order, ...).
 static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
@@ -886,8 +1179,12 @@ static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size
 
     hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(PREP_BLOCK), 0, s, tris, reinterpret_cast<const float4*>(positions),
                        (uint32_t)n, ctx->lb_bounds.as<LucidaAabb>(), ctx->lb_partial.as<float>());
+    // the two-phase fit's lists (k_karras fills them)
+    const bool lists_on = n >= 2 && ctx->lbvh_fit == 0;
+    if (lists_on) LUCIDA_HIP(ctx->lb_level.ensure((15 * (n - 1) + 16) * 4), LUCIDA_OOM);
+    const FitLists lists = lists_on ? fit_lists(ctx, n) : FitLists{nullptr, nullptr, nullptr, nullptr};
     hipLaunchKernelGGL(k_fold_bounds, dim3(1), dim3(PREP_BLOCK), 0, s, ctx->lb_partial.as<float>(), (uint32_t)nb,
-                       scene_box);
+                       scene_box, n >= 2 ? ctx->lb_parent.as<uint32_t>() : nullptr, lists.counts);
     hipLaunchKernelGGL(k_morton, dim3(nb), dim3(PREP_BLOCK), 0, s, ctx->lb_bounds.as<LucidaAabb>(), (uint32_t)n,
                        scene_box, ctx->lb_codes.as<uint32_t>(), ctx->lb_vals.as<uint32_t>());
     if (keep_unsorted_codes)
@@ -900,15 +1197,18 @@ static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size
     if (st != LUCIDA_OK) return st;
     if (timed) LUCIDA_HIP(hipEventRecord(ev[2], s), LUCIDA_VALIDATION);
     if (n >= 2) {
-        LUCIDA_HIP(hipMemsetAsync(ctx->lb_parent.ptr, 0xFF, 4, s), LUCIDA_VALIDATION);  // root has no parent
+        // the root's parent and the list counters were cleared by k_fold_bounds
         hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, ctx->lb_codes.as<uint32_t>(),
                            (uint32_t)n, ctx->lb_karras.as<uint2>(), ctx->lb_rank.as<uint2>(),
-                           ctx->lb_parent.as<uint32_t>());
+                           ctx->lb_parent.as<uint32_t>(), lists);
     }
     if (timed) LUCIDA_HIP(hipEventRecord(ev[3], s), LUCIDA_VALIDATION);
     if (n >= 2) {
         uint32_t* subtree = ctx->lb_subtree.as<uint32_t>();
-        if (ctx->lbvh_fit == 1) {
+        if (ctx->lbvh_fit == 0) {
+            lucida_status fs = fit_treelets(ctx, n);
+            if (fs != LUCIDA_OK) return fs;
+        } else if (ctx->lbvh_fit == 1) {
             LUCIDA_HIP(hipMemsetAsync(ctx->lb_flags.ptr, 0, (n - 1) * 4, s), LUCIDA_VALIDATION);
             hipLaunchKernelGGL(k_fit, dim3(blocks_for(n, 256)), dim3(256), 0, s, ctx->lb_bounds.as<LucidaAabb>(),
                                ctx->lb_vals.as<uint32_t>(), (uint32_t)n, ctx->lb_karras.as<uint2>(),
@@ -930,10 +1230,10 @@ static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size
     return LUCIDA_OK;
 }
 
-// Relayout into GpuBvhNode on the device; returns the node count (a one-word
-// readback of the root's emitted-internal count).
-static lucida_status relayout_device(lucida_ctx* ctx, size_t n, GpuBvhNode* nodes_out, uint32_t* prims_out,
-                                     size_t* num_nodes) {
+// Relayout into GpuBvhNode on the device. The node count is 1 + 2 x the
+// root's emitted-internal count, read back by the caller (`inner` word 0)
+// once everything is queued.
+static lucida_status relayout_device(lucida_ctx* ctx, size_t n, GpuBvhNode* nodes_out, uint32_t* prims_out) {
     hipStream_t s = ctx->stream;
     const size_t internal = n - 1;
     uint32_t* subtree = ctx->lb_subtree.as<uint32_t>();
@@ -946,11 +1246,6 @@ static lucida_status relayout_device(lucida_ctx* ctx, size_t n, GpuBvhNode* node
                        rank, ctx->lb_nodebox.as<LucidaAabb>(), nodes_out);
     if (prims_out)
         LUCIDA_HIP(hipMemcpyAsync(prims_out, ctx->lb_vals.ptr, n * 4, hipMemcpyDeviceToDevice, s), LUCIDA_VALIDATION);
-    uint32_t root_inner = 0;
-    LUCIDA_HIP(hipMemcpyAsync(&root_inner, inner, 4, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
-    LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
-    // The root, when it is a collapsed leaf, has subtree <= 4 and inner 0.
-    *num_nodes = 1 + 2 * (size_t)root_inner;
     return LUCIDA_OK;
 }
 
@@ -976,15 +1271,24 @@ lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t
                            prim_indices);
         *num_nodes = 1;
     } else {
-        st = relayout_device(ctx, n, nodes, prim_indices, num_nodes);
+        st = relayout_device(ctx, n, nodes, prim_indices);
         if (st != LUCIDA_OK) return st;
     }
     if (tris_sorted)
         hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, tris,
                            ctx->lb_vals.as<uint32_t>(), (uint32_t)n, tris_sorted);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
+    if (stage_us) LUCIDA_HIP(hipEventRecord(e1, ctx->stream), LUCIDA_VALIDATION);
+    if (n > 1) {
+        // the root's emitted-internal count (0 when the root is a collapsed leaf)
+        uint32_t root_inner = 0;
+        LUCIDA_HIP(hipMemcpyAsync(&root_inner, ctx->lb_subtree.as<uint32_t>() + (n - 1), 4, hipMemcpyDeviceToHost,
+                                  ctx->stream),
+                   LUCIDA_READBACK);
+        LUCIDA_HIP(hipStreamSynchronize(ctx->stream), LUCIDA_VALIDATION);
+        *num_nodes = 1 + 2 * (size_t)root_inner;
+    }
     if (stage_us) {
-        LUCIDA_HIP(hipEventRecord(e1, ctx->stream), LUCIDA_VALIDATION);
         LUCIDA_HIP(hipEventSynchronize(e1), LUCIDA_VALIDATION);
         float ms = 0;
         LUCIDA_HIP(hipEventElapsedTime(&ms, e0, e1), LUCIDA_VALIDATION);
@@ -1003,7 +1307,8 @@ using namespace lucida;
 extern "C" {
 
 lucida_status lucida_ctx_set_lbvh_fit(lucida_ctx* ctx, int mode) {
-    if (!ctx || mode < 0 || mode > 1) return set_error(LUCIDA_INVALID_ARG, "lucida_ctx_set_lbvh_fit: mode is 0 or 1");
+    if (!ctx || mode < 0 || mode > 2)
+        return set_error(LUCIDA_INVALID_ARG, "lucida_ctx_set_lbvh_fit: mode is 0, 1 or 2");
     ctx->lbvh_fit = mode;
     return LUCIDA_OK;
 }

@@ -43,7 +43,7 @@ def check_sort(gpu, oracle, keys, label):
     np.testing.assert_array_equal(vo, ev, err_msg=f"{label}: payload (stability)")
 
 
-@pytest.mark.parametrize("n", [1, 2, 255, 256, 257, 511, 512, 1000, 2047, 2048, 2049, 4096, 100_000])
+@pytest.mark.parametrize("n", [1, 2, 255, 256, 257, 511, 512, 1000, 2047, 2048, 2049, 4096, 100_000, 3_000_001])
 def test_sorts_random_keys(gpu, oracle, n):
     check_sort(gpu, oracle, xorshift(0x12345678, n), f"random n={n}")
 
@@ -156,21 +156,20 @@ def test_full_size_c2_build_is_bit_exact(gpu, oracle):
     np.testing.assert_array_equal(bits(nodes), bits(o["nodes"]))
 
 
-@pytest.mark.parametrize("case", ["icosphere-3", "cornell-mesh", "duplicates", "n=2", "n=5", "sphere-c2"])
-def test_atomic_and_level_fits_agree(gpu, oracle, case):
-    """The atomic bottom-up fit and the level-synchronous fit produce the same
-    tree (and both the oracle's)."""
+@pytest.mark.parametrize("case", ["icosphere-3", "cornell-mesh", "duplicates", "n=2", "n=5", "n=1023", "n=1025", "n=5000", "sphere-c2"])
+def test_fit_orders_agree(gpu, oracle, case):
+    """The two-phase fit, the atomic bottom-up fit and the level-synchronous
+    fit produce the same tree, the oracle's."""
     tris, pos = geometry(case) if case == "sphere-c2" else case_geometry(case)
     say(f"lbvh fit modes case: {case} n={len(tris)}")
-    try:
-        n_atomic, p_atomic = L.LbvhBuilder(gpu, fit="atomic").build(tris, pos)
-    finally:
-        L.LbvhBuilder(gpu, fit="levels")
-    n_levels, p_levels = L.LbvhBuilder(gpu, fit="levels").build(tris, pos)
-    np.testing.assert_array_equal(bits(n_atomic), bits(n_levels))
-    np.testing.assert_array_equal(p_atomic, p_levels)
     o = oracle.lbvh_build(tris, pos)
-    np.testing.assert_array_equal(bits(n_levels), bits(o["nodes"]))
+    try:
+        for fit in ("atomic", "levels", "blocks"):
+            nodes, prims = L.LbvhBuilder(gpu, fit=fit).build(tris, pos)
+            np.testing.assert_array_equal(bits(nodes), bits(o["nodes"]), err_msg=fit)
+            np.testing.assert_array_equal(prims, o["prim_indices"], err_msg=fit)
+    finally:
+        L.LbvhBuilder(gpu, fit="blocks")
 
 
 def test_scene_lbvh_is_device_resident_and_compacted(gpu, oracle):
