@@ -356,17 +356,30 @@ struct Surface {
         V3 f_avg = fresnel_average(normal_reflectance());
         return vmin(vmax(f * e_ss + f_avg * (1.0f - e_ss), v3s(0.0f)), v3s(1.0f));
     }
-    __device__ inline float specular_probability(float cos_o) const {
+    // specular_probability(cos_o), given sa = specular_albedo(cos_o)
+    __device__ inline float specular_probability(float cos_o, V3 sa) const {
         if (transmission > 0.0f) {
             float f = fresnel_dielectric(fabsf(cos_o), relative_ior);
             if (f >= 1.0f) return 1.0f;
             return clampf(f, 0.05f, 0.95f);
         }
-        V3 sa = specular_albedo(cos_o);
         float spec = luminance(sa);
-        float diff = luminance(diffuse_albedo * (v3s(1.0f) - specular_albedo(cos_o)));
+        float diff = luminance(diffuse_albedo * (v3s(1.0f) - sa));
         if (spec + diff <= 0.0f) return 1.0f;
         return clampf(spec / (spec + diff), 0.1f, 0.9f);
+    }
+    // The terms that depend on the outgoing direction alone, computed once
+    // per bounce and shared by every surface_eval_pdf / surface_sample of
+    // that bounce (each must be given the wo bound here): the same values
+    // those functions would otherwise compute per call.
+    V3 wo_spec_albedo, wo_compensation;
+    float wo_p_spec, alpha, wo_lambda;
+    __device__ inline void bind(V3 wo) {
+        alpha = roughness_to_alpha(roughness);
+        wo_lambda = smith_lambda(alpha, wo.z);
+        wo_spec_albedo = specular_albedo(wo.z);
+        wo_p_spec = specular_probability(wo.z, wo_spec_albedo);
+        wo_compensation = transmission > 0.0f ? v3s(1.0f) : compensation(wo.z, normal_reflectance());
     }
     __device__ inline V3 reflection_fresnel(float c) const {
         if (transmission > 0.0f && !conductor) return v3s(fresnel_dielectric(c, relative_ior));
@@ -423,52 +436,48 @@ __device__ inline float dielectric_pdf(float alpha, float eta, V3 wo, V3 wi) {
     float jac = fabsf(eta * eta * wi_h) / (denom * denom);
     return ggx_vndf_half_pdf(alpha, wo, h) * jac;
 }
-__device__ inline float specular_pdf(float alpha, V3 wo, V3 wi) {
-    if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
-    V3 h = normalize_or(wo + wi, v3s(0.0f));
-    if (is_zero(h)) return 0.0f;
-    return ggx_vndf_pdf(alpha, wo, h);
-}
 
-__device__ inline V3 surface_eval(const Surface& s, V3 wo, V3 wi) {
-    if (wo.z <= 0.0f) return v3s(0.0f);
-    float alpha = roughness_to_alpha(s.roughness);
+// The BSDF value and its sampling pdf for one (wo, wi) (Bsdf::eval and
+// Bsdf::pdf, bsdf.rs), computed together: the half vector and the GGX
+// distribution once for both, the wo terms from Surface::bind (which must
+// have been given this wo; so must surface_sample).
+__device__ inline void surface_eval_pdf(const Surface& s, V3 wo, V3 wi, V3* f_out, float* pdf_out) {
+    *f_out = v3s(0.0f);
+    *pdf_out = 0.0f;
+    if (wo.z <= 0.0f) return;
+    const float alpha = s.alpha;
+    const float p_spec = s.wo_p_spec;
+    const float rest = 1.0f - p_spec;
     if (wi.z < 0.0f) {
-        if (s.transmission <= 0.0f) return v3s(0.0f);
-        float t = dielectric_eval(alpha, s.relative_ior, wo, wi);
-        return v3s(t) * s.transmission * s.tint;
+        if (s.transmission <= 0.0f) return;
+        const float t = dielectric_eval(alpha, s.relative_ior, wo, wi);
+        *f_out = v3s(t) * s.transmission * s.tint;
+        *pdf_out = rest * s.transmission * dielectric_pdf(alpha, s.relative_ior, wo, wi);
+        return;
     }
-    if (wi.z <= 0.0f) return v3s(0.0f);
-    V3 transmitted = v3s(1.0f) - s.specular_albedo(wo.z);
+    if (wi.z <= 0.0f) return;
+    V3 transmitted = v3s(1.0f) - s.wo_spec_albedo;
     V3 f = s.diffuse_albedo * INV_PI * transmitted * (1.0f - s.transmission);
     V3 h = normalize_or(wo + wi, v3s(0.0f));
+    float spec_pdf = 0.0f;
     if (!is_zero(h)) {
-        float d = ggx_d(alpha, h.z);
-        float g2 = smith_g2(alpha, wo.z, wi.z);
+        const float d = ggx_d(alpha, h.z);
+        const float g2 = 1.0f / (1.0f + s.wo_lambda + smith_lambda(alpha, wi.z));  // smith_g2(alpha, wo.z, wi.z)
         V3 fr = s.reflection_fresnel(fmaxf(dot(wo, h), 0.0f));
         V3 single = fr * (d * g2 / (4.0f * wo.z * wi.z));
-        f = f + (s.transmission > 0.0f ? single : single * s.compensation(wo.z, s.normal_reflectance()));
+        f = f + (s.transmission > 0.0f ? single : single * s.wo_compensation);
+        spec_pdf = (1.0f / (1.0f + s.wo_lambda)) * d / (4.0f * wo.z);  // ggx_vndf_pdf(alpha, wo, h)
     }
-    return f;
+    *f_out = f;
+    *pdf_out = p_spec * spec_pdf + rest * (1.0f - s.transmission) * (wi.z * INV_PI);
 }
-__device__ inline float surface_pdf(const Surface& s, V3 wo, V3 wi) {
-    if (wo.z <= 0.0f) return 0.0f;
-    float p_spec = s.specular_probability(wo.z);
-    float alpha = roughness_to_alpha(s.roughness);
-    float rest = 1.0f - p_spec;
-    if (wi.z < 0.0f) {
-        if (s.transmission <= 0.0f) return 0.0f;
-        return rest * s.transmission * dielectric_pdf(alpha, s.relative_ior, wo, wi);
-    }
-    if (wi.z <= 0.0f) return 0.0f;
-    return p_spec * specular_pdf(alpha, wo, wi) + rest * (1.0f - s.transmission) * (wi.z * INV_PI);
-}
+
 // Returns pdf (0 when invalid); wi and weight through pointers.
 __device__ inline float surface_sample(const Surface& s, V3 wo, float u_lobe, float ux, float uy, V3* wi_out,
                                        V3* weight) {
     if (wo.z <= 0.0f) return 0.0f;
-    float p_spec = s.specular_probability(wo.z);
-    float alpha = roughness_to_alpha(s.roughness);
+    const float p_spec = s.wo_p_spec;
+    const float alpha = s.alpha;
     V3 wi;
     if (u_lobe < p_spec) {
         V3 m = sample_ggx_vndf(wo, alpha, ux, uy);
@@ -486,8 +495,9 @@ __device__ inline float surface_sample(const Surface& s, V3 wo, float u_lobe, fl
     }
     if (wi.z == 0.0f || is_zero(wi)) return 0.0f;
     if (wi.z < 0.0f && s.transmission <= 0.0f) return 0.0f;
-    V3 f = surface_eval(s, wo, wi);
-    float pdf = surface_pdf(s, wo, wi);
+    V3 f;
+    float pdf;
+    surface_eval_pdf(s, wo, wi, &f, &pdf);
     if (!(pdf > 0.0f)) return 0.0f;
     *wi_out = wi;
     *weight = f * (fabsf(wi.z) / pdf);

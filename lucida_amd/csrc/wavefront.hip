@@ -1109,7 +1109,7 @@ __device__ __forceinline__ bool shade_bounce(const SceneView& sc, const GpuUnifo
         }
         if (w > 0.0f) P.radiance = P.radiance + P.throughput * emissive * w;
     }
-    const Surface surf = surface_from_material(m, front, sc.ggx);
+    Surface surf = surface_from_material(m, front, sc.ggx);
     if (depth == 0) {
         V3 ga = vmax(surf.diffuse_albedo + surf.f0 + emissive, v3s(1.0e-3f));
         guide = true;
@@ -1118,6 +1118,7 @@ __device__ __forceinline__ bool shade_bounce(const SceneView& sc, const GpuUnifo
     }
     const V3 wo = to_local(-dir, normal);
     if (!(wo.z > 0.0f)) return false;
+    surf.bind(wo);
     Sampler& s = P.s;
     if (mode != LUCIDA_SAMPLING_BSDF && depth + 1 < u.max_depth && total > 0) {
         // direct_light (integrator.rs:418): the contribution is resolved
@@ -1173,9 +1174,11 @@ __device__ __forceinline__ bool shade_bounce(const SceneView& sc, const GpuUnifo
         if (valid && pdf > 0.0f) {
             V3 wi = to_local(wi_world, normal);
             if (wi.z != 0.0f && (wi.z > 0.0f || surf.transmission > 0.0f)) {
-                V3 f = surface_eval(surf, wo, wi);
+                V3 f;
+                float bsdf_pdf;
+                surface_eval_pdf(surf, wo, wi, &f, &bsdf_pdf);
                 if (max3(f) > 0.0f) {
-                    float mis = mode == LUCIDA_SAMPLING_MIS ? power_heuristic(pdf, surface_pdf(surf, wo, wi)) : 1.0f;
+                    float mis = mode == LUCIDA_SAMPLING_MIS ? power_heuristic(pdf, bsdf_pdf) : 1.0f;
                     if (mis > 0.0f) {
                         sr.o = offset_ray_origin(position, wi.z < 0.0f ? -gnormal : gnormal);
                         sr.d = wi_world;
