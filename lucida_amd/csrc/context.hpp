@@ -98,7 +98,7 @@ struct lucida_scene {
     GpuVertexAttr* vertex_attrs = nullptr;
     GpuTriangle* triangles = nullptr;
     float4* tri_edges = nullptr;  // per triangle: p0, p1 - p0, p2 - p0 (the intersection's own operands)
-    float4* prim_w = nullptr;     // per analytic primitive a quad's w = n / |n|^2 (quad_w), then their box (2)
+    float4* prim_w = nullptr;     // per analytic primitive a quad's w = n / |n|^2 (quad_w), then their boxes
     void* wide = nullptr;         // 4-wide BVH collapsed from bvh_nodes (128-byte nodes), or null
     size_t num_wide = 0;
     GpuBvhNode* bvh_nodes = nullptr;

@@ -866,23 +866,31 @@ __device__ inline bool traverse_instances(const SceneView& sc, RayStack& st, V3 
 }
 
 // The closest of the analytic primitives (scene.wgsl: each tested in turn
-// before the tree; on equal t the later one wins), into best_t / best. Their
-// padded bounding box (k_prim_box, after the primitives' quad_w planes) is
-// tested first: a ray that misses it misses every primitive, so the loop is
-// skipped without changing the result.
+// before the tree; on equal t the later one wins), into best_t / best. The
+// primitives' padded boxes (k_prim_box, after their quad_w planes) are
+// tested first: the box of them all, then that of each run of PRIM_CHUNK
+// consecutive primitives. A ray that misses a box misses every primitive in
+// it, so skipping them leaves the result as the full loop's.
+constexpr uint32_t PRIM_CHUNK = 4;
 __device__ inline void analytic_closest(const SceneView& sc, V3 o, V3 d, float* best_t, uint32_t* best) {
-    if (sc.num_prims == 0) return;
+    const uint32_t n = sc.num_prims;
+    if (n == 0) return;
     const V3 inv = safe_inv_dir(d);
     float entry;
-    if (!slab(sc.prim_w[sc.num_prims], sc.prim_w[sc.num_prims + 1], o, inv, T_MIN, *best_t, &entry)) return;
-    for (uint32_t k = 0; k < sc.num_prims; ++k) {
-        const GpuPrimitive& pr = sc.prims[k];
-        float t;
-        const bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, *best_t, &t)
-                                                        : intersect_sphere(pr, o, d, T_MIN, *best_t, &t);
-        if (h) {
-            *best_t = t;
-            *best = k;
+    if (!slab(sc.prim_w[n], sc.prim_w[n + 1], o, inv, T_MIN, *best_t, &entry)) return;
+    for (uint32_t g = 0; g * PRIM_CHUNK < n; ++g) {
+        if (!slab(sc.prim_w[n + 2 + 2 * g], sc.prim_w[n + 3 + 2 * g], o, inv, T_MIN, *best_t, &entry)) continue;
+        const uint32_t end = min(g * PRIM_CHUNK + PRIM_CHUNK, n);
+        for (uint32_t k = g * PRIM_CHUNK; k < end; ++k) {
+            const GpuPrimitive& pr = sc.prims[k];
+            float t;
+            const bool h = pr.kind == LUCIDA_PRIM_KIND_QUAD
+                               ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, *best_t, &t)
+                               : intersect_sphere(pr, o, d, T_MIN, *best_t, &t);
+            if (h) {
+                *best_t = t;
+                *best = k;
+            }
         }
     }
 }
@@ -1645,29 +1653,35 @@ __global__ void k_prim_planes(const GpuPrimitive* __restrict__ prims, size_t n, 
 }
 
 // The box around every analytic primitive (a quad by its four corners, a
-// sphere by center +- radius) at w[n] (low) and w[n + 1] (high), padded by
-// 1e-3 of its largest coordinate magnitude (plus 1e-6): far beyond the
-// rounding of the primitive tests and of the slab test, so a ray that any
-// primitive test accepts always hits the box (a flat set of quads gets a
-// thickness).
+// sphere by center +- radius) at w[n] (low) and w[n + 1] (high), then the box
+// of each run of PRIM_CHUNK consecutive primitives g at w[n + 2 + 2g] and
+// w[n + 3 + 2g], all padded by 1e-3 of the largest coordinate magnitude
+// (plus 1e-6): far beyond the rounding of the primitive tests and of the
+// slab test, so a ray that a primitive test accepts always hits its boxes
+// (a flat set of quads gets a thickness).
+__device__ inline void prim_bounds(const GpuPrimitive& q, float* lo, float* hi) {
+    for (int a = 0; a < 3; ++a) {
+        if (q.kind == LUCIDA_PRIM_KIND_QUAD) {
+            const float p = q.position[a], u = q.edge_u[a], v = q.edge_v[a];
+            lo[a] = fminf(fminf(p, p + u), fminf(p + v, p + u + v));
+            hi[a] = fmaxf(fmaxf(p, p + u), fmaxf(p + v, p + u + v));
+        } else {
+            lo[a] = q.position[a] - fabsf(q.radius);
+            hi[a] = q.position[a] + fabsf(q.radius);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_prim_box(const GpuPrimitive* __restrict__ prims, size_t n,
                                                   float4* __restrict__ w) {
     __shared__ float red[6][256];
     float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
     for (size_t i = threadIdx.x; i < n; i += 256) {
-        const GpuPrimitive& q = prims[i];
+        float l[3], h[3];
+        prim_bounds(prims[i], l, h);
         for (int a = 0; a < 3; ++a) {
-            float l, h;
-            if (q.kind == LUCIDA_PRIM_KIND_QUAD) {
-                const float p = q.position[a], u = q.edge_u[a], v = q.edge_v[a];
-                l = fminf(fminf(p, p + u), fminf(p + v, p + u + v));
-                h = fmaxf(fmaxf(p, p + u), fmaxf(p + v, p + u + v));
-            } else {
-                l = q.position[a] - fabsf(q.radius);
-                h = q.position[a] + fabsf(q.radius);
-            }
-            lo[a] = fminf(lo[a], l);
-            hi[a] = fmaxf(hi[a], h);
+            lo[a] = fminf(lo[a], l[a]);
+            hi[a] = fmaxf(hi[a], h[a]);
         }
     }
     for (int a = 0; a < 3; ++a) {
@@ -1683,12 +1697,25 @@ __global__ __launch_bounds__(256) void k_prim_box(const GpuPrimitive* __restrict
             }
         __syncthreads();
     }
+    float m = 0.0f;
+    for (int a = 0; a < 6; ++a) m = fmaxf(m, fabsf(red[a][0]));
+    const float pad = 1.0e-3f * m + 1.0e-6f;
     if (threadIdx.x == 0) {
-        float m = 0.0f;
-        for (int a = 0; a < 6; ++a) m = fmaxf(m, fabsf(red[a][0]));
-        const float pad = 1.0e-3f * m + 1.0e-6f;
         w[n] = make_float4(red[0][0] - pad, red[1][0] - pad, red[2][0] - pad, 0.0f);
         w[n + 1] = make_float4(red[3][0] + pad, red[4][0] + pad, red[5][0] + pad, 0.0f);
+    }
+    for (size_t g = threadIdx.x; g * PRIM_CHUNK < n; g += 256) {
+        float l[3] = {3.0e38f, 3.0e38f, 3.0e38f}, h[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+        for (size_t k = g * PRIM_CHUNK; k < n && k < g * PRIM_CHUNK + PRIM_CHUNK; ++k) {
+            float pl[3], ph[3];
+            prim_bounds(prims[k], pl, ph);
+            for (int a = 0; a < 3; ++a) {
+                l[a] = fminf(l[a], pl[a]);
+                h[a] = fmaxf(h[a], ph[a]);
+            }
+        }
+        w[n + 2 + 2 * g] = make_float4(l[0] - pad, l[1] - pad, l[2] - pad, 0.0f);
+        w[n + 3 + 2 * g] = make_float4(h[0] + pad, h[1] + pad, h[2] + pad, 0.0f);
     }
 }
 
@@ -1711,8 +1738,9 @@ __global__ void k_tri_edges(const GpuTriangle* __restrict__ tris, const float4* 
 lucida_status build_prim_planes(lucida_ctx* ctx, lucida_scene* scene) {
     const size_t n = scene->num_primitives;
     if (n == 0) return LUCIDA_OK;
-    // per primitive its quad_w, then the primitives' box (k_prim_box)
-    if (!scene->prim_w) LUCIDA_HIP(hipMalloc(&scene->prim_w, (n + 2) * sizeof(float4)), LUCIDA_OOM);
+    // per primitive its quad_w, then the primitives' boxes (k_prim_box)
+    const size_t chunks = (n + PRIM_CHUNK - 1) / PRIM_CHUNK;
+    if (!scene->prim_w) LUCIDA_HIP(hipMalloc(&scene->prim_w, (n + 2 + 2 * chunks) * sizeof(float4)), LUCIDA_OOM);
     hipLaunchKernelGGL(k_prim_planes, dim3(grid_for(n, 64)), dim3(64), 0, ctx->stream, scene->primitives, n,
                        scene->prim_w);
     hipLaunchKernelGGL(k_prim_box, dim3(1), dim3(256), 0, ctx->stream, scene->primitives, n, scene->prim_w);
