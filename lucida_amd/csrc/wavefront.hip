@@ -895,6 +895,28 @@ __device__ inline void analytic_closest(const SceneView& sc, V3 o, V3 d, float* 
     }
 }
 
+// Whether any analytic primitive blocks the segment (T_MIN, t_max)
+// (light.rs:327 unoccluded; scene.rs:577), behind the same boxes.
+__device__ inline bool analytic_any(const SceneView& sc, V3 o, V3 d, float t_max) {
+    const uint32_t n = sc.num_prims;
+    if (n == 0) return false;
+    const V3 inv = safe_inv_dir(d);
+    float entry;
+    if (!slab(sc.prim_w[n], sc.prim_w[n + 1], o, inv, T_MIN, t_max, &entry)) return false;
+    for (uint32_t g = 0; g * PRIM_CHUNK < n; ++g) {
+        if (!slab(sc.prim_w[n + 2 + 2 * g], sc.prim_w[n + 3 + 2 * g], o, inv, T_MIN, t_max, &entry)) continue;
+        const uint32_t end = min(g * PRIM_CHUNK + PRIM_CHUNK, n);
+        for (uint32_t k = g * PRIM_CHUNK; k < end; ++k) {
+            const GpuPrimitive& pr = sc.prims[k];
+            float t;
+            if (pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, t_max, &t)
+                                                 : intersect_sphere(pr, o, d, T_MIN, t_max, &t))
+                return true;
+        }
+    }
+    return false;
+}
+
 // ---------------------------------------------------------------------------
 // EXTEND (extend.wgsl:124, scene.rs:452): analytic primitives by brute force,
 // then the BVH with t_max = the best analytic hit. Writes (t, u, v, id):
@@ -1318,13 +1340,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
         const SceneView& sc = BATCH ? jobs[j].sc : sc0;
         const V3 o = f4xyz(a), d = f4xyz(b);
         const float t_max = a.w;
-        bool blocked = false;
-        for (uint32_t k = 0; k < sc.num_prims && !blocked; ++k) {
-            const GpuPrimitive& pr = sc.prims[k];
-            float t;
-            blocked = pr.kind == LUCIDA_PRIM_KIND_QUAD ? intersect_quad(pr, f4xyz(sc.prim_w[k]), o, d, T_MIN, t_max, &t)
-                                                       : intersect_sphere(pr, o, d, T_MIN, t_max, &t);
-        }
+        bool blocked = analytic_any(sc, o, d, t_max);
         if (!blocked) {
             float t, u, v;
             uint32_t tri, inst;
@@ -1481,14 +1497,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_megakernel(SceneView sc, const Gpu
                 // CONNECT
                 if (shadow) {
                     ++shadow_rays;
-                    bool blocked = false;
-                    for (uint32_t k = 0; k < sc.num_prims && !blocked; ++k) {
-                        const GpuPrimitive& pr = sc.prims[k];
-                        float tt;
-                        blocked = pr.kind == LUCIDA_PRIM_KIND_QUAD
-                                      ? intersect_quad(pr, f4xyz(sc.prim_w[k]), sr.o, sr.d, T_MIN, sr.t_max, &tt)
-                                      : intersect_sphere(pr, sr.o, sr.d, T_MIN, sr.t_max, &tt);
-                    }
+                    bool blocked = analytic_any(sc, sr.o, sr.d, sr.t_max);
                     if (!blocked) {
                         float tt, uu, vv;
                         uint32_t tri2, inst2;
