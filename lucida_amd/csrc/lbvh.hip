@@ -217,9 +217,10 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_hist(const uint32_t* __res
     hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram, in three launches: per-segment
-// sums (k_scan_reduce), a scan of those sums (k_scan_top), and a scan of each
-// segment from its carry (k_scan_down). Every global access is coalesced; a
+// Exclusive scan of the digit-major histogram, in two launches: per-segment
+// sums (k_scan_reduce), then a scan of each segment from its carry, the sum
+// of the segment sums before it, which each block adds up itself
+// (k_scan_down). Every global access is coalesced; a
 // segment is SCAN_ITEMS words per thread of a SCAN_BLOCK block.
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 8;
@@ -263,20 +264,6 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* __re
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-// One block: exclusive scan of `count` partial sums in place, SCAN_BLOCK at a time.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_top(uint32_t* __restrict__ partials, uint32_t count) {
-    __shared__ uint32_t wsum[SCAN_BLOCK / 64];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < count; base += SCAN_BLOCK) {
-        const uint32_t idx = base + threadIdx.x;
-        const uint32_t v = idx < count ? partials[idx] : 0u;
-        uint32_t total;
-        const uint32_t inc = block_inclusive_scan(v, wsum, &total);
-        if (idx < count) partials[idx] = carry + inc - v;
-        carry += total;
-    }
-}
-
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(uint32_t* __restrict__ data, uint32_t count,
                                                           const uint32_t* __restrict__ partials) {
     __shared__ uint32_t seg[SCAN_SEG];
@@ -296,8 +283,12 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(uint32_t* __restrict__
         local[i] = seg[threadIdx.x * SCAN_ITEMS + i];
         s += local[i];
     }
-    uint32_t total;
-    uint32_t run = partials[blockIdx.x] + block_inclusive_scan(s, wsum, &total) - s;
+    // this segment's carry: the sums of the segments before it
+    uint32_t before = 0;
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += SCAN_BLOCK) before += partials[b];
+    uint32_t carry, total;
+    (void)block_inclusive_scan(before, wsum, &carry);
+    uint32_t run = carry + block_inclusive_scan(s, wsum, &total) - s;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
         seg[threadIdx.x * SCAN_ITEMS + i] = run;
@@ -941,7 +932,6 @@ lucida_status scan_u32_device(lucida_ctx* ctx, uint32_t* data, uint32_t count, u
     if (count == 0) return LUCIDA_OK;
     const uint32_t segs = blocks_for(count, SCAN_SEG);
     hipLaunchKernelGGL(k_scan_reduce, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, data, count, partials);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, partials, segs);
     hipLaunchKernelGGL(k_scan_down, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, data, count, partials);
     LUCIDA_HIP(hipGetLastError(), LUCIDA_SHADER);
     return LUCIDA_OK;
@@ -970,8 +960,7 @@ static lucida_status sort_device_with(lucida_ctx* ctx, uint32_t* keys, uint32_t*
         hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_BLOCK), 0, ctx->stream, src_k, (uint32_t)n,
                            nullptr, shift, hist, tiles);
         hipLaunchKernelGGL(k_scan_reduce, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, hist, hist_words, partials);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, ctx->stream, partials, segs);
-        hipLaunchKernelGGL(k_scan_down, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, hist, hist_words, partials);
+            hipLaunchKernelGGL(k_scan_down, dim3(segs), dim3(SCAN_BLOCK), 0, ctx->stream, hist, hist_words, partials);
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_BLOCK), 0, ctx->stream, src_k, src_v,
                            (uint32_t)n, nullptr, shift, hist, tiles, dst_k, dst_v);
         std::swap(src_k, dst_k);
@@ -1060,7 +1049,6 @@ lucida_status sort_pairs_counted(hipStream_t stream, uint32_t* keys, uint32_t* v
         hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_BLOCK), 0, stream, src_k, (uint32_t)capacity, count,
                            shift, hist, tiles);
         hipLaunchKernelGGL(k_scan_reduce, dim3(segs), dim3(SCAN_BLOCK), 0, stream, hist, hist_words, partials);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_BLOCK), 0, stream, partials, segs);
         hipLaunchKernelGGL(k_scan_down, dim3(segs), dim3(SCAN_BLOCK), 0, stream, hist, hist_words, partials);
         // the last pass needs only the values
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_BLOCK), 0, stream, src_k, src_v,
