@@ -581,7 +581,7 @@ def main() -> int:
             t0 = time.perf_counter()
             progress = lambda msg: print(f"denoiser training: {msg}", file=sys.stderr, flush=True)
             pairs, data_paths = T.make_training_set(
-                gpu, a.net_scenes, ref_spp=a.net_ref_spp,
+                gpu, a.net_scenes, input_spp=T.INPUT_SPP, ref_spp=a.net_ref_spp,
                 log=lambda msg: progress(msg) if msg.startswith("scene ") and int(msg.split()[1][:-1]) % 8 == 7
                 else None)
             t1 = time.perf_counter()
@@ -597,7 +597,7 @@ def main() -> int:
                 "target_spp": a.net_ref_spp, "data_paths": data_paths, "data_s": round(t1 - t0, 2),
                 "train_s": round(t2 - t1, 2), "ms_per_step": round((t2 - t1) / a.net_steps * 1e3, 3),
                 "loss_first_100": float(np.mean(losses[:100])), "loss_last_100": float(np.mean(losses[-100:])),
-                "input_spp": [1, 2, 4, 8, 16, 32, 64], "input_channels": 16,
+                "input_spp": list(T.INPUT_SPP), "input_channels": 16,
                 "held_out": "the benchmark scene (seed 0, frequency 229) is not in the training set"}
         out["time_to_relmse"] = time_to_relmse(L, gpu, scene, blob, a.width, a.height, a.depth, (0.01, 1e-3),
                                                a.relmse_ref_spp, a.relmse_max_spp, net)

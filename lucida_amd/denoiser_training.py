@@ -53,8 +53,13 @@ def scene_variant(k: int, rng: np.random.Generator):
     return L.Blob("box-c1" if kind < 0.93 else "cornell-mesh"), 0.0, 0.0
 
 
+# Input sample counts of the training pairs (one per scene, drawn uniformly):
+# the range the denoiser is used over, 1 to 256 spp.
+INPUT_SPP = (1, 2, 4, 8, 16, 32, 64, 128, 256)
+
+
 def make_training_set(gpu: L.Gpu, scenes: int = 24, width: int = 256, height: int = 256,
-                      input_spp=(1, 2, 4, 8, 16, 32, 64), ref_spp: int = 4096, depth: int = 8, seed: int = 7,
+                      input_spp=INPUT_SPP, ref_spp: int = 4096, depth: int = 8, seed: int = 7,
                       log=None):
     """Renders the (input, target) film pairs; returns them and the paths traced."""
     rng = np.random.default_rng(seed)

@@ -95,7 +95,8 @@ def test_untrained_network_returns_the_atrous_estimate(gpu, held_out):
 
 def test_trained_network_beats_atrous_on_held_out_scenes(gpu, held_out):
     say("training: 64 scenes at 256^2 (1-64 spp inputs, 2048-spp targets), 3000 Adam steps, batch 8 of 256^2")
-    pairs, paths = T.make_training_set(gpu, 64, width=RES, height=RES, ref_spp=2048,
+    pairs, paths = T.make_training_set(gpu, 64, width=RES, height=RES, input_spp=(1, 2, 4, 8, 16, 32, 64),
+                                       ref_spp=2048,
                                        log=lambda m: say(m) if int(m.split()[1][:-1]) % 16 == 15 else None)
     net = L.LearnedDenoiser(gpu, seed=1)
     losses = T.train(gpu, net, pairs, 3000, crop_w=RES, crop_h=RES, log_every=500, log=say)
