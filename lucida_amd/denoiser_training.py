@@ -1,17 +1,21 @@
 """Training data and training loop for the learned denoiser.
 
 The tracer renders procedurally varied scenes twice: a low-sample input film
-(1 to 32 samples per pixel, with its albedo / normal / depth guides) and an
+(1 to 256 samples per pixel, with its albedo / normal / depth guides) and an
 independent high-sample target film of the same view (another frame seed, so
 no sample is shared). The network is trained on random crops of those pairs
 by the library's own forward, backward and Adam kernels; everything stays
 resident on the device.
 
     python -m lucida_amd.denoiser_training --scenes 32 --steps 3000 --out weights.f32
+
+With --eval the trained network is scored on held-out scenes (held_out_curves)
+and a JSON summary of the run is written to --json.
 """
 from __future__ import annotations
 
 import argparse
+import json
 import math
 import time
 
@@ -104,25 +108,95 @@ def train(gpu: L.Gpu, net: L.LearnedDenoiser, pairs, steps: int = 1500, batch: i
     return losses
 
 
+def relmse(x: np.ndarray, r: np.ndarray) -> float:
+    """Mean over pixels and channels of (x - r)^2 / (r^2 + 1e-2)."""
+    x = x.astype(np.float64)
+    r = r.astype(np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+
+# Scenes no training pair comes from (training seeds are 1000 + k): the
+# benchmark scene (seed 0) at its full size, and the two scenes the gain test
+# scores.
+HELD_OUT = (("benchmark sphere-c2 seed 0, 1024^2", dict(name="sphere-c2", frequency=229), 1024),
+            ("sphere-c2 seed 77, 256^2", dict(name="sphere-c2", seed=77, frequency=40, displacement=0.1), 256),
+            ("instances-small seed 500, 256^2", dict(name="instances-small", seed=500, frequency=5), 256))
+
+
+def held_out_curves(gpu: L.Gpu, net: L.LearnedDenoiser, max_spp: int = 256, ref_spp: int = 4096,
+                    depth: int = 8, log=None) -> list:
+    """relMSE of the plain estimate, the a-trous filter and the network after
+    1, 2, 4, ..., max_spp progressive samples, against an independent
+    ref_spp render (another frame seed), per held-out scene."""
+    out = []
+    dn = L.Denoiser(gpu)
+    for label, kw, res in HELD_OUT:
+        kw = dict(kw)
+        blob = L.Blob(kw.pop("name"), **kw)
+        scene = L.Scene(gpu, blob, build_lbvh=True)
+        mk = lambda spp, off, seed: blob.uniforms(res, res, samples=spp, max_depth=depth, sampling=L.SAMPLING_MIS,
+                                                  sampler=L.SAMPLER_SOBOL, sample_offset=off, frame_seed=seed)
+        ref_film = L.Film(gpu, res, res)
+        L.render_wavefront(gpu, scene, mk(ref_spp, 0, 0x0BADF00D), ref_spp, ref_film)
+        ref = L.film_mean(ref_film.download(), res, res)
+        del ref_film
+        film, den = L.Film(gpu, res, res), L.Film(gpu, res, res)
+        rows, done, spp = [], 0, 1
+        while spp <= max_spp:
+            L.render_wavefront(gpu, scene, mk(spp - done, done, 0x5EED1234), spp - done, film)
+            done = spp
+            plain = relmse(L.film_mean(film.download(), res, res), ref)
+            dn.run(film, den)
+            atrous = relmse(L.film_mean(den.download(), res, res), ref)
+            net.denoise(film, den)
+            learned = relmse(L.film_mean(den.download(), res, res), ref)
+            rows.append({"spp": spp, "plain": plain, "atrous": atrous, "learned": learned,
+                         "learned_over_atrous": round(learned / atrous, 4)})
+            spp *= 2
+        gpu.check_errors()
+        out.append({"scene": label, "reference_spp": ref_spp, "curve": rows})
+        if log:
+            log(f"held out {label}: learned / a-trous " +
+                " ".join(f"{r['spp']}:{r['learned_over_atrous']:.3f}" for r in rows))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--scenes", type=int, default=32)
     ap.add_argument("--ref-spp", type=int, default=1024)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--crop", type=int, default=128, help="training crop side")
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--out", type=str, default="")
+    ap.add_argument("--out", type=str, default="", help="write the trained parameters (f32) here")
+    ap.add_argument("--eval", action="store_true", help="score the network on the held-out scenes")
+    ap.add_argument("--json", type=str, default="", help="write a JSON summary of the run here")
     a = ap.parse_args()
+    say = lambda m: print(m, flush=True)
     gpu = L.Gpu(0)
     t0 = time.perf_counter()
-    pairs, paths = make_training_set(gpu, a.scenes, ref_spp=a.ref_spp, log=print)
+    pairs, paths = make_training_set(gpu, a.scenes, ref_spp=a.ref_spp, log=say)
     t1 = time.perf_counter()
-    print(f"data: {len(pairs)} pairs, {paths / 1e9:.2f} Gpaths in {t1 - t0:.1f} s", flush=True)
+    say(f"data: {len(pairs)} pairs, {paths / 1e9:.2f} Gpaths in {t1 - t0:.1f} s")
     net = L.LearnedDenoiser(gpu, seed=1)
-    losses = train(gpu, net, pairs, a.steps, a.batch, lr=a.lr, log_every=100, log=lambda s: print(s, flush=True))
-    print(f"trained {a.steps} steps in {time.perf_counter() - t1:.1f} s, final loss {np.mean(losses[-100:]):.5f}")
+    losses = train(gpu, net, pairs, a.steps, a.batch, crop_w=a.crop, crop_h=a.crop, lr=a.lr, log_every=500,
+                   log=say)
+    t2 = time.perf_counter()
+    del pairs
+    say(f"trained {a.steps} steps in {t2 - t1:.1f} s, final loss {np.mean(losses[-100:]):.5f}")
     if a.out:
         net.params().tofile(a.out)
+    summary = {"scenes": a.scenes, "input_spp": list(INPUT_SPP), "target_spp": a.ref_spp, "crop": a.crop,
+               "batch": a.batch, "lr": a.lr, "steps": a.steps, "params": L.LearnedDenoiser.param_count(),
+               "input_channels": 16, "data_paths": paths, "data_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 1),
+               "loss_first_500": float(np.mean(losses[:500])), "loss_last_500": float(np.mean(losses[-500:])),
+               "loss_every_500": [round(float(np.mean(losses[k:k + 500])), 6) for k in range(0, len(losses), 500)]}
+    if a.eval:
+        summary["held_out"] = held_out_curves(gpu, net, log=say)
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(summary, f, indent=1)
     gpu.close()
     return 0
 
