@@ -1,12 +1,13 @@
 """What the trained network gains over the method without it.
 
 The learned denoiser is trained on the GPU by the library's own kernels
-(forward, backward, Adam) on 64 generated scenes (the config-2 and config-3
-generators and Cornell boxes, seeds 1000-1063, rendered at 256^2 with 1-64
-input samples against 2048-sample targets), then scored on scenes it never
-saw: a displaced sphere in its room (seed 77) and an instanced field of
-rough dielectric and GGX meshes under a sky with hot spots (seed 500), both
-at 256^2. Score: relMSE, the mean over pixels and channels of
+(forward, backward, Adam) on 96 generated scenes (the config-2 and config-3
+generators and Cornell boxes, seeds 1000-1095, rendered at 256^2 with 1-64
+input samples against 2048-sample targets; 6000 Adam steps), then scored on
+scenes it never saw: a displaced sphere in its room (seed 77) and an
+instanced field of
+rough dielectric and GGX meshes under a sky with hot spots (seed 500),
+both at 256^2. Score: relMSE, the mean over pixels and channels of
 (x - r)^2 / (r^2 + 1e-2), against an independent 4096-sample render. The
 trained network must beat the a-trous filter alone (its fallback, and the
 estimate it corrects) at 1, 4 and 16 samples per pixel on the sphere scene
@@ -17,7 +18,10 @@ holds: it then compares two noisy estimates of the reference's own noise,
 so it is reported and not asserted.
 
 An untrained network (output layer at zero) must return the a-trous estimate
-itself, so any gain is the training's."""
+itself, so any gain is the training's. (The weight gradients are summed with
+float atomics, so two training runs differ in their last bits and drift
+apart over the steps; the run is sized so that the margins hold across
+runs.)"""
 from __future__ import annotations
 
 import sys
@@ -94,12 +98,12 @@ def test_untrained_network_returns_the_atrous_estimate(gpu, held_out):
 
 
 def test_trained_network_beats_atrous_on_held_out_scenes(gpu, held_out):
-    say("training: 64 scenes at 256^2 (1-64 spp inputs, 2048-spp targets), 3000 Adam steps, batch 8 of 256^2")
-    pairs, paths = T.make_training_set(gpu, 64, width=RES, height=RES, input_spp=(1, 2, 4, 8, 16, 32, 64),
+    say("training: 96 scenes at 256^2 (1-64 spp inputs, 2048-spp targets), 6000 Adam steps, batch 8 of 256^2")
+    pairs, paths = T.make_training_set(gpu, 96, width=RES, height=RES, input_spp=(1, 2, 4, 8, 16, 32, 64),
                                        ref_spp=2048,
                                        log=lambda m: say(m) if int(m.split()[1][:-1]) % 16 == 15 else None)
     net = L.LearnedDenoiser(gpu, seed=1)
-    losses = T.train(gpu, net, pairs, 3000, crop_w=RES, crop_h=RES, log_every=500, log=say)
+    losses = T.train(gpu, net, pairs, 6000, crop_w=RES, crop_h=RES, log_every=1000, log=say)
     del pairs
     assert np.all(np.isfinite(losses)) and np.mean(losses[-100:]) < 0.5 * np.mean(losses[:100])
     dn = L.Denoiser(gpu)
