@@ -344,6 +344,29 @@ lucida_status lucida_lbvh_build(lucida_ctx* ctx, const GpuTriangle* tris, size_t
                                 size_t num_positions, GpuBvhNode* nodes_out, size_t* num_nodes,
                                 uint32_t* prim_indices_out);
 
+/* No counterpart in the reference, whose GPU path traces the LBVH as built.
+ * The LBVH above, then its topology rebuilt for a lower surface-area cost
+ * (treelet restructuring, Karras and Aila, HPG 2013): `rounds` bottom-up
+ * passes, each of which replaces the treelet of up to LUCIDA_TREELET_LEAVES
+ * leaves under every node of at least LUCIDA_TREELET_MIN_SIZE primitives with
+ * the treelet topology of least cost, and subtrees of at most
+ * LUCIDA_TREELET_MAX_LEAF primitives become leaves where that costs no more.
+ * A node costs LUCIDA_TREELET_CI x its surface area plus its children's
+ * costs, a leaf LUCIDA_TREELET_CT x area x primitives. Output layout as
+ * lucida_lbvh_build's (children of an internal node adjacent, depth-first
+ * left-first order, leaves index prim_indices_out); rounds 0 is taken as 1.
+ * The closest hit along any ray is the one over the LBVH, up to ties at
+ * equal distance. */
+#define LUCIDA_TREELET_LEAVES 7
+#define LUCIDA_TREELET_MIN_SIZE 3
+#define LUCIDA_TREELET_MAX_LEAF 4
+#define LUCIDA_TREELET_CI 1.2f
+#define LUCIDA_TREELET_CT 1.0f
+#define LUCIDA_TREELET_ROUNDS 3
+lucida_status lucida_bvh_build_treelet(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
+                                       size_t num_positions, uint32_t rounds, GpuBvhNode* nodes_out,
+                                       size_t* num_nodes, uint32_t* prim_indices_out);
+
 /* Device-resident LBVH build: all arrays are device pointers. tris_sorted
  * receives the triangles permuted into leaf order (Bvh::compact_primitives,
  * crates/core/src/bvh.rs:517), so the leaves index triangles directly.
@@ -379,8 +402,10 @@ typedef struct lucida_scene_desc {
     const float* env_marginal;
     float env_total_weight;
     /* When nonzero and num_bvh_nodes == 0 with triangles present, the scene's
-     * BVH is built on the GPU by the LBVH and the triangles are permuted into
-     * leaf order on the device. */
+     * BVH is built on the GPU and the triangles are permuted into leaf order on
+     * the device: 1 the LBVH (lucida_lbvh_build's tree), 2 the LBVH
+     * restructured by treelets (lucida_bvh_build_treelet's tree,
+     * LUCIDA_TREELET_ROUNDS rounds). */
     uint32_t build_lbvh;
 } lucida_scene_desc;
 

@@ -131,6 +131,8 @@ class BlobParams(C.Structure):
 
 SAMPLING_BSDF, SAMPLING_NEE, SAMPLING_MIS = 0, 1, 2
 SAMPLER_INDEPENDENT, SAMPLER_SOBOL = 0, 1
+# lucida_bvh_build_treelet's default number of restructuring rounds
+TREELET_ROUNDS = 3
 SORT_NOALLOC_MAX_KEYS = 1 << 24  # LUCIDA_SORT_NOALLOC_MAX_KEYS
 
 # ---------------------------------------------------------------------------
@@ -176,6 +178,7 @@ def lib() -> C.CDLL:
             "lucida_sort_u32_device_noalloc": ([P, P, P, P, P, S], I),
             "lucida_lbvh_build_parts": ([P, P, S, P, S, C.POINTER(LbvhParts)], I),
             "lucida_lbvh_build": ([P, P, S, P, S, P, C.POINTER(S), P], I),
+            "lucida_bvh_build_treelet": ([P, P, S, P, S, U32, P, C.POINTER(S), P], I),
             "lucida_lbvh_build_device": ([P, P, S, P, P, C.POINTER(S), P, P, P], I),
             "lucida_scene_create": ([P, C.POINTER(SceneDesc), C.POINTER(P)], I),
             "lucida_scene_destroy": ([P], None),
@@ -435,14 +438,30 @@ class LbvhBuilder:
                                        C.byref(count), _ptr(prims)))
         return nodes[:count.value].copy(), prims[:n].copy()
 
+    def build_treelet(self, triangles: np.ndarray, positions: np.ndarray,
+                      rounds: int = TREELET_ROUNDS) -> tuple[np.ndarray, np.ndarray]:
+        """The LBVH restructured by treelets (lucida_bvh_build_treelet; no
+        reference counterpart): (nodes, prim_indices) in build()'s layout."""
+        tris = np.ascontiguousarray(triangles, dtype=TRIANGLE)
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 4)
+        n = len(tris)
+        nodes = np.empty(max(2 * n - 1, 1), BVH_NODE)
+        prims = np.empty(max(n, 1), np.uint32)
+        count = C.c_size_t()
+        _check(lib().lucida_bvh_build_treelet(self.gpu._h, _ptr(tris), n, _ptr(pos), len(pos), rounds, _ptr(nodes),
+                                              C.byref(count), _ptr(prims)))
+        return nodes[:count.value].copy(), prims[:n].copy()
+
 
 class Scene:
     """A device-resident scene (the buffers render_native_on uploads)."""
 
-    def __init__(self, gpu: Gpu, blob: Blob, build_lbvh: bool = True):
+    def __init__(self, gpu: Gpu, blob: Blob, build_lbvh: bool = True, treelet: bool = False):
+        """build_lbvh: build the tree on the device (else the blob's own nodes
+        are uploaded); treelet: restructure that LBVH by treelets."""
         self.gpu = gpu
         d = blob.desc()
-        d.build_lbvh = 1 if build_lbvh else 0
+        d.build_lbvh = (2 if treelet else 1) if build_lbvh else 0
         h = C.c_void_p()
         meshes, placements = blob.instancing()
         if len(placements):

@@ -300,7 +300,8 @@ lucida_status lucida_scene_create(lucida_ctx* ctx, const lucida_scene_desc* d, l
             st = upload(ctx, &s->bvh_nodes, d->bvh_nodes, d->num_bvh_nodes);
             s->num_bvh_nodes = d->num_bvh_nodes;
         } else if (d->build_lbvh && d->num_triangles) {
-            // LBVH on the device, then the triangles permuted into leaf order
+            // LBVH on the device (restructured by treelets when build_lbvh is
+            // 2), then the triangles permuted into leaf order
             // (Bvh::compact_primitives, bvh.rs:517) so leaves index them directly.
             size_t n = d->num_triangles;
             GpuTriangle* sorted = nullptr;
@@ -309,8 +310,11 @@ lucida_status lucida_scene_create(lucida_ctx* ctx, const lucida_scene_desc* d, l
                 st = set_error(LUCIDA_OOM, "scene BVH allocation failed");
             } else {
                 size_t num_nodes = 0;
-                st = lbvh_build_device(ctx, s->triangles, n, s->positions, s->bvh_nodes, &num_nodes, nullptr, sorted,
-                                       nullptr);
+                st = d->build_lbvh == 2
+                         ? bvh_build_treelet_device(ctx, s->triangles, n, s->positions, LUCIDA_TREELET_ROUNDS,
+                                                    s->bvh_nodes, &num_nodes, nullptr, sorted, nullptr)
+                         : lbvh_build_device(ctx, s->triangles, n, s->positions, s->bvh_nodes, &num_nodes, nullptr,
+                                             sorted, nullptr);
                 if (st == LUCIDA_OK) {
                     (void)hipStreamSynchronize(ctx->stream);
                     (void)hipFree(s->triangles);

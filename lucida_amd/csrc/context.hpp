@@ -77,6 +77,8 @@ struct lucida_ctx {
     // LBVH scratch, reused between builds.
     lucida::DevBuf lb_bounds, lb_partial, lb_codes, lb_vals, lb_tmpk, lb_tmpv, lb_hist, lb_karras, lb_parent,
         lb_nodebox, lb_subtree, lb_flags, lb_misc, lb_rank, lb_nodes, lb_tris, lb_pos, lb_prims, lb_level;
+    // Treelet restructuring: per-node costs, leaf choices, the leaf order.
+    lucida::DevBuf tl_cost, tl_leafy, tl_order;
     int lbvh_fit = 0;  // 0: two-phase fit (block-local, then the top), 1: atomic bottom-up, 2: level-synchronous
     // GGX directional-albedo table (energy compensation), 32 x 32 floats.
     lucida::DevBuf ggx_table;
@@ -126,6 +128,16 @@ lucida_status ensure_ggx_table(lucida_ctx* ctx);
 lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
                                 GpuBvhNode* nodes, size_t* num_nodes, uint32_t* prim_indices,
                                 GpuTriangle* tris_sorted, float* stage_us);
+// The LBVH build up to the fitted tree (lbvh.hip): Karras children, parents,
+// every node's box, subtree sizes and the sorted order in the context's lb_*
+// buffers; n >= 2.
+lucida_status lbvh_fit_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions);
+// The LBVH restructured by treelets (treelet.hip), laid out like
+// lbvh_build_device's output; stage_us (optional) receives the fit, the
+// restructuring rounds and the layout in microseconds.
+lucida_status bvh_build_treelet_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
+                                       uint32_t rounds, GpuBvhNode* nodes, size_t* num_nodes,
+                                       uint32_t* prim_indices, GpuTriangle* tris_sorted, float* stage_us);
 // Resolves the current device's copy of the sort's fixed scratch array
 // (lbvh.hip), once per device; lucida_ctx_create calls it with ctx->device
 // current.

@@ -1218,6 +1218,13 @@ static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size
     return LUCIDA_OK;
 }
 
+// The build up to the fitted tree, for passes that lay it out themselves:
+// the Karras children (lb_karras), parents (lb_parent), boxes of every node
+// (lb_nodebox), subtree sizes (lb_subtree) and the sorted order (lb_vals).
+lucida_status lbvh_fit_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions) {
+    return build_stages(ctx, tris, n, positions, nullptr, false);
+}
+
 // Relayout into GpuBvhNode on the device. The node count is 1 + 2 x the
 // root's emitted-internal count, read back by the caller (`inner` word 0)
 // once everything is queued.

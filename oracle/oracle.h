@@ -39,6 +39,19 @@ float oracle_sah_cost(const GpuBvhNode* nodes, size_t n);
 size_t oracle_validate(const GpuBvhNode* nodes, size_t n, const uint32_t* prim_indices, size_t num_prims,
                        size_t triangle_count);
 
+/* Treelet restructuring of the LBVH (treelet.cpp; no reference counterpart):
+ * the CPU statement of lucida_bvh_build_treelet. nodes [2n-1], prim_indices
+ * [n]; returns the node count. _with takes the cost model's constants. */
+size_t oracle_treelet_build(const GpuTriangle* tris, size_t n, const float* positions, uint32_t rounds,
+                            GpuBvhNode* nodes, uint32_t* prim_indices, float* sah_cost);
+size_t oracle_treelet_build_with(const GpuTriangle* tris, size_t n, const float* positions, uint32_t rounds,
+                                 float ci, float ct, uint32_t max_leaf, uint32_t min_size, GpuBvhNode* nodes,
+                                 uint32_t* prim_indices, float* sah_cost);
+/* The subset search over k <= 7 leaves: least cost, best split's left side
+ * and leaf choice per subset mask ([2^k] each; entry 0 unused). */
+void oracle_treelet_dp(int k, const LucidaAabb* leaf_box, const float* leaf_cost, const uint32_t* leaf_size,
+                       float* cost_out, uint8_t* part_out, uint8_t* leafy_out);
+
 /* Stable (key, value) sort on the CPU: the slice::sort of (key, index) pairs
  * the reference's radix tests compare against (crates/gpu/tests/radix_sort.rs). */
 void oracle_sort_pairs(const uint32_t* keys, const uint32_t* vals, size_t n, uint32_t* keys_out, uint32_t* vals_out);

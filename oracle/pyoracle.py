@@ -46,6 +46,9 @@ def lib() -> C.CDLL:
             "oracle_sah_cost": ([P, S], F),
             "oracle_validate": ([P, S, P, S, S], S),
             "oracle_sort_pairs": ([P, P, S, P, P], None),
+            "oracle_treelet_build": ([P, S, P, U32, P, P, P], S),
+            "oracle_treelet_build_with": ([P, S, P, U32, F, F, U32, U32, P, P, P], S),
+            "oracle_treelet_dp": ([I, P, P, P, P, P, P], None),
             "oracle_sobol": ([U32, U32], U32),
             "oracle_owen_scramble": ([U32, U32], U32),
             "oracle_hash_u32": ([U32], U32),
@@ -140,6 +143,34 @@ def lbvh_build(tris, positions, max_leaf: int = 4) -> dict:
         r["subtree_size"] = r["subtree_size"][:0]
     r["sah_cost"] = sah.value
     return r
+
+
+def treelet_build(tris, positions, rounds: int = 3) -> dict:
+    """The LBVH restructured by treelets (oracle/treelet.cpp; no reference
+    counterpart): nodes, prim_indices and the SAH cost."""
+    tris = np.ascontiguousarray(tris, TRIANGLE)
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 4)
+    n = len(tris)
+    nodes = np.empty(max(2 * n - 1, 1), BVH_NODE)
+    prims = np.empty(max(n, 1), np.uint32)
+    sah = C.c_float()
+    count = lib().oracle_treelet_build(_p(tris), n, _p(pos), rounds, _p(nodes), _p(prims), C.byref(sah))
+    return {"nodes": nodes[:count].copy(), "prim_indices": prims[:n].copy(), "sah_cost": sah.value}
+
+
+def treelet_dp(boxes, costs, sizes) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """The treelet subset search over k <= 7 leaves: (least cost, best split's
+    left side, leaf choice) per subset mask."""
+    boxes = np.ascontiguousarray(boxes, AABB)
+    costs = np.ascontiguousarray(costs, np.float32)
+    sizes = np.ascontiguousarray(sizes, np.uint32)
+    k = len(boxes)
+    assert 1 <= k <= 7 and len(costs) == k and len(sizes) == k
+    c = np.empty(1 << k, np.float32)
+    part = np.empty(1 << k, np.uint8)
+    leafy = np.empty(1 << k, np.uint8)
+    lib().oracle_treelet_dp(k, _p(boxes), _p(costs), _p(sizes), _p(c), _p(part), _p(leafy))
+    return c, part, leafy
 
 
 def sah_cost(nodes) -> float:
