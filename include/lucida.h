@@ -358,7 +358,7 @@ lucida_status lucida_lbvh_build(lucida_ctx* ctx, const GpuTriangle* tris, size_t
  * The closest hit along any ray is the one over the LBVH, up to ties at
  * equal distance. */
 #define LUCIDA_TREELET_LEAVES 7
-#define LUCIDA_TREELET_MIN_SIZE 3
+#define LUCIDA_TREELET_MIN_SIZE 7
 #define LUCIDA_TREELET_MAX_LEAF 4
 #define LUCIDA_TREELET_CI 1.2f
 #define LUCIDA_TREELET_CT 1.0f

@@ -78,7 +78,7 @@ struct lucida_ctx {
     lucida::DevBuf lb_bounds, lb_partial, lb_codes, lb_vals, lb_tmpk, lb_tmpv, lb_hist, lb_karras, lb_parent,
         lb_nodebox, lb_subtree, lb_flags, lb_misc, lb_rank, lb_nodes, lb_tris, lb_pos, lb_prims, lb_level;
     // Treelet restructuring: per-node costs, leaf choices, the leaf order.
-    lucida::DevBuf tl_cost, tl_leafy, tl_order;
+    lucida::DevBuf tl_cost, tl_leafy, tl_order, tl_list;
     int lbvh_fit = 0;  // 0: two-phase fit (block-local, then the top), 1: atomic bottom-up, 2: level-synchronous
     // GGX directional-albedo table (energy compensation), 32 x 32 floats.
     lucida::DevBuf ggx_table;

@@ -5,12 +5,12 @@
 // HPG 2013.
 //
 // After the bit-exact LBVH (Morton sort, Karras hierarchy, fitted boxes), each
-// round climbs the tree bottom-up, one lane per primitive: at every internal
-// node the first arrival stops and the second continues, so a node is handled
-// once both of its subtrees are final. A node of at least
-// LUCIDA_TREELET_MIN_SIZE primitives is handled by its whole wave: it grows a
-// treelet of up to 7 leaves under the node (always opening the leaf of
-// largest surface area), finds the treelet topology of least cost by dynamic
+// round handles every internal node once its subtrees are final: nodes are
+// bucketed by depth and one launch per level, deepest first, handles that
+// level. A node of at least LUCIDA_TREELET_MIN_SIZE primitives gets a whole
+// wave, which grows a treelet of up to 7 leaves under the node (always
+// opening the leaf of largest surface area), finds the treelet topology of
+// least cost by dynamic
 // programming over the 127 leaf subsets (subset boxes and counts in LDS, one
 // subset size after another), and rewrites the treelet's internal nodes in
 // place. Smaller nodes only take their cost from their children. A node
@@ -48,11 +48,35 @@ struct TreeletLds {
     float cost[SUBSETS];
     uint32_t count[SUBSETS];
     uint32_t part[SUBSETS];  // left side of the best split | LEAF_BIT
+    uint32_t icnt[SUBSETS];  // internal nodes the layout emits for the subset's best subtree
     uint32_t leaf[8];        // the treelet's leaves
     uint32_t inner[8];       // its internal nodes, the root first
     uint32_t stack_set[8], stack_slot[8];
-    uint32_t rec_slot[8], rec_left[8], rec_right[8], rec_leaf[8];
 };
+
+// The 127 nonempty subsets of 7 leaves ordered by size, then mask; off[m] is
+// where the subsets of m leaves start.
+struct SubsetTable {
+    uint8_t mask[SUBSETS - 1];
+    uint8_t off[LEAVES + 2];
+};
+constexpr int popcount7(int v) { return (v & 1) + (v >> 1 & 1) + (v >> 2 & 1) + (v >> 3 & 1) + (v >> 4 & 1) + (v >> 5 & 1) + (v >> 6 & 1); }
+constexpr SubsetTable make_subset_table() {
+    SubsetTable t{};
+    int n = 0;
+    for (int m = 1; m <= LEAVES; ++m) {
+        t.off[m] = (uint8_t)n;
+        for (int v = 1; v < (int)SUBSETS; ++v)
+            if (popcount7(v) == m) t.mask[n++] = (uint8_t)v;
+    }
+    t.off[LEAVES + 1] = (uint8_t)n;
+    return t;
+}
+__constant__ SubsetTable c_subsets = make_subset_table();
+// Lanes that share one subset's partitions, as a shift, per subset size: the
+// subsets of one size and their 2^(m-1) - 1 partitions spread over the wave
+// (21 x 1, 35 x 1, 35 x 1, 21 x 2, 7 x 8 and 1 x 64 lanes).
+constexpr uint32_t GROUP_SHIFT[LEAVES + 1] = {0, 0, 0, 0, 0, 1, 3, 6};
 
 __device__ inline float area(float lx, float ly, float lz, float hx, float hy, float hz) {
     const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
@@ -86,38 +110,95 @@ struct TreeArrays {
 __device__ inline uint32_t size_of(const TreeArrays& t, uint32_t id) { return id >= t.leaf_base ? 1u : t.size[id]; }
 __device__ inline uint32_t inner_of(const TreeArrays& t, uint32_t id) { return id >= t.leaf_base ? 0u : t.inner[id]; }
 
-// The least-cost subtree over subset s of the treelet's leaves, from the
-// costs of its proper subsets (already in LDS). Partitions keep the lowest
-// leaf on the left and are tried in ascending order of that side's mask.
-__device__ inline void solve_subset(TreeletLds& L, uint32_t s) {
-    const uint32_t low = s & (0u - s), rest = s ^ low;
-    float best = __builtin_huge_valf();
-    uint32_t best_p = low;  // a valid split even if no cost compares below infinity
-    for (uint32_t q = 0; q != rest; q = (q - rest) & rest) {
-        const uint32_t pm = low | q;
-        const float cand = L.cost[pm] + L.cost[s ^ pm];
-        if (cand < best) {
-            best = cand;
-            best_p = pm;
+// The bits of j placed, lowest first, at the set bits of mask.
+__device__ inline uint32_t deposit(uint32_t j, uint32_t mask) {
+    uint32_t out = 0;
+    for (uint32_t b = 1; mask; b <<= 1) {
+        const uint32_t low = mask & (0u - mask);
+        if (j & b) out |= low;
+        mask ^= low;
+    }
+    return out;
+}
+
+// The least-cost subtrees over all subsets of m leaves, from the costs of
+// their proper subsets (already in LDS). A subset's partitions keep its
+// lowest leaf on the left; partition j's left side is that leaf plus the
+// bits of j deposited on the others, so ascending j is ascending mask. The
+// lanes sharing a subset take every G-th partition, keep their first least
+// cost, and then agree on the least (cost, mask) pair: the first least cost
+// in ascending mask order. Costs that do not compare below infinity (NaN) are
+// never chosen; with none below infinity the split is the lowest leaf alone.
+__device__ inline void solve_size(TreeletLds& L, uint32_t lane, int m, uint32_t full) {
+    const uint32_t gs = GROUP_SHIFT[m];
+    const uint32_t g = lane >> gs, r = lane & ((1u << gs) - 1u);
+    const uint32_t first = c_subsets.off[m], nsub = c_subsets.off[m + 1] - first;
+    const float inf = __builtin_huge_valf();
+    float key = inf;
+    uint32_t bp = 0xFFu, s = 0;
+    if (g < nsub) {
+        s = c_subsets.mask[first + g];
+        if (s <= full) {
+            const uint32_t low = s & (0u - s), rest = s ^ low;
+            const uint32_t parts = (1u << (m - 1)) - 1u;
+            for (uint32_t j = r; j < parts; j += 1u << gs) {
+                const uint32_t pm = low | deposit(j, rest);
+                const float cand = L.cost[pm] + L.cost[s ^ pm];
+                if (cand < key) {
+                    key = cand;
+                    bp = pm;
+                }
+            }
         }
     }
-    const float a = area(L.box[0][s], L.box[1][s], L.box[2][s], L.box[3][s], L.box[4][s], L.box[5][s]);
-    const float inner = CI * a + best;
-    const float leaf = CT * a * (float)L.count[s];
-    const bool as_leaf = L.count[s] <= MAX_LEAF && leaf <= inner;
-    L.cost[s] = as_leaf ? leaf : inner;
-    L.part[s] = best_p | (as_leaf ? LEAF_BIT : 0u);
+    for (uint32_t o = 1; o < (1u << gs); o <<= 1) {
+        const float ok = __shfl_xor(key, o, 64);
+        const uint32_t op = __shfl_xor(bp, o, 64);
+        if (ok < key || (ok == key && op < bp)) {
+            key = ok;
+            bp = op;
+        }
+    }
+    if (r == 0 && g < nsub && s <= full) {
+        if (bp == 0xFFu) bp = s & (0u - s);
+        const float a = area(L.box[0][s], L.box[1][s], L.box[2][s], L.box[3][s], L.box[4][s], L.box[5][s]);
+        const float inner = CI * a + key;
+        const float leaf = CT * a * (float)L.count[s];
+        const bool as_leaf = L.count[s] <= MAX_LEAF && leaf <= inner;
+        L.cost[s] = as_leaf ? leaf : inner;
+        L.part[s] = bp | (as_leaf ? LEAF_BIT : 0u);
+        L.icnt[s] = as_leaf ? 0u : 1u + L.icnt[bp] + L.icnt[s ^ bp];
+    }
 }
 
 // The whole wave rebuilds the treelet under node x (uniform).
 __device__ void restructure(const TreeArrays& t, TreeletLds& L, uint32_t lane, uint32_t x) {
     wave_sync();  // the previous treelet's LDS reads are done
-    // Grow: lane i < k holds leaf i; the leaf of largest area (the first
-    // such) is replaced by its left child and its right child appended.
+    // The three levels under x, fetched across lanes first: heap slot p < 14
+    // (lane p) holds a node id and its area (-1 for a primitive, which is
+    // never opened); slots 0 and 1 are x's children and slot p >= 2 is child
+    // p & 1 of slot (p - 2) / 2.
     const uint2 c0 = t.child[x];
-    uint32_t id = lane == 0 ? c0.x : (lane == 1 ? c0.y : NONE);
-    float id_area = -1.0f;  // internal leaves only; a primitive is never opened
-    if (lane < 2 && id < t.leaf_base) id_area = node_area(t.box, id);
+    uint32_t hid = lane == 0 ? c0.x : (lane == 1 ? c0.y : NONE);
+    for (uint32_t lo = 2, hi = 6; lo < 14; lo = hi, hi = 2 * hi + 2) {
+        const uint32_t par = __shfl(hid, lane >= 2 ? (lane - 2) >> 1 : 0u, 64);
+        if (lane >= lo && lane < hi && par < t.leaf_base) {
+            const uint2 c = t.child[par];
+            hid = (lane & 1u) ? c.y : c.x;
+        }
+    }
+    const float harea = lane < 14 && hid < t.leaf_base ? node_area(t.box, hid) : -1.0f;
+    // Grow: lane i < k holds leaf i (id, area, heap slot or NONE); the leaf of
+    // largest area (the first such) is replaced by its left child and its
+    // right child appended.
+    uint32_t id = __shfl(hid, lane < 2 ? lane : 0u, 64);
+    float id_area = __shfl(harea, lane < 2 ? lane : 0u, 64);
+    uint32_t hpos = lane;
+    if (lane >= 2) {
+        id = NONE;
+        id_area = -1.0f;
+        hpos = NONE;
+    }
     uint32_t inner_id = lane == 0 ? x : NONE;
     int k = 2;
     while (k < LEAVES) {
@@ -132,15 +213,33 @@ __device__ void restructure(const TreeArrays& t, TreeletLds& L, uint32_t lane, u
         }
         if (best < 0) break;
         const uint32_t e = __shfl(id, best, 64);
-        const uint2 ce = t.child[e];
+        const uint32_t ep = __shfl(hpos, best, 64);
+        uint32_t cx, cy, px = NONE, py = NONE;
+        float ax, ay;
+        if (ep < 6) {  // its children are in the heap
+            px = 2 * ep + 2;
+            py = px + 1;
+            cx = __shfl(hid, px, 64);
+            cy = __shfl(hid, py, 64);
+            ax = __shfl(harea, px, 64);
+            ay = __shfl(harea, py, 64);
+        } else {
+            const uint2 ce = t.child[e];
+            cx = ce.x;
+            cy = ce.y;
+            ax = cx < t.leaf_base ? node_area(t.box, cx) : -1.0f;
+            ay = cy < t.leaf_base ? node_area(t.box, cy) : -1.0f;
+        }
         if (lane == (uint32_t)(k - 1)) inner_id = e;
         if (lane == (uint32_t)best) {
-            id = ce.x;
-            id_area = ce.x < t.leaf_base ? node_area(t.box, ce.x) : -1.0f;
+            id = cx;
+            id_area = ax;
+            hpos = px;
         }
         if (lane == (uint32_t)k) {
-            id = ce.y;
-            id_area = ce.y < t.leaf_base ? node_area(t.box, ce.y) : -1.0f;
+            id = cy;
+            id_area = ay;
+            hpos = py;
         }
         ++k;
     }
@@ -155,6 +254,7 @@ __device__ void restructure(const TreeArrays& t, TreeletLds& L, uint32_t lane, u
         my_cost = t.cost[id];
         my_size = size_of(t, id);
         L.leaf[lane] = id;
+        L.icnt[1u << lane] = inner_of(t, id);
     }
     if (lane < (uint32_t)(k - 1)) L.inner[lane] = inner_id;
     // Subset boxes and counts: lane l covers subsets l + 1 and l + 65; each
@@ -197,17 +297,14 @@ __device__ void restructure(const TreeArrays& t, TreeletLds& L, uint32_t lane, u
     }
     wave_sync();
     // Least costs, subsets of 2 leaves first, then 3, ...
-    const int m0 = __popc(s0), m1 = __popc(s1);
     for (int m = 2; m <= k; ++m) {
-        if (s0 <= full && m0 == m) solve_subset(L, s0);
-        if (s1 <= full && m1 == m) solve_subset(L, s1);
+        solve_size(L, lane, m, full);
         wave_sync();
     }
     // Rebuild in pre-order, left side first, reusing the treelet's internal
-    // nodes in the order they were taken; then the emitted-internal counts in
-    // reverse pre-order (children before parents).
+    // nodes in the order they were taken.
     if (lane == 0) {
-        uint32_t sp = 1, next = 1, nrec = 0;
+        uint32_t sp = 1, next = 1;
         L.stack_set[0] = full;
         L.stack_slot[0] = x;
         while (sp) {
@@ -225,13 +322,8 @@ __device__ void restructure(const TreeArrays& t, TreeletLds& L, uint32_t lane, u
             bp[1] = make_float4(L.box[3][s], L.box[4][s], L.box[5][s], 0.0f);
             t.size[slot] = L.count[s];
             t.cost[slot] = L.cost[s];
-            const uint32_t as_leaf = (pw & LEAF_BIT) ? 1u : 0u;
-            t.leafy[slot] = as_leaf;
-            L.rec_slot[nrec] = slot;
-            L.rec_left[nrec] = lc;
-            L.rec_right[nrec] = rc;
-            L.rec_leaf[nrec] = as_leaf;
-            ++nrec;
+            t.leafy[slot] = (pw & LEAF_BIT) ? 1u : 0u;
+            t.inner[slot] = L.icnt[s];
             if (__popc(qm) > 1) {
                 L.stack_set[sp] = qm;
                 L.stack_slot[sp] = rc;
@@ -243,71 +335,99 @@ __device__ void restructure(const TreeArrays& t, TreeletLds& L, uint32_t lane, u
                 ++sp;
             }
         }
-        for (int r = (int)nrec - 1; r >= 0; --r)
-            t.inner[L.rec_slot[r]] =
-                L.rec_leaf[r] ? 0u : 1u + inner_of(t, L.rec_left[r]) + inner_of(t, L.rec_right[r]);
     }
 }
 
-// Longest climb a valid tree allows (its depth is far below this); a longer
-// one means broken parent links, and the pass stops and reports it.
+// Longest walk to the root a valid tree allows (its depth is far below this);
+// a longer one means broken parent links, and the pass stops and reports it.
 constexpr uint32_t MAX_CLIMB = 1u << 16;
+constexpr uint32_t MAX_LEVELS = 128;  // depth limit of the level schedule
 
-// One restructuring round. Each lane starts at one primitive; the second
-// arrival at a node continues (agent-scope release before the arrival count,
-// acquire after it, as the subtrees were written by other waves, possibly on
-// other XCDs). Every lane's climb ends at its first arrival or past the root,
-// so the loop runs at most the tree's depth times.
-__global__ __launch_bounds__(TL_BLOCK) void k_treelet_round(TreeArrays t, uint32_t* __restrict__ arrivals,
-                                                            uint32_t* __restrict__ fault) {
-    __shared__ TreeletLds lds[TL_WAVES];
-    TreeletLds& L = lds[threadIdx.x / 64];
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t k = blockIdx.x * TL_BLOCK + threadIdx.x;
-    bool active = k < t.n;
-    uint32_t cur = NONE;
-    if (active) {
-        const uint32_t id = t.leaf_base + k;
-        t.cost[id] = CT * node_area(t.box, id);
-        cur = t.parent[id];
-        active = cur != NONE;
-    }
-    for (uint32_t step = 0;; ++step) {
-        if (step == MAX_CLIMB) {  // uniform: every lane of the wave counts the same steps
-            if (lane == 0) atomicOr(fault, 1u);
-            break;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (active) {
-            const uint32_t old = __hip_atomic_fetch_add(&arrivals[cur], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            active = old != 0;  // the first arrival stops; the sibling's lane finishes this node
-        }
-        if (__ballot(active) == 0) break;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const bool big = active && t.size[cur] >= MIN_SIZE;
-        uint64_t todo = __ballot(big);
-        while (todo) {
-            const int l = __ffsll((unsigned long long)todo) - 1;
-            todo &= todo - 1;
-            restructure(t, L, lane, __shfl(cur, l, 64));
-        }
-        if (active && !big) {
-            const uint2 c = t.child[cur];
-            const float a = node_area(t.box, cur);
-            const float split = t.cost[c.x] + t.cost[c.y];
-            const float inner = CI * a + split;
-            const uint32_t sz = t.size[cur];
-            const float leaf = CT * a * (float)sz;
-            const bool as_leaf = sz <= MAX_LEAF && leaf <= inner;
-            t.cost[cur] = as_leaf ? leaf : inner;
-            t.leafy[cur] = as_leaf ? 1u : 0u;
-            t.inner[cur] = as_leaf ? 0u : 1u + inner_of(t, c.x) + inner_of(t, c.y);
-        }
-        if (active) {
+// One node's cost, leaf choice and emitted-internal count from its children's.
+__device__ inline void combine(const TreeArrays& t, uint32_t x) {
+    const uint2 c = t.child[x];
+    const float a = node_area(t.box, x);
+    const float split = t.cost[c.x] + t.cost[c.y];
+    const float inner = CI * a + split;
+    const uint32_t sz = t.size[x];
+    const float leaf = CT * a * (float)sz;
+    const bool as_leaf = sz <= MAX_LEAF && leaf <= inner;
+    t.cost[x] = as_leaf ? leaf : inner;
+    t.leafy[x] = as_leaf ? 1u : 0u;
+    t.inner[x] = as_leaf ? 0u : 1u + inner_of(t, c.x) + inner_of(t, c.y);
+}
+
+// A round is scheduled by depth: every internal node's depth (its walk to the
+// root) and class (treelet root or not) are counted per level, the nodes are
+// bucketed into per-level lists, and one launch per level, deepest first,
+// handles that level's nodes. A node's descendants are all deeper, so they
+// are final when it is handled, and rebuilding a treelet touches only nodes
+// below its root; launch boundaries order the levels, so no cross-wave
+// synchronisation is needed inside a launch.
+__global__ __launch_bounds__(256) void k_tl_depth(TreeArrays t, uint32_t* __restrict__ depth,
+                                                  uint32_t* __restrict__ level_count, uint32_t* __restrict__ fault) {
+    __shared__ uint32_t h[2 * MAX_LEVELS];
+    for (int i = threadIdx.x; i < 2 * (int)MAX_LEVELS; i += 256) h[i] = 0;
+    __syncthreads();
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x < t.n - 1) {
+        uint32_t d = 0, cur = x;
+        while (cur != 0 && d < MAX_LEVELS) {
             cur = t.parent[cur];
-            active = cur != NONE;
+            ++d;
         }
+        if (d >= MAX_LEVELS) {
+            atomicOr(fault, 8u);
+            d = MAX_LEVELS - 1;
+        }
+        const uint32_t key = 2 * d + (t.size[x] >= MIN_SIZE ? 1u : 0u);
+        depth[x] = key;
+        atomicAdd(&h[key], 1u);
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * (int)MAX_LEVELS; i += 256)
+        if (h[i]) atomicAdd(&level_count[i], h[i]);
+}
+
+// Scatters node ids into the per-(level, class) lists; cursor[key] starts at
+// that list's offset.
+__global__ __launch_bounds__(256) void k_tl_bucket(uint32_t internal, const uint32_t* __restrict__ depth,
+                                                   uint32_t* __restrict__ cursor, uint32_t* __restrict__ list) {
+    __shared__ uint32_t h[2 * MAX_LEVELS], base[2 * MAX_LEVELS];
+    for (int i = threadIdx.x; i < 2 * (int)MAX_LEVELS; i += 256) h[i] = 0;
+    __syncthreads();
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    uint32_t key = 0, local = 0;
+    if (x < internal) {
+        key = depth[x];
+        local = atomicAdd(&h[key], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * (int)MAX_LEVELS; i += 256) base[i] = h[i] ? atomicAdd(&cursor[i], h[i]) : 0u;
+    __syncthreads();
+    if (x < internal) list[base[key] + local] = x;
+}
+
+// One level: the first `waves` waves each rebuild the treelet of one node of
+// `big` (nodes of at least MIN_SIZE primitives); the lanes of the remaining
+// waves each combine one node of `small`.
+__global__ __launch_bounds__(TL_BLOCK) void k_tl_level(TreeArrays t, const uint32_t* __restrict__ big,
+                                                       uint32_t waves, const uint32_t* __restrict__ small,
+                                                       uint32_t count) {
+    __shared__ TreeletLds lds[TL_WAVES];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w = blockIdx.x * TL_WAVES + threadIdx.x / 64;
+    if (w < waves) {
+        restructure(t, lds[threadIdx.x / 64], lane, big[w]);
+        return;
+    }
+    const uint32_t i = (w - waves) * 64 + lane;
+    if (i < count) combine(t, small[i]);
+}
+
+__global__ __launch_bounds__(256) void k_tl_leaf_cost(TreeArrays t) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k < t.n) t.cost[t.leaf_base + k] = CT * node_area(t.box, t.leaf_base + k);
 }
 
 __global__ __launch_bounds__(256) void k_clear_words(uint32_t* __restrict__ p, uint32_t count) {
@@ -415,13 +535,40 @@ lucida_status bvh_build_treelet_device(lucida_ctx* ctx, const GpuTriangle* tris,
     t.inner = t.size + internal;
     t.cost = ctx->tl_cost.as<float>();
     t.leafy = ctx->tl_leafy.as<uint32_t>();
-    uint32_t* arrivals = ctx->lb_flags.as<uint32_t>();
     uint32_t* fault = t.leafy + internal;
+    LUCIDA_HIP(ctx->tl_list.ensure((internal + 4 * MAX_LEVELS) * 4), LUCIDA_OOM);
+    uint32_t* list = ctx->tl_list.as<uint32_t>();
+    uint32_t* level_count = list + internal;
+    uint32_t* cursor = level_count + 2 * MAX_LEVELS;
+    uint32_t* key = ctx->lb_flags.as<uint32_t>();  // per internal node: 2 x depth + treelet root
     hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(256), 0, s, fault, 1u);
+    hipLaunchKernelGGL(k_tl_leaf_cost, dim3(blocks_for(n, 256)), dim3(256), 0, s, t);
     for (uint32_t r = 0; r < (rounds ? rounds : 1u); ++r) {
-        hipLaunchKernelGGL(k_clear_words, dim3(blocks_for(internal, 256)), dim3(256), 0, s, arrivals,
-                           (uint32_t)internal);
-        hipLaunchKernelGGL(k_treelet_round, dim3(blocks_for(n, TL_BLOCK)), dim3(TL_BLOCK), 0, s, t, arrivals, fault);
+        hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(256), 0, s, level_count, 2 * MAX_LEVELS);
+        hipLaunchKernelGGL(k_tl_depth, dim3(blocks_for(internal, 256)), dim3(256), 0, s, t, key, level_count, fault);
+        uint32_t host[2 * MAX_LEVELS + 1];
+        LUCIDA_HIP(hipMemcpyAsync(host, level_count, 2 * MAX_LEVELS * 4, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
+        LUCIDA_HIP(hipMemcpyAsync(host + 2 * MAX_LEVELS, fault, 4, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
+        LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
+        if (host[2 * MAX_LEVELS]) return set_error(LUCIDA_VALIDATION, "treelet BVH: deeper than 128 levels");
+        uint32_t off[2 * MAX_LEVELS], acc = 0;
+        int deepest = 0;
+        for (uint32_t k = 0; k < 2 * MAX_LEVELS; ++k) {
+            off[k] = acc;
+            acc += host[k];
+            if (host[k]) deepest = (int)(k / 2);
+        }
+        // Blocking copy: the stream is idle here (synchronised above) and `off` is a stack array.
+        LUCIDA_HIP(hipMemcpy(cursor, off, sizeof off, hipMemcpyHostToDevice), LUCIDA_VALIDATION);
+        hipLaunchKernelGGL(k_tl_bucket, dim3(blocks_for(internal, 256)), dim3(256), 0, s, (uint32_t)internal, key,
+                           cursor, list);
+        for (int d = deepest; d >= 0; --d) {
+            const uint32_t nsmall = host[2 * d], nbig = host[2 * d + 1];
+            if (!nsmall && !nbig) continue;
+            const uint32_t waves = nbig + (nsmall + 63) / 64;
+            hipLaunchKernelGGL(k_tl_level, dim3(blocks_for(waves, TL_WAVES)), dim3(TL_BLOCK), 0, s, t,
+                               list + off[2 * d + 1], nbig, list + off[2 * d], nsmall);
+        }
     }
     if (stage_us) LUCIDA_HIP(hipEventRecord(ev[2], s), LUCIDA_VALIDATION);
     uint32_t* order = prim_indices ? prim_indices : ctx->tl_order.as<uint32_t>();
