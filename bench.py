@@ -3,8 +3,10 @@
 Workload (one "step"): render the noise-displaced geodesic sphere (1,048,820
 triangles) in its 1,024-triangle room lit by 16 emissive quads, at 1024x1024,
 16 samples per pixel, 8 bounces, NEE + MIS, Owen-scrambled Sobol, into a
-cleared film. The scene, its GPU-built LBVH and the film are resident in HBM
-before timing starts. value = paths per second over the timed steps.
+cleared film. The scene, its GPU-built tree (the LBVH restructured by
+treelets; --tree lbvh keeps the LBVH as built) and the film are resident in
+HBM before timing starts. value = paths per second over the timed steps; the
+same step over the LBVH as built is timed beside it (extra.lbvh_tree_render).
 
 Prints one JSON line. Extra figures (Mrays/s, LBVH Mtris/s, a-trous time,
 per-kernel times) ride in "extra". "time_to_relmse" holds the metric's second
@@ -56,8 +58,9 @@ def profile_figure(path: Path, key: str):
     return None
 
 
-def lbvh_build_rate(L, gpu, blob, runs: int = 7) -> dict:
-    """Device LBVH build of the scene's triangles (inputs already on the GPU)."""
+def lbvh_build_rate(L, gpu, blob, runs: int = 7, treelet: bool = False) -> dict:
+    """Device build of the scene's tree (inputs already on the GPU): the LBVH,
+    or the LBVH restructured by treelets."""
     a = blob.arrays()
     tris = np.ascontiguousarray(a["triangles"])
     pos = np.ascontiguousarray(a["positions"])
@@ -78,8 +81,13 @@ def lbvh_build_rate(L, gpu, blob, runs: int = 7) -> dict:
         st = np.zeros(5, np.float32)
         gpu.check_errors()
         t0 = time.perf_counter()
-        L._check(lib.lucida_lbvh_build_device(gpu._h, ptrs["tris"], n, ptrs["pos"], ptrs["nodes"], C.byref(count),
-                                              ptrs["prims"], ptrs["sorted"], st.ctypes.data))
+        if treelet:
+            L._check(lib.lucida_bvh_build_treelet_device(gpu._h, ptrs["tris"], n, ptrs["pos"], L.TREELET_ROUNDS,
+                                                         ptrs["nodes"], C.byref(count), ptrs["prims"],
+                                                         ptrs["sorted"], st.ctypes.data))
+        else:
+            L._check(lib.lucida_lbvh_build_device(gpu._h, ptrs["tris"], n, ptrs["pos"], ptrs["nodes"],
+                                                  C.byref(count), ptrs["prims"], ptrs["sorted"], st.ctypes.data))
         gpu.check_errors()
         t1 = time.perf_counter()
         if i > 0:
@@ -90,6 +98,12 @@ def lbvh_build_rate(L, gpu, blob, runs: int = 7) -> dict:
     med = float(np.median(times))
     st = np.median(np.stack(stages), axis=0)
     device_us = float(st.sum())
+    if treelet:
+        return {"triangles": n, "nodes": int(count.value), "rounds": L.TREELET_ROUNDS, "wall_ms": round(med * 1e3, 4),
+                "device_us": round(device_us, 1), "runs": runs, "statistic": "median",
+                "stage_us": dict(zip(("lbvh_to_fitted_tree", "treelet_rounds", "layout"),
+                                     [round(float(x), 1) for x in st[:3]])),
+                "mtris_per_s": round(n / (device_us * 1e-6) / 1e6, 1), "mtris_per_s_wall": round(n / med / 1e6, 1)}
     gbs = LBVH_BYTES_PER_TRI * n / (device_us * 1e-6) / 1e9
     return {"triangles": n, "nodes": int(count.value), "wall_ms": round(med * 1e3, 4),
             "device_us": round(device_us, 1), "runs": runs, "statistic": "median",
@@ -361,6 +375,10 @@ def main() -> int:
                     help="0/2: the binary walk (default), 4: the 4-wide walk")
     ap.add_argument("--ray-order", type=int, default=0, choices=(0, 1, 2),
                     help="extension-queue order: 0 automatic (sorted for this scene), 1 kept, 2 always sorted")
+    ap.add_argument("--tree", choices=("treelet", "lbvh"), default="treelet",
+                    help="the scene's tree: the LBVH restructured by treelets, or the LBVH as built")
+    ap.add_argument("--no-lbvh-compare", action="store_true",
+                    help="skip timing the step over the LBVH as built (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--relmse-ref-spp", type=int, default=4096)
@@ -429,7 +447,9 @@ def main() -> int:
         gpu.close()
         return 0
     lbvh = lbvh_build_rate(L, gpu, blob)
-    scene = L.Scene(gpu, blob, build_lbvh=True)
+    treelet = a.tree == "treelet"
+    treelet_build = lbvh_build_rate(L, gpu, blob, treelet=True) if treelet else None
+    scene = L.Scene(gpu, blob, build_lbvh=True, treelet=treelet)
     nodes, tris_sorted = scene.download_bvh()
     blob.set_bvh(nodes, tris_sorted)  # the CPU baseline traverses the same tree
     film = L.Film(gpu, a.width, a.height)
@@ -460,6 +480,27 @@ def main() -> int:
     gpu.check_errors()
     total_s = time.perf_counter() - t0
     sec = float(np.median(step_s))
+
+    # The same step over the LBVH as built (the reference's tree), timed the
+    # same way, beside the restructured tree's.
+    lbvh_tree_render = None
+    if treelet and not a.no_lbvh_compare:
+        # a fresh blob: `blob` now carries the restructured nodes for the CPU baseline
+        plain = L.Scene(gpu, L.Blob("sphere-c2", frequency=a.frequency), build_lbvh=True)
+        film.clear()
+        L.render_wavefront(gpu, plain, u, a.spp, film, cfg)
+        plain_s = []
+        for _ in range(3):
+            film.clear()
+            ts = time.perf_counter()
+            L.render_wavefront(gpu, plain, u, a.spp, film, cfg)
+            plain_s.append(time.perf_counter() - ts)
+        gpu.check_errors()
+        plain_ms = float(np.median(plain_s)) * 1e3
+        lbvh_tree_render = {"ms_per_step": round(plain_ms, 3), "steps": 3, "statistic": "median",
+                            "mpaths_per_s": round(a.width * a.height * a.spp / (plain_ms * 1e-3) / 1e6, 3),
+                            "treelet_speedup": round(plain_ms / (sec * 1e3), 4)}
+        del plain
 
     def kernel_times(serialize: int) -> dict:
         film.clear()
@@ -530,7 +571,9 @@ def main() -> int:
         "config": {"workload": f"sphere-c2: displaced geodesic sphere {ntris - 1024} tris + 1,024-tri room, "
                                f"16 emissive quads, {a.width}x{a.height}, {a.spp} spp, {a.depth} bounces, "
                                f"NEE+MIS, Sobol", "triangles": ntris, "width": a.width, "height": a.height,
-                   "spp": a.spp, "max_depth": a.depth, "parallelism": "single GPU"},
+                   "spp": a.spp, "max_depth": a.depth, "parallelism": "single GPU",
+                   "tree": (f"LBVH restructured by treelets ({L.TREELET_ROUNDS} rounds), built on the GPU"
+                            if treelet else "LBVH as built on the GPU")},
         "timing": {"statistic": "median of the timed steps' wall times (each step a complete render)",
                    "mean_ms_per_step": round(total_s / a.steps * 1e3, 3),
                    "min_ms": round(min(step_s) * 1e3, 3), "max_ms": round(max(step_s) * 1e3, 3),
@@ -566,6 +609,8 @@ def main() -> int:
                                  "avg_launch_ms": round(con_solo_ms, 4), "bytes_per_launch": int(con_bytes),
                                  "measured": measured("connect", con_solo_ms)},
             "lbvh": lbvh,
+            "treelet_build": treelet_build,
+            "lbvh_tree_render": lbvh_tree_render,
             "atrous": {"ms": round(atrous_ms, 3), "bytes_per_px": ATROUS_BYTES_PER_PX,
                        "achieved_gbs": round(atrous_gbs, 1), "frac_of_8tbs": round(atrous_gbs / HBM_PEAK_GBS, 4)},
         },

@@ -366,6 +366,14 @@ lucida_status lucida_lbvh_build(lucida_ctx* ctx, const GpuTriangle* tris, size_t
 lucida_status lucida_bvh_build_treelet(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
                                        size_t num_positions, uint32_t rounds, GpuBvhNode* nodes_out,
                                        size_t* num_nodes, uint32_t* prim_indices_out);
+/* The same build on device arrays, like lucida_lbvh_build_device: nodes
+ * [2n-1], prim_indices [n] (optional), tris_sorted [n] (optional: the
+ * triangles in leaf order), stage_us (optional, host) [3]: the LBVH up to its
+ * fitted tree, the restructuring rounds, the layout, in microseconds. */
+lucida_status lucida_bvh_build_treelet_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n,
+                                              const float* positions, uint32_t rounds, GpuBvhNode* nodes,
+                                              size_t* num_nodes, uint32_t* prim_indices, GpuTriangle* tris_sorted,
+                                              float* stage_us);
 
 /* Device-resident LBVH build: all arrays are device pointers. tris_sorted
  * receives the triangles permuted into leaf order (Bvh::compact_primitives,

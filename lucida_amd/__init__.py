@@ -179,6 +179,7 @@ def lib() -> C.CDLL:
             "lucida_lbvh_build_parts": ([P, P, S, P, S, C.POINTER(LbvhParts)], I),
             "lucida_lbvh_build": ([P, P, S, P, S, P, C.POINTER(S), P], I),
             "lucida_bvh_build_treelet": ([P, P, S, P, S, U32, P, C.POINTER(S), P], I),
+            "lucida_bvh_build_treelet_device": ([P, P, S, P, U32, P, C.POINTER(S), P, P, P], I),
             "lucida_lbvh_build_device": ([P, P, S, P, P, C.POINTER(S), P, P, P], I),
             "lucida_scene_create": ([P, C.POINTER(SceneDesc), C.POINTER(P)], I),
             "lucida_scene_destroy": ([P], None),

@@ -509,6 +509,7 @@ inline uint32_t blocks_for(size_t n, int block) { return (uint32_t)((n + block -
 lucida_status bvh_build_treelet_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
                                        uint32_t rounds, GpuBvhNode* nodes, size_t* num_nodes,
                                        uint32_t* prim_indices, GpuTriangle* tris_sorted, float* stage_us) {
+    if (stage_us) stage_us[0] = stage_us[1] = stage_us[2] = 0.0f;
     if (n < 2)  // nothing to restructure: the LBVH's single leaf
         return lbvh_build_device(ctx, tris, n, positions, nodes, num_nodes, prim_indices, tris_sorted, nullptr);
     if (n > (1u << 30)) return set_error(LUCIDA_INVALID_ARG, "treelet BVH: more than 2^30 triangles");
@@ -635,6 +636,21 @@ lucida_status lucida_bvh_build_treelet(lucida_ctx* ctx, const GpuTriangle* tris,
     LUCIDA_HIP(hipMemcpyAsync(prim_indices_out, ctx->lb_prims.ptr, n * 4, hipMemcpyDeviceToHost, s), LUCIDA_READBACK);
     LUCIDA_HIP(hipStreamSynchronize(s), LUCIDA_VALIDATION);
     return LUCIDA_OK;
+}
+
+lucida_status lucida_bvh_build_treelet_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t n,
+                                              const float* positions, uint32_t rounds, GpuBvhNode* nodes,
+                                              size_t* num_nodes, uint32_t* prim_indices, GpuTriangle* tris_sorted,
+                                              float* stage_us) {
+    if (!ctx || !num_nodes || (n && (!tris || !positions || !nodes)))
+        return set_error(LUCIDA_INVALID_ARG, "lucida_bvh_build_treelet_device: null argument");
+    clear_error();
+    if (n == 0) {
+        *num_nodes = 0;
+        return LUCIDA_OK;
+    }
+    return bvh_build_treelet_device(ctx, tris, n, positions, rounds, nodes, num_nodes, prim_indices, tris_sorted,
+                                    stage_us);
 }
 
 }  // extern "C"

@@ -104,3 +104,42 @@ def test_render_over_treelet_tree_equals_lbvh_render(gpu, name, kw):
     diff = np.abs(a - b)
     assert diff.mean() <= 1e-5, diff.mean()
     assert (diff > 1e-4).mean() < 1e-3
+
+
+def test_device_entry_point_matches_oracle(gpu, oracle):
+    """lucida_bvh_build_treelet_device on device arrays: nodes, leaf order and
+    the triangles gathered into leaf order."""
+    import ctypes as C
+    lib = L.lib()
+    tris, pos = geometry("cornell-mesh")
+    tris = np.ascontiguousarray(tris)
+    pos = np.ascontiguousarray(pos, np.float32)
+    n = len(tris)
+    say(f"treelet device entry case: cornell-mesh n={n}")
+    sizes = {"tris": tris.nbytes, "pos": pos.nbytes, "nodes": (2 * n - 1) * 32, "prims": n * 4, "sorted": tris.nbytes}
+    ptrs = {}
+    try:
+        for k, b in sizes.items():
+            p = C.c_void_p()
+            L._check(lib.lucida_device_alloc(gpu._h, b, C.byref(p)))
+            ptrs[k] = p
+        L._check(lib.lucida_copy_to_device(gpu._h, ptrs["tris"], tris.ctypes.data, tris.nbytes))
+        L._check(lib.lucida_copy_to_device(gpu._h, ptrs["pos"], pos.ctypes.data, pos.nbytes))
+        count = C.c_size_t()
+        stage = np.zeros(3, np.float32)
+        L._check(lib.lucida_bvh_build_treelet_device(gpu._h, ptrs["tris"], n, ptrs["pos"], L.TREELET_ROUNDS,
+                                                     ptrs["nodes"], C.byref(count), ptrs["prims"], ptrs["sorted"],
+                                                     stage.ctypes.data))
+        nodes = np.empty(count.value, L.BVH_NODE)
+        prims = np.empty(n, np.uint32)
+        sorted_tris = np.empty(n, L.TRIANGLE)
+        L._check(lib.lucida_copy_to_host(gpu._h, nodes.ctypes.data, ptrs["nodes"], nodes.nbytes))
+        L._check(lib.lucida_copy_to_host(gpu._h, prims.ctypes.data, ptrs["prims"], prims.nbytes))
+        L._check(lib.lucida_copy_to_host(gpu._h, sorted_tris.ctypes.data, ptrs["sorted"], sorted_tris.nbytes))
+    finally:
+        for p in ptrs.values():
+            lib.lucida_device_free(gpu._h, p)
+    ref = oracle.treelet_build(tris, pos)
+    assert_same_tree(nodes, prims, ref, "device entry")
+    np.testing.assert_array_equal(sorted_tris.view(np.uint32), tris[ref["prim_indices"]].view(np.uint32))
+    assert (stage > 0).all()
