@@ -146,19 +146,22 @@ def cpu_baseline(blob, u, target_s: float, runs: int = 7) -> dict:
                       f"median of {runs} runs after a warm-up, {dt:.2f} s each"}
 
 
-def other_configs(L, gpu) -> dict:
+def other_configs(L, gpu, treelet: bool) -> dict:
     """Throughput of the other two render workloads at their full sizes: the
     Cornell box (256^2, 64 spp, 8 bounces, independent PCG samples) and the
     two-level instanced scene (4,096 instances of a 9,680-triangle mesh, a
-    2048x1024 environment map, 1024^2, 32 spp, 12 bounces, Sobol). Scenes are
-    resident before timing; one warm-up render each."""
+    2048x1024 environment map, 1024^2, 32 spp, 12 bounces, Sobol), the latter
+    over both trees when `treelet` (both levels restructured; and as built).
+    Scenes are resident before timing; one warm-up render each."""
     out = {}
-    for key, name, w, h, spp, depth, sampler in (
-            ("box_c1", "box-c1", 256, 256, 64, 8, L.SAMPLER_INDEPENDENT),
-            ("instances_c3", "instances-c3", 1024, 1024, 32, 12, L.SAMPLER_SOBOL)):
+    runs = [("box_c1", "box-c1", 256, 256, 64, 8, L.SAMPLER_INDEPENDENT, treelet),
+            ("instances_c3", "instances-c3", 1024, 1024, 32, 12, L.SAMPLER_SOBOL, treelet)]
+    if treelet:
+        runs.append(("instances_c3_lbvh_tree", "instances-c3", 1024, 1024, 32, 12, L.SAMPLER_SOBOL, False))
+    for key, name, w, h, spp, depth, sampler, tl in runs:
         blob = L.Blob(name)
         t0 = time.perf_counter()
-        scene = L.Scene(gpu, blob)
+        scene = L.Scene(gpu, blob, treelet=tl)
         gpu.check_errors()
         build_ms = (time.perf_counter() - t0) * 1e3
         film = L.Film(gpu, w, h)
@@ -175,7 +178,7 @@ def other_configs(L, gpu) -> dict:
                     "device_ms": round(st.elapsed_ms, 2), "width": w, "height": h, "spp": spp, "max_depth": depth,
                     "extension_rays": st.extension_rays, "shadow_rays": st.shadow_rays,
                     "scene_upload_and_build_ms": round(build_ms, 1), "triangles": counts.get("triangles"),
-                    "instances": counts.get("instances")}
+                    "instances": counts.get("instances"), "tree": "treelet" if tl else "lbvh"}
         del film, scene
     return out
 
@@ -617,7 +620,7 @@ def main() -> int:
     }
     out["extra"]["convergence_meter"] = round(film.convergence(), 6)
     if not a.no_configs:
-        out["extra"]["other_configs"] = other_configs(L, gpu)
+        out["extra"]["other_configs"] = other_configs(L, gpu, treelet)
         out["extra"]["architectures_512x512_8spp"] = architectures(L, gpu)
     if not a.no_relmse:
         net = None

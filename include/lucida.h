@@ -442,8 +442,9 @@ typedef struct lucida_instance_placement {
  * concatenated BLAS first, TLAS last (tlas_root = its first node), and the
  * instances appended to the primitives in TLAS leaf order. `desc` carries the
  * shared geometry and must have no BVH nodes and no instances of its own.
- * The reference builds both levels with binned SAH on the host; the tree
- * differs, the hits do not. */
+ * With desc->build_lbvh == 2 both levels are restructured by treelets
+ * (lucida_bvh_build_treelet). The reference builds both levels with binned
+ * SAH on the host; the tree differs, the hits do not. */
 lucida_status lucida_scene_create_instanced(lucida_ctx* ctx, const lucida_scene_desc* desc,
                                             const lucida_mesh_range* meshes, size_t num_meshes,
                                             const lucida_instance_placement* placements, size_t num_placements,

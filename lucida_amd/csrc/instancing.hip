@@ -109,6 +109,14 @@ static lucida_status build_two_level(lucida_ctx* ctx, lucida_scene* s, const luc
     hipStream_t st = ctx->stream;
     DevTmp tmp;
     hipError_t e = hipSuccess;
+    // Both levels are LBVHs, restructured by treelets when build_lbvh is 2.
+    const bool treelet = d->build_lbvh == 2;
+    auto build_tree = [&](const GpuTriangle* t, size_t count, const float* pos, GpuBvhNode* out, size_t* nn,
+                          uint32_t* order, GpuTriangle* sorted) {
+        return treelet ? bvh_build_treelet_device(ctx, t, count, pos, LUCIDA_TREELET_ROUNDS, out, nn, order, sorted,
+                                                  nullptr)
+                       : lbvh_build_device(ctx, t, count, pos, out, nn, order, sorted, nullptr);
+    };
     // --- BLASes -----------------------------------------------------------------
     size_t max_tris = 0;
     for (size_t m = 0; m < num_meshes; ++m) max_tris = std::max<size_t>(max_tris, meshes[m].triangle_count);
@@ -126,8 +134,7 @@ static lucida_status build_two_level(lucida_ctx* ctx, lucida_scene* s, const luc
     for (size_t m = 0; m < num_meshes; ++m) {
         const uint32_t first = meshes[m].first_triangle, count = meshes[m].triangle_count;
         size_t nn = 0;
-        lucida_status r = lbvh_build_device(ctx, s->triangles + first, count, s->positions, blas_nodes, &nn, nullptr,
-                                            blas_sorted, nullptr);
+        lucida_status r = build_tree(s->triangles + first, count, s->positions, blas_nodes, &nn, nullptr, blas_sorted);
         if (r != LUCIDA_OK) return r;
         // compact_primitives (bvh.rs:517): the mesh's slice takes leaf order
         LUCIDA_HIP(hipMemcpyAsync(s->triangles + first, blas_sorted, (size_t)count * sizeof(GpuTriangle),
@@ -184,7 +191,7 @@ static lucida_status build_two_level(lucida_ctx* ctx, lucida_scene* s, const luc
     LUCIDA_HIP(hipMemcpyAsync(d_tris, box_tris.data(), n * sizeof(GpuTriangle), hipMemcpyHostToDevice, st),
                LUCIDA_VALIDATION);
     size_t tn = 0;
-    lucida_status r = lbvh_build_device(ctx, d_tris, n, d_pos, tlas, &tn, order, d_sorted, nullptr);
+    lucida_status r = build_tree(d_tris, n, d_pos, tlas, &tn, order, d_sorted);
     if (r != LUCIDA_OK) return r;
     std::vector<uint32_t> h_order(n);
     LUCIDA_HIP(hipMemcpyAsync(h_order.data(), order, n * 4, hipMemcpyDeviceToHost, st), LUCIDA_READBACK);

@@ -165,8 +165,8 @@ def test_cast_matches_oracle_single_level(gpu, oracle, name):
                 np.testing.assert_array_equal(g[same].view(np.uint32), o[same].view(np.uint32))
 
 
-def render_pair(gpu, oracle, blob, w, h, spp, depth):
-    scene = L.Scene(gpu, blob)
+def render_pair(gpu, oracle, blob, w, h, spp, depth, treelet=False):
+    scene = L.Scene(gpu, blob, treelet=treelet)
     film = L.Film(gpu, w, h)
     u = blob.uniforms(w, h, samples=spp, max_depth=depth)
     st = L.render_wavefront(gpu, scene, u, spp, film)
@@ -197,5 +197,43 @@ def test_config3_scene_renders_and_matches_oracle(gpu, oracle):
     img, ref, st = render_pair(gpu, oracle, blob, 64, 64, 4, 12)
     c = oracle.compare(img, ref)
     say(f"mean_rel {c['mean_rel']:.2e}, means {c['mean_a']:.4f} / {c['mean_b']:.4f}")
+    assert c["mean_rel"] < 2e-3
+    assert abs(c["mean_a"] / c["mean_b"] - 1.0) < 1e-3
+
+
+@pytest.mark.gpu
+def test_treelet_instanced_casts_match_oracle_and_lbvh(gpu, oracle, small):
+    """Both levels restructured by treelets (build_lbvh = 2): casts equal the
+    oracle's walk of the same tree, and every distance equals the one over the
+    LBVH scene."""
+    say("instancing treelet cast case: instances-small")
+    scene = L.Scene(gpu, small, treelet=True)
+    plain = L.Scene(gpu, small)
+    assert scene.counts()["instances"] == 64
+    d, keep = scene.host_desc(small)
+    lo, hi = field_box(small)
+    rays = H.random_rays(20000, lo, hi, seed=7)
+    g, gi = L.scene_cast(gpu, scene, rays, False)
+    o, oi = oracle.scene_cast(d, rays, False)
+    gid, oid = g[:, 3].view(np.uint32), o[:, 3].view(np.uint32)
+    same = gid == oid
+    assert same.mean() >= 0.9995
+    hit = same & (gid != 0xFFFFFFFF)
+    np.testing.assert_array_equal(g[hit].view(np.uint32), o[hit].view(np.uint32))
+    p, _ = L.scene_cast(gpu, plain, rays, False)
+    np.testing.assert_array_equal(g[:, 0], p[:, 0])  # the same closest distance for every ray
+    ga, _ = L.scene_cast(gpu, scene, rays, True)
+    pa, _ = L.scene_cast(gpu, plain, rays, True)
+    np.testing.assert_array_equal(ga[:, 3].view(np.uint32) != 0xFFFFFFFF, pa[:, 3].view(np.uint32) != 0xFFFFFFFF)
+
+
+@pytest.mark.gpu
+def test_treelet_config3_scene_matches_oracle(gpu, oracle):
+    say("instanced treelet render case: instances-c3 64x64 x 4 spp, 12 bounces")
+    blob = L.Blob("instances-c3")
+    img, ref, st = render_pair(gpu, oracle, blob, 64, 64, 4, 12, treelet=True)
+    c = oracle.compare(img, ref)
+    say(f"mean_rel {c['mean_rel']:.2e}, means {c['mean_a']:.4f} / {c['mean_b']:.4f}")
+    assert st.stack_overflows == 0
     assert c["mean_rel"] < 2e-3
     assert abs(c["mean_a"] / c["mean_b"] - 1.0) < 1e-3
