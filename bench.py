@@ -308,18 +308,19 @@ def time_to_relmse(L, gpu, scene, blob, width: int, height: int, depth: int, thr
     return out
 
 
-def batch_c4(L, gpu, jobs_n: int = 64, seed: int = 6) -> dict:
+def batch_c4(L, gpu, jobs_n: int = 64, seed: int = 6, treelet: bool = True) -> dict:
     """The batched request stream: `jobs_n` renders drawn by a PCG64 stream
     (seed 6) over resolution {128, 256, 512, 1024}^2 x spp {1, 4, 16, 64} x
     scene {the Cornell box, four displaced-sphere rooms (seeds 1-4, 81,920 +
-    1,024 triangles)}, 8 bounces, all LBVHs resident. Run once job after job
+    1,024 triangles)}, 8 bounces, all trees resident (LBVHs restructured by
+    treelets when `treelet`, else as built). Run once job after job
     (each lucida_render_wavefront waits for the previous) and once as one
     lucida_render_batch with the autotuned launch shape; latency = time from
     the start until a job's film is complete, all jobs arriving at once."""
     from lucida_amd import autotune
     from lucida_amd import request_stream as RS
     blobs = RS.stream_blobs()
-    scenes = [L.Scene(gpu, b) for b in blobs]
+    scenes = [L.Scene(gpu, b, treelet=treelet) for b in blobs]
     reqs = RS.stream_requests(jobs_n, seed, len(blobs))
     paths = sum(r * r * spp for _, r, spp, _ in reqs)
 
@@ -351,7 +352,7 @@ def batch_c4(L, gpu, jobs_n: int = 64, seed: int = 6) -> dict:
     gpu.check_errors()
     batch_ms = (time.perf_counter() - t0) * 1e3
     q = lambda a, p: round(float(np.percentile(a, p)), 2)
-    return {"jobs": jobs_n, "paths": int(paths), "max_depth": 8,
+    return {"jobs": jobs_n, "paths": int(paths), "max_depth": 8, "tree": "treelet" if treelet else "lbvh",
             "sequential": {"ms": round(seq_ms, 2), "mpaths_per_s": round(paths / seq_ms / 1e3, 2),
                            "latency_p50_ms": q(seq_done, 50), "latency_p99_ms": q(seq_done, 99)},
             "batched": {"ms": round(batch_ms, 2), "device_ms": round(st.elapsed_ms, 2),
@@ -374,8 +375,6 @@ def main() -> int:
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--frequency", type=int, default=229)
     ap.add_argument("--samples-per-batch", type=int, default=0)
-    ap.add_argument("--bvh-width", type=int, default=0, choices=(0, 2, 4),
-                    help="0/2: the binary walk (default), 4: the 4-wide walk")
     ap.add_argument("--ray-order", type=int, default=0, choices=(0, 1, 2),
                     help="extension-queue order: 0 automatic (sorted for this scene), 1 kept, 2 always sorted")
     ap.add_argument("--tree", choices=("treelet", "lbvh"), default="treelet",
@@ -465,7 +464,7 @@ def main() -> int:
     # one stream), whose times are solo times; node visits and triangle tests
     # from one more, counted, render.
     mk_cfg = lambda count=0, profile=0, serialize=0: L.WavefrontConfig(
-        a.samples_per_batch, 0, a.bvh_width, 0, count, profile, a.ray_order, serialize)
+        a.samples_per_batch, 0, 0, 0, count, profile, a.ray_order, serialize)
     cfg = mk_cfg(serialize=1 if a.serialize else 0)
 
     for _ in range(a.warmup):
@@ -596,7 +595,7 @@ def main() -> int:
             "device": dev_name, "compute_units": cus,
             "mrays_per_s": round((ext_rays + shadow_rays) / sec / 1e6, 2),
             "extension_rays_per_step": ext_rays, "shadow_rays_per_step": shadow_rays,
-            "bvh_walk": "4-wide" if a.bvh_width == 4 else "binary",
+            "bvh_walk": "binary",
             "walk_simd_efficiency": {"extend": round(ext_visits / max(ext_slots, 1), 3),
                                      "connect": round(con_visits / max(con_slots, 1), 3)},
             "node_visits_per_ray": round(cst.node_visits / max(ext_rays + shadow_rays, 1), 2),
@@ -650,7 +649,7 @@ def main() -> int:
         out["time_to_relmse"] = time_to_relmse(L, gpu, scene, blob, a.width, a.height, a.depth, (0.01, 1e-3),
                                                a.relmse_ref_spp, a.relmse_max_spp, net)
     if not a.no_batch:
-        out["batch"] = batch_c4(L, gpu)
+        out["batch"] = batch_c4(L, gpu, treelet=treelet)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(blob, u, a.cpu_seconds)
     else:

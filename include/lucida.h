@@ -470,9 +470,8 @@ lucida_status lucida_scene_download_bvh(lucida_scene* scene, GpuBvhNode* nodes, 
  * index of a triangle hit in an instanced scene. Host arrays. */
 lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, const float* rays, size_t n, int flags,
                                 float* hits, uint32_t* instances);
-/* flags for lucida_scene_cast: */
+/* flags for lucida_scene_cast (any other bit is LUCIDA_INVALID_ARG): */
 #define LUCIDA_CAST_ANY_HIT 1 /* occlusion query */
-#define LUCIDA_CAST_WIDE 2    /* walk the scene's 4-wide BVH instead of the binary tree */
 
 /* A device-resident accumulation buffer (width*height GpuAccum). */
 lucida_status lucida_film_create(lucida_ctx* ctx, uint32_t width, uint32_t height, lucida_film** out);
@@ -512,8 +511,8 @@ typedef struct lucida_render_stats {
 typedef struct lucida_wavefront_config {
     uint32_t samples_per_batch; /* samples of every pixel in flight at once (0: about 2^23 paths) */
     uint32_t max_paths;         /* batched renders: paths per round at most (0: 2^24)            */
-    uint32_t bvh_width;         /* 0 or 2: the binary tree (the reference's order, whose node     *
-                                 * visit counts the oracle repeats); 4: the scene's 4-wide BVH    */
+    uint32_t bvh_width;         /* 0 or 2: the binary tree, walked in the reference's order (any  *
+                                 * other value is LUCIDA_INVALID_ARG)                              */
     uint32_t use_graph;         /* batched renders: capture every round's launches in one hipGraph */
     uint32_t count_traversal;   /* keep node/triangle counters (costs a little)                   */
     uint32_t profile_kernels;   /* time every kernel with HIP events (single renders)             */

@@ -623,7 +623,7 @@ def render_batch(gpu: Gpu, jobs, config: WavefrontConfig | None = None):
     return st, done
 
 
-def scene_cast(gpu: Gpu, scene: Scene, rays: np.ndarray, any_hit: bool = False, wide: bool = False):
+def scene_cast(gpu: Gpu, scene: Scene, rays: np.ndarray, any_hit: bool = False):
     """Batch closest-hit (or occlusion) queries through the renderer's own
     traversal: rays (n, 8) float32 = origin, unused, direction, t_max. Returns
     hits (n, 4) float32 = t, u, v, id bits (triangle, 0x80000000 | primitive or
@@ -632,7 +632,7 @@ def scene_cast(gpu: Gpu, scene: Scene, rays: np.ndarray, any_hit: bool = False, 
     n = len(rays)
     hits = np.zeros((n, 4), np.float32)
     inst = np.zeros(n, np.uint32)
-    flags = (1 if any_hit else 0) | (2 if wide else 0)
+    flags = 1 if any_hit else 0
     _check(lib().lucida_scene_cast(gpu._h, scene._h, _ptr(rays), n, flags, _ptr(hits), _ptr(inst)))
     return hits, inst
 

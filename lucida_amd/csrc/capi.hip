@@ -66,8 +66,7 @@ static lucida_status upload(lucida_ctx* ctx, T** dst, const T* src, size_t count
 
 static void scene_free(lucida_scene* s) {
     void* ptrs[] = {s->materials, s->primitives, s->positions,  s->vertex_attrs,    s->triangles,   s->tri_edges,
-                    s->prim_w,
-                    s->wide,      s->bvh_nodes,  s->lights,     s->env_pixels,      s->env_conditional, s->env_marginal};
+                    s->prim_w,    s->bvh_nodes,  s->lights,     s->env_pixels,      s->env_conditional, s->env_marginal};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -328,7 +327,6 @@ lucida_status lucida_scene_create(lucida_ctx* ctx, const lucida_scene_desc* d, l
     }
     if (st == LUCIDA_OK && s->num_triangles) st = build_tri_edges(ctx, s);
     if (st == LUCIDA_OK) st = build_prim_planes(ctx, s);
-    if (st == LUCIDA_OK && s->num_bvh_nodes) st = build_wide_bvh(ctx, s);
     if (st == LUCIDA_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
         st = set_error(LUCIDA_VALIDATION, "scene upload failed");
     if (st != LUCIDA_OK) {

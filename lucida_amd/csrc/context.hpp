@@ -101,8 +101,6 @@ struct lucida_scene {
     GpuTriangle* triangles = nullptr;
     float4* tri_edges = nullptr;  // per triangle: p0, p1 - p0, p2 - p0 (the intersection's own operands)
     float4* prim_w = nullptr;     // per analytic primitive a quad's w = n / |n|^2 (quad_w), then their boxes
-    void* wide = nullptr;         // 4-wide BVH collapsed from bvh_nodes (128-byte nodes), or null
-    size_t num_wide = 0;
     GpuBvhNode* bvh_nodes = nullptr;
     GpuLight* lights = nullptr;
     float* env_pixels = nullptr;
@@ -154,9 +152,6 @@ size_t sort_counted_scratch_words(size_t capacity);
 lucida_status sort_pairs_counted(hipStream_t stream, uint32_t* keys, uint32_t* vals, uint32_t* tk, uint32_t* tv,
                                  size_t capacity, const uint32_t* count, uint32_t bits, uint32_t* scratch,
                                  uint32_t** sorted_vals);
-// Builds scene->wide (the 4-wide BVH traversed by default) from its binary
-// BVH; a no-op for instanced scenes and scenes without a BVH.
-lucida_status build_wide_bvh(lucida_ctx* ctx, lucida_scene* scene);
 // Fills scene->prim_w (quad planes and the primitives' box) from its analytic
 // primitives (none: a no-op).
 lucida_status build_prim_planes(lucida_ctx* ctx, lucida_scene* scene);

@@ -30,7 +30,6 @@
 #include "context.hpp"
 #include "shading.hpp"
 #include "raystack.hpp"
-#include "widebvh.hpp"
 
 namespace lucida {
 namespace {
@@ -43,10 +42,9 @@ constexpr int SHADE_WG = 256;   // SHADE: four waves, one queue atomic per block
 constexpr int TRACE_WG = 128;   // traversal workgroups (two waves)
 constexpr int STACK = 32;       // traversal stack entries per lane (shaders/common/bvh.wgsl:133)
 constexpr int LDS_STACK = 16;   // the newest entries, in LDS; older ones spill to memory
-constexpr int WIDE_STACK = 48;  // the 4-wide walk pushes up to three entries per visit; the two-level
-                                // walk adds a RETURN marker and up to three waiting instances
+constexpr int INST_STACK = 48;  // the two-level walk adds a RETURN marker and up to three waiting instances
 // Traversal variants a kernel is compiled for.
-constexpr int TRAV_BINARY = 0, TRAV_WIDE = 1, TRAV_INSTANCED = 2;
+constexpr int TRAV_BINARY = 0, TRAV_INSTANCED = 1;
 constexpr int MAX_BOUNCES = 64;
 // Counter words: queue length per bounce at [0, 72) (bounce b+1's count is
 // written by bounce b's SHADE), shadow-ray count per bounce at [SHADOW_AT,
@@ -96,7 +94,7 @@ struct Planes {
     uint32_t* q1;
     uint32_t* counters;  // see SHADOW_AT / STATS_AT / TOTALS_AT
     uint32_t* job;       // job of each path (batched renders only)
-    uint32_t* spill;     // traversal stack spill columns of EXTEND (WIDE_STACK - LDS_STACK per lane)
+    uint32_t* spill;     // traversal stack spill columns of EXTEND (INST_STACK - LDS_STACK per lane)
     uint32_t* spill2;    // the same for CONNECT, which may run beside the next EXTEND
     uint32_t* qkey;      // sort key of each extension-queue entry SHADE appends (ray_order_key)
     uint32_t* skey;      // sort key of each shadow ray SHADE appends, and
@@ -119,8 +117,6 @@ struct SceneView {
     const float4* __restrict__ edges;  // per triangle: p0, p1 - p0, p2 - p0
     const GpuBvhNode* __restrict__ nodes;
     const GpuInstance* __restrict__ insts;  // the primitive array read as instances (absolute indices)
-    const WideNode* __restrict__ wide;      // 4-wide BVH (null: walk the binary tree)
-    uint32_t num_wide;
     const GpuLight* __restrict__ lights;
     const float* __restrict__ ggx;
     EnvView env;
@@ -591,112 +587,6 @@ __device__ inline bool traverse(const SceneView& sc, RayStack& st, uint32_t root
     return found;
 }
 
-// The 4-wide walk (widebvh.hpp) over the same leaves: a visit tests the four
-// child boxes of one 64-byte node (8-bit planes, rounded outward, so every
-// box the binary walk enters is entered); hit leaves are intersected at once
-// (so `closest` tightens before the inner children are tested), hit inner
-// children are visited nearest first, the others pushed. Hits equal the
-// binary walk's except between triangles at exactly equal t and rays that
-// graze a box within the binary slab test's rounding.
-__device__ inline void cswap(float& ea, uint32_t& ra, float& eb, uint32_t& rb) {
-    if (eb < ea) {
-        const float e = ea;
-        ea = eb;
-        eb = e;
-        const uint32_t r = ra;
-        ra = rb;
-        rb = r;
-    }
-}
-
-// Child k's slab on a compressed wide node: plane = origin + q * 2^e, so
-// t = (origin - o) * inv + q * (2^e * inv), one fma per plane with the
-// per-node terms b (base) and s (scale). Returns the entry (at least t_min)
-// and the exit before clipping to the current closest hit; the box is hit
-// when entry <= min(exit, closest), exactly the clipped slab test.
-__device__ inline float qbyte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
-
-__device__ inline void qslab(uint32_t lx, uint32_t ly, uint32_t lz, uint32_t hx, uint32_t hy, uint32_t hz, int k,
-                             V3 b, V3 s, float t_min, float* entry, float* exit) {
-    const float t0x = fmaf(qbyte(lx, k), s.x, b.x), t1x = fmaf(qbyte(hx, k), s.x, b.x);
-    const float t0y = fmaf(qbyte(ly, k), s.y, b.y), t1y = fmaf(qbyte(hy, k), s.y, b.y);
-    const float t0z = fmaf(qbyte(lz, k), s.z, b.z), t1z = fmaf(qbyte(hz, k), s.z, b.z);
-    *entry = fmaxf(fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z)), t_min);
-    *exit = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-}
-
-__device__ inline float exp2_int(uint32_t word, int axis) {
-    const int e = (int)(int8_t)((word >> (8 * axis)) & 0xFFu);
-    return __int_as_float((e + 127) << 23);
-}
-
-template <bool ANY_HIT>
-__device__ inline bool traverse_wide(const SceneView& sc, RayStack& st, V3 o, V3 d, float t_min, float t_max,
-                                     float* t_out, float* u_out, float* v_out, uint32_t* tri_out, uint32_t* visits,
-                                     uint32_t* tests, uint32_t* overflow) {
-    bool found = false;
-    float closest = t_max;
-    const V3 inv = safe_inv_dir(d);
-    uint32_t cur = 0;
-    const uint32_t cap = 2u * sc.num_wide + 8u;
-    for (uint32_t iter = 0; iter < cap; ++iter) {
-        *visits += 1;
-        const uint4* q = reinterpret_cast<const uint4*>(sc.wide + cur);
-        const uint4 a = q[0], ch = q[1], lo = q[2], hi = q[3];
-        // lo = (qlo_x, qlo_y, qlo_z, qhi_x) words, hi = (qhi_y, qhi_z, pad, pad)
-        const V3 s{exp2_int(a.w, 0) * inv.x, exp2_int(a.w, 1) * inv.y, exp2_int(a.w, 2) * inv.z};
-        const V3 b{(__uint_as_float(a.x) - o.x) * inv.x, (__uint_as_float(a.y) - o.y) * inv.y,
-                   (__uint_as_float(a.z) - o.z) * inv.z};
-        const uint32_t CH[4] = {ch.x, ch.y, ch.z, ch.w};
-        float EN[4], EX[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) qslab(lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, k, b, s, t_min, &EN[k], &EX[k]);
-        // leaves first
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t c = CH[k];
-            if (c == WIDE_EMPTY || !(c & WIDE_LEAF) || !(EN[k] <= fminf(EX[k], closest))) continue;
-            const uint32_t first = (c & ~WIDE_LEAF) >> COUNT_BITS, count = c & COUNT_MASK;
-            for (uint32_t m = 0; m < count; ++m) {
-                const uint32_t ti = first + m;
-                *tests += 1;
-                float t, u, v;
-                if (tri_test(sc.edges, ti, o, d, t_min, closest, &t, &u, &v)) {
-                    if (ANY_HIT) return true;
-                    closest = t;
-                    *t_out = t; *u_out = u; *v_out = v; *tri_out = ti;
-                    found = true;
-                }
-            }
-        }
-        // inner children, tested against the tightened `closest`
-        float E[4];
-        uint32_t R[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t c = CH[k];
-            const bool inner = c != WIDE_EMPTY && !(c & WIDE_LEAF);
-            E[k] = inner && EN[k] <= fminf(EX[k], closest) ? EN[k] : INFINITY;
-            R[k] = c;
-        }
-        cswap(E[0], R[0], E[1], R[1]);
-        cswap(E[2], R[2], E[3], R[3]);
-        cswap(E[0], R[0], E[2], R[2]);
-        cswap(E[1], R[1], E[3], R[3]);
-        cswap(E[1], R[1], E[2], R[2]);
-#pragma unroll
-        for (int k = 3; k >= 1; --k)
-            if (E[k] != INFINITY && !st.push(R[k])) *overflow += 1;
-        if (E[0] != INFINITY) {
-            cur = R[0];
-            continue;
-        }
-        if (st.sp == 0) break;
-        cur = st.pop();
-    }
-    return found;
-}
-
 // An instance's world-to-object transform (GpuInstance, gpu_layout.rs:496).
 struct InstanceXf {
     V3 r0, r1, r2, t;
@@ -933,7 +823,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJ
     const uint32_t count = *count_ptr;
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (i < count) {
-      RayStack st = make_stack(stack_mem, p.spill, MODE == TRAV_BINARY ? STACK : WIDE_STACK);
+      RayStack st = make_stack(stack_mem, p.spill, MODE == TRAV_BINARY ? STACK : INST_STACK);
       const uint32_t pid = queue[i];
       const uint32_t jid = BATCH ? p.job[pid] : 0u;
       for (;;) {
@@ -951,9 +841,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_extend(SceneView sc0, const BatchJ
         // single-level one (scene.wgsl:31-75): their triangles live in object space.
         const bool two_level = MODE == TRAV_INSTANCED && sc.num_instances;
         bool th;
-        if (MODE == TRAV_WIDE)
-            th = traverse_wide<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests, &overflow);
-        else if (two_level)
+        if (two_level)
             th = traverse_instances<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &inst, &visits, &tests,
                                            &overflow);
         else
@@ -1331,7 +1219,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
     uint32_t visits = 0, tests = 0, overflow = 0;
     if (q < count) {
       const uint32_t i = order ? order[q] : q;  // the shadow record this lane traces
-      RayStack st = make_stack(stack_mem, p.spill2, MODE == TRAV_BINARY ? STACK : WIDE_STACK);
+      RayStack st = make_stack(stack_mem, p.spill2, MODE == TRAV_BINARY ? STACK : INST_STACK);
       const float4 a = p.sro[i], b = p.srd[i];
       const uint32_t jid = BATCH ? p.job[__float_as_uint(b.w)] : 0u;
       for (;;) {
@@ -1344,9 +1232,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
         if (!blocked) {
             float t, u, v;
             uint32_t tri, inst;
-            if (MODE == TRAV_WIDE)
-                blocked = traverse_wide<true>(sc, st, o, d, T_MIN, t_max, &t, &u, &v, &tri, &visits, &tests, &overflow);
-            else if (MODE == TRAV_INSTANCED && sc.num_instances)
+            if (MODE == TRAV_INSTANCED && sc.num_instances)
                 blocked = traverse_instances<true>(sc, st, o, d, T_MIN, t_max, &t, &u, &v, &tri, &inst, &visits,
                                                    &tests, &overflow);
             else
@@ -1375,13 +1261,11 @@ __global__ __launch_bounds__(TRACE_WG) void k_connect(SceneView sc0, const Batch
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* __restrict__ rays, uint32_t n,
                                                    uint32_t any_hit, float4* __restrict__ hits,
-                                                   uint32_t* __restrict__ insts, uint32_t* __restrict__ spill,
-                                                   uint32_t wide) {
+                                                   uint32_t* __restrict__ insts, uint32_t* __restrict__ spill) {
     __shared__ uint32_t stack_mem[LDS_STACK * TRACE_WG];
     const uint32_t i = blockIdx.x * TRACE_WG + threadIdx.x;
     if (i >= n) return;
-    const bool use_wide = wide && sc.wide && !sc.num_instances;
-    RayStack st = make_stack(stack_mem, spill, (use_wide || sc.num_instances) ? WIDE_STACK : STACK);
+    RayStack st = make_stack(stack_mem, spill, sc.num_instances ? INST_STACK : STACK);
     const float4 a = rays[2 * i], b = rays[2 * i + 1];
     const V3 o = f4xyz(a), d = f4xyz(b);
     float best_t = b.w;
@@ -1406,17 +1290,13 @@ __global__ __launch_bounds__(TRACE_WG) void k_cast(SceneView sc, const float4* _
         if (any_hit)
             got = sc.num_instances ? traverse_instances<true>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &inst,
                                                               &visits, &tests, &overflow)
-                  : use_wide ? traverse_wide<true>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
-                                                   &overflow)
-                             : traverse<true>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
-                                              &overflow);
+                                   : traverse<true>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
+                                                    &overflow);
         else
             got = sc.num_instances ? traverse_instances<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &inst,
                                                                &visits, &tests, &overflow)
-                  : use_wide ? traverse_wide<false>(sc, st, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
-                                                    &overflow)
-                             : traverse<false>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
-                                               &overflow);
+                                   : traverse<false>(sc, st, 0, o, d, T_MIN, best_t, &t, &u, &v, &tri, &visits, &tests,
+                                                     &overflow);
         if (got) out = any_hit ? make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u)) : make_float4(t, u, v, __uint_as_float(tri));
     }
     hits[i] = out;
@@ -1445,7 +1325,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_megakernel(SceneView sc, const Gpu
     uint32_t visits = 0, tests = 0, overflow = 0, rays = 0, shadow_rays = 0;
     if (px < band.count) {
         const GpuUniforms& u = *U;
-        RayStack st = make_stack(stack_mem, spill, MODE == TRAV_BINARY ? STACK : WIDE_STACK);
+        RayStack st = make_stack(stack_mem, spill, MODE == TRAV_BINARY ? STACK : INST_STACK);
         const uint32_t pixel = band.first + px;
         V3 sum{0, 0, 0}, sq{0, 0, 0}, al{0, 0, 0}, nr{0, 0, 0};
         float dp = 0.0f;
@@ -1472,10 +1352,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_megakernel(SceneView sc, const Gpu
                 uint32_t tri = 0, inst = 0;
                 const bool two_level = MODE == TRAV_INSTANCED && sc.num_instances;
                 bool th;
-                if (MODE == TRAV_WIDE)
-                    th = traverse_wide<false>(sc, st, o, d, T_MIN, best_t, &t, &hu, &hv, &tri, &visits, &tests,
-                                              &overflow);
-                else if (two_level)
+                if (two_level)
                     th = traverse_instances<false>(sc, st, o, d, T_MIN, best_t, &t, &hu, &hv, &tri, &inst, &visits,
                                                    &tests, &overflow);
                 else
@@ -1501,10 +1378,7 @@ __global__ __launch_bounds__(TRACE_WG) void k_megakernel(SceneView sc, const Gpu
                     if (!blocked) {
                         float tt, uu, vv;
                         uint32_t tri2, inst2;
-                        if (MODE == TRAV_WIDE)
-                            blocked = traverse_wide<true>(sc, st, sr.o, sr.d, T_MIN, sr.t_max, &tt, &uu, &vv, &tri2,
-                                                          &visits, &tests, &overflow);
-                        else if (two_level)
+                        if (two_level)
                             blocked = traverse_instances<true>(sc, st, sr.o, sr.d, T_MIN, sr.t_max, &tt, &uu, &vv,
                                                                &tri2, &inst2, &visits, &tests, &overflow);
                         else
@@ -1803,7 +1677,7 @@ static lucida_status ensure_wavefront(WfPipe& w, size_t P, Planes* out, GpuUnifo
         // q0, q1, job, the sorts' keys, orders and temporaries, then their scratch
         LUCIDA_HIP(w.queues.ensure((P * 10 + 2 * sort_counted_scratch_words(P)) * 4), LUCIDA_OOM);
         // spill columns for every lane of a traversal grid over P items
-        LUCIDA_HIP(w.spill.ensure(2 * (P + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
+        LUCIDA_HIP(w.spill.ensure(2 * (P + TRACE_WG) * (INST_STACK - LDS_STACK) * 4), LUCIDA_OOM);
         w.capacity = P;
     }
     LUCIDA_HIP(w.counters.ensure(COUNTER_BYTES), LUCIDA_OOM);
@@ -1826,7 +1700,7 @@ static lucida_status ensure_wavefront(WfPipe& w, size_t P, Planes* out, GpuUnifo
     out->sort_scratch = q + 10 * C;
     out->sort_scratch2 = out->sort_scratch + sort_counted_scratch_words(C);
     out->spill = w.spill.as<uint32_t>();
-    out->spill2 = out->spill + (C + TRACE_WG) * (WIDE_STACK - LDS_STACK);
+    out->spill2 = out->spill + (C + TRACE_WG) * (INST_STACK - LDS_STACK);
     out->counters = w.counters.as<uint32_t>();
     *dU = w.uniforms.as<GpuUniforms>();
     return LUCIDA_OK;
@@ -1836,13 +1710,14 @@ static lucida_status ensure_wavefront(WfPipe& w, size_t P, Planes* out, GpuUnifo
 
 using namespace lucida;
 
+// The walks are binary: bvh_width 0 (default) or 2.
+static bool binary_width(const lucida_wavefront_config& cfg) { return cfg.bvh_width == 0 || cfg.bvh_width == 2; }
+
 // EXTEND / CONNECT for the traversal variant `mode` (TRAV_*).
 template <bool BATCH>
 static void launch_extend(int mode, dim3 grid, hipStream_t s, const SceneView& sc, const BatchJob* jobs,
                           const Planes& p, const uint32_t* queue, const uint32_t* count, uint32_t stats_on) {
-    if (mode == TRAV_WIDE)
-        hipLaunchKernelGGL((k_extend<BATCH, TRAV_WIDE>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, queue, count, stats_on);
-    else if (mode == TRAV_INSTANCED)
+    if (mode == TRAV_INSTANCED)
         hipLaunchKernelGGL((k_extend<BATCH, TRAV_INSTANCED>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, queue, count,
                            stats_on);
     else
@@ -1853,10 +1728,7 @@ static void launch_extend(int mode, dim3 grid, hipStream_t s, const SceneView& s
 template <bool BATCH>
 static void launch_connect(int mode, dim3 grid, hipStream_t s, const SceneView& sc, const BatchJob* jobs,
                            const Planes& p, const uint32_t* count, const uint32_t* order, uint32_t stats_on) {
-    if (mode == TRAV_WIDE)
-        hipLaunchKernelGGL((k_connect<BATCH, TRAV_WIDE>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, order,
-                           stats_on);
-    else if (mode == TRAV_INSTANCED)
+    if (mode == TRAV_INSTANCED)
         hipLaunchKernelGGL((k_connect<BATCH, TRAV_INSTANCED>), grid, dim3(TRACE_WG), 0, s, sc, jobs, p, count, order,
                            stats_on);
     else
@@ -1919,8 +1791,6 @@ static SceneView make_view(lucida_ctx* ctx, const lucida_scene* scene) {
     sc.edges = scene->tri_edges;
     sc.nodes = scene->bvh_nodes;
     sc.insts = reinterpret_cast<const GpuInstance*>(scene->primitives);
-    sc.wide = reinterpret_cast<const WideNode*>(scene->wide);
-    sc.num_wide = (uint32_t)scene->num_wide;
     sc.num_instances = (uint32_t)scene->num_instances;
     sc.tlas_root = scene->tlas_root;
     sc.lights = scene->lights;
@@ -1961,6 +1831,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     if (scene->num_bvh_nodes >= (1u << 26) || scene->num_triangles + scene->num_primitives + scene->num_instances >= (1u << 26))
         return set_error(LUCIDA_INVALID_ARG, "scene too large: node and primitive indices are packed in 26 bits");
     if (stats) std::memset(stats, 0, sizeof *stats);
+    if (config && !binary_width(*config)) return set_error(LUCIDA_INVALID_ARG, "bvh_width must be 0 or 2 (the binary walk)");
     if (samples == 0 || uniforms->max_depth == 0) return LUCIDA_OK;
     lucida_status st = ensure_ggx_table(ctx);
     if (st != LUCIDA_OK) return st;
@@ -1982,9 +1853,7 @@ lucida_status lucida_render_wavefront(lucida_ctx* ctx, const lucida_scene* scene
     if (P >= 0x7FFFFFFFull / 4) return set_error(LUCIDA_INVALID_ARG, "too many paths in one batch");
 
     const SceneView sc = make_view(ctx, scene);
-    const int mode = scene->num_instances ? TRAV_INSTANCED
-                     : (scene->wide && cfg.bvh_width == 4) ? TRAV_WIDE
-                                                           : TRAV_BINARY;
+    const int mode = scene->num_instances ? TRAV_INSTANCED : TRAV_BINARY;
 
     // The render's batches: spb samples of every pixel each. They alternate
     // between the context's two pipelines: while one runs its latency-bound
@@ -2169,6 +2038,7 @@ lucida_status lucida_render_megakernel(lucida_ctx* ctx, const lucida_scene* scen
     if (scene->num_bvh_nodes >= (1u << 26) || scene->num_triangles + scene->num_primitives + scene->num_instances >= (1u << 26))
         return set_error(LUCIDA_INVALID_ARG, "scene too large: node and primitive indices are packed in 26 bits");
     if (stats) std::memset(stats, 0, sizeof *stats);
+    if (config && !binary_width(*config)) return set_error(LUCIDA_INVALID_ARG, "bvh_width must be 0 or 2 (the binary walk)");
     if (samples == 0 || uniforms->max_depth == 0) return LUCIDA_OK;
     lucida_status st = ensure_ggx_table(ctx);
     if (st != LUCIDA_OK) return st;
@@ -2183,11 +2053,9 @@ lucida_status lucida_render_megakernel(lucida_ctx* ctx, const lucida_scene* scen
     spb = std::min(spb, samples);
     const uint32_t batches = (samples + spb - 1) / spb;
     const SceneView sc = make_view(ctx, scene);
-    const int mode = scene->num_instances ? TRAV_INSTANCED
-                     : (scene->wide && cfg.bvh_width == 4) ? TRAV_WIDE
-                                                           : TRAV_BINARY;
+    const int mode = scene->num_instances ? TRAV_INSTANCED : TRAV_BINARY;
     hipStream_t s = ctx->stream;
-    LUCIDA_HIP(ctx->mk_spill.ensure((pixels + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
+    LUCIDA_HIP(ctx->mk_spill.ensure((pixels + TRACE_WG) * (INST_STACK - LDS_STACK) * 4), LUCIDA_OOM);
     LUCIDA_HIP(ctx->mk_state.ensure(COUNTER_BYTES + (size_t)batches * sizeof(GpuUniforms)), LUCIDA_OOM);
     uint32_t* counters = ctx->mk_state.as<uint32_t>();
     GpuUniforms* dU = reinterpret_cast<GpuUniforms*>(ctx->mk_state.as<char>() + COUNTER_BYTES);
@@ -2207,9 +2075,6 @@ lucida_status lucida_render_megakernel(lucida_ctx* ctx, const lucida_scene* scen
         const dim3 grid(grid_for(pixels, TRACE_WG)), block(TRACE_WG);
         if (mode == TRAV_INSTANCED)
             hipLaunchKernelGGL(k_megakernel<TRAV_INSTANCED>, grid, block, 0, s, sc, dU + b, film->accum, whole,
-                               ctx->mk_spill.as<uint32_t>(), counters, stats_on);
-        else if (mode == TRAV_WIDE)
-            hipLaunchKernelGGL(k_megakernel<TRAV_WIDE>, grid, block, 0, s, sc, dU + b, film->accum, whole,
                                ctx->mk_spill.as<uint32_t>(), counters, stats_on);
         else
             hipLaunchKernelGGL(k_megakernel<TRAV_BINARY>, grid, block, 0, s, sc, dU + b, film->accum, whole,
@@ -2243,8 +2108,9 @@ lucida_status lucida_render_megakernel(lucida_ctx* ctx, const lucida_scene* scen
 
 lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, const float* rays, size_t n, int flags,
                                 float* hits, uint32_t* instances) {
-    const int any_hit = flags & 1;
+    const int any_hit = flags & LUCIDA_CAST_ANY_HIT;
     if (!ctx || !scene || (n && (!rays || !hits))) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_cast: bad argument");
+    if (flags & ~LUCIDA_CAST_ANY_HIT) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_cast: unknown flag");
     if (n == 0) return LUCIDA_OK;
     if (n > (1u << 26)) return set_error(LUCIDA_INVALID_ARG, "lucida_scene_cast: more than 2^26 rays");
     clear_error();
@@ -2253,14 +2119,14 @@ lucida_status lucida_scene_cast(lucida_ctx* ctx, const lucida_scene* scene, cons
     LUCIDA_HIP(d_rays.ensure(n * 32), LUCIDA_OOM);
     LUCIDA_HIP(d_hits.ensure(n * 16), LUCIDA_OOM);
     LUCIDA_HIP(d_inst.ensure(n * 4), LUCIDA_OOM);
-    LUCIDA_HIP(d_spill.ensure((n + TRACE_WG) * (WIDE_STACK - LDS_STACK) * 4), LUCIDA_OOM);
+    LUCIDA_HIP(d_spill.ensure((n + TRACE_WG) * (INST_STACK - LDS_STACK) * 4), LUCIDA_OOM);
     lucida_status st = LUCIDA_OK;
     if (hipMemcpyAsync(d_rays.ptr, rays, n * 32, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         st = set_error(LUCIDA_VALIDATION, "lucida_scene_cast: upload failed");
     if (st == LUCIDA_OK) {
         hipLaunchKernelGGL(k_cast, dim3(grid_for(n, TRACE_WG)), dim3(TRACE_WG), 0, ctx->stream, sc,
                            d_rays.as<float4>(), (uint32_t)n, any_hit ? 1u : 0u, d_hits.as<float4>(),
-                           d_inst.as<uint32_t>(), d_spill.as<uint32_t>(), (flags & 2) ? 1u : 0u);
+                           d_inst.as<uint32_t>(), d_spill.as<uint32_t>());
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(hits, d_hits.ptr, n * 16, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
             (instances &&
@@ -2286,6 +2152,7 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     if (n > 65536) return set_error(LUCIDA_INVALID_ARG, "lucida_render_batch: more than 65,536 jobs");
     lucida_wavefront_config cfg{};
     if (config) cfg = *config;
+    if (!binary_width(cfg)) return set_error(LUCIDA_INVALID_ARG, "lucida_render_batch: bvh_width must be 0 or 2");
     for (size_t j = 0; j < n; ++j) {
         const lucida_job& J = jobs[j];
         if (!J.scene || !J.film) return set_error(LUCIDA_INVALID_ARG, "lucida_render_batch: a job has no scene or film");
@@ -2325,16 +2192,13 @@ lucida_status lucida_render_batch(lucida_ctx* ctx, const lucida_job* jobs, size_
     std::vector<BatchJob> table;
     std::vector<SceneView> views(n);
     std::vector<int> solo_mode(n);
-    bool any_instanced = false, all_wide = true;
+    bool any_instanced = false;
     for (size_t j = 0; j < n; ++j) {
         views[j] = make_view(ctx, jobs[j].scene);
         any_instanced = any_instanced || jobs[j].scene->num_instances > 0;
-        all_wide = all_wide && jobs[j].scene->wide != nullptr;
-        solo_mode[j] = jobs[j].scene->num_instances                        ? TRAV_INSTANCED
-                       : (jobs[j].scene->wide && cfg.bvh_width == 4) ? TRAV_WIDE
-                                                                         : TRAV_BINARY;
+        solo_mode[j] = jobs[j].scene->num_instances ? TRAV_INSTANCED : TRAV_BINARY;
     }
-    const int mode = any_instanced ? TRAV_INSTANCED : (all_wide && cfg.bvh_width == 4) ? TRAV_WIDE : TRAV_BINARY;
+    const int mode = any_instanced ? TRAV_INSTANCED : TRAV_BINARY;
     // A job whose chunks hold kSoloPaths paths or more fills the device by
     // itself: its chunks run as rounds of their own with the single-render
     // kernels (no per-path job lookup). The other jobs' chunks are packed

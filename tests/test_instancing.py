@@ -156,13 +156,12 @@ def test_cast_matches_oracle_single_level(gpu, oracle, name):
     pos = blob.arrays()["positions"][:, :3]
     rays = H.random_rays(20000, pos.min(axis=0), pos.max(axis=0), seed=6)
     for any_hit in (False, True):
-        for wide in (False, True):  # the reference's binary walk and the 4-wide one
-            g, _ = L.scene_cast(gpu, scene, rays, any_hit, wide=wide)
-            o, _ = oracle.scene_cast(d, rays, any_hit)
-            assert (g[:, 3].view(np.uint32) == o[:, 3].view(np.uint32)).mean() >= 0.9995
-            if not any_hit:
-                same = g[:, 3].view(np.uint32) == o[:, 3].view(np.uint32)
-                np.testing.assert_array_equal(g[same].view(np.uint32), o[same].view(np.uint32))
+        g, _ = L.scene_cast(gpu, scene, rays, any_hit)
+        o, _ = oracle.scene_cast(d, rays, any_hit)
+        assert (g[:, 3].view(np.uint32) == o[:, 3].view(np.uint32)).mean() >= 0.9995
+        if not any_hit:
+            same = g[:, 3].view(np.uint32) == o[:, 3].view(np.uint32)
+            np.testing.assert_array_equal(g[same].view(np.uint32), o[same].view(np.uint32))
 
 
 def render_pair(gpu, oracle, blob, w, h, spp, depth, treelet=False):
