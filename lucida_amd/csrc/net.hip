@@ -159,13 +159,17 @@ __global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ in, int c
 // block walks its slice of 32-pixel chunks (32 consecutive pixels in NHWC
 // order, the last chunk zero-padded), staging both operands in LDS, and its
 // four waves split the (co-tile, ci-tile) pairs, at most WG_MAX_PAIRS each.
-// Partial sums go out with fp32 atomics.
+// Each block owns its slice's partial sums and stores them to its own row of
+// `part` (row length 9 * cin * cout + cout, bias partials at the end from the
+// tap-4 blocks); k_wgrad_sum then adds the rows up in slice order, so the
+// gradients, and the training built on them, are the same on every run.
 // ---------------------------------------------------------------------------
 constexpr int WG_MAX_PAIRS = 12;
+constexpr int WG_SLICES = 64;
 
 __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ in, int cin, const bf16* __restrict__ dpre,
-                                               int cout, int n, int h, int w, int slices, float* __restrict__ dW,
-                                               float* __restrict__ db) {
+                                               int cout, int n, int h, int w, int slices,
+                                               float* __restrict__ part) {
     extern __shared__ unsigned char smem_raw[];
     bf16* s_in = reinterpret_cast<bf16*>(smem_raw);  // [32][cin]
     bf16* s_dp = s_in + 32 * cin;                    // [32][cout]
@@ -233,6 +237,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ in, int 
         }
     }
     // D[m = co][n = ci]: lane holds co = mt*16 + 4g + j, ci = nt*16 + r.
+    float* row = part + (size_t)slice * ((size_t)9 * cin * cout + cout);
 #pragma unroll
     for (int q = 0; q < WG_MAX_PAIRS; ++q) {
         const int pr = wv + 4 * q;
@@ -241,11 +246,21 @@ __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ in, int 
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int co = mt * 16 + 4 * g + j, ci = nt * 16 + r;
-                atomicAdd(&dW[(tap * cin + ci) * cout + co], acc[q][j]);
+                row[(tap * cin + ci) * cout + co] = acc[q][j];
             }
         }
     }
-    if (tap == 4 && (int)threadIdx.x < cout) atomicAdd(&db[threadIdx.x], bsum);
+    if (tap == 4 && (int)threadIdx.x < cout) row[(size_t)9 * cin * cout + threadIdx.x] = bsum;
+}
+
+// grad[e] = sum over slices k = 0, 1, ... of part[k][e], in that order.
+__global__ __launch_bounds__(256) void k_wgrad_sum(const float* __restrict__ part, size_t row, int slices,
+                                                   float* __restrict__ grad) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= row) return;
+    float acc = 0.0f;
+    for (int k = 0; k < slices; ++k) acc += part[(size_t)k * row + e];
+    grad[e] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -539,6 +554,7 @@ using namespace lucida;
 struct lucida_net {
     lucida_ctx* ctx = nullptr;
     DevBuf params, grads, adam_m, adam_v;  // fp32, layout: per layer W[9][cin][cout] then b[cout]
+    DevBuf wg_part;                        // weight-gradient partials: WG_SLICES rows of one layer's W and b
     DevBuf frag_fwd, frag_bwd;             // bf16 fragments
     size_t frag_fwd_off[kNetNumLayers], frag_bwd_off[kNetNumLayers], param_off[kNetNumLayers];
     DevBuf acts, dacts, target, loss_partial;
@@ -687,7 +703,12 @@ static lucida_status net_backward(lucida_net* net) {
     float* G = net->grads.as<float>();
     const bf16* FB = net->frag_bwd.as<bf16>();
     const int H[4] = {H0, H0 / 2, H0 / 4, H0 / 8}, Wd[4] = {W0, W0 / 2, W0 / 4, W0 / 8};
-    LUCIDA_HIP(hipMemsetAsync(G, 0, net_param_count() * 4, s), LUCIDA_VALIDATION);
+    // Every layer's W and b gradients are written whole by its k_wgrad_sum;
+    // the partials buffer is sized once, for the largest layer.
+    size_t max_row = 0;
+    for (int li = 0; li < kNetNumLayers; ++li)
+        max_row = std::max(max_row, (size_t)9 * kNetLayers[li].cin * kNetLayers[li].cout + kNetLayers[li].cout);
+    LUCIDA_HIP(net->wg_part.ensure((size_t)WG_SLICES * max_row * 4), LUCIDA_OOM);
     lucida_status st;
     // d(layer li's pre-activation) -> weight grads, and d(its input) into d_in.
     auto layer_bwd = [&](int li, size_t in_off, int lvl, size_t d_in_off) -> lucida_status {
@@ -697,11 +718,12 @@ static lucida_status net_backward(lucida_net* net) {
             hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(count)), dim3(256), 0, s, D + a.l[li], A + a.l[li], count);
         if ((L.cin / 16) * (L.cout / 16) > 4 * WG_MAX_PAIRS)
             return set_error(LUCIDA_INVALID_ARG, "weight gradient: too many channel tiles for one block");
-        const int slices = 64;
         const size_t lds = (size_t)32 * (L.cin + L.cout) * 2;
-        hipLaunchKernelGGL(k_wgrad, dim3(9 * slices), dim3(256), lds, s, A + in_off, L.cin, D + a.l[li], L.cout, n,
-                           H[lvl], Wd[lvl], slices, G + net->param_off[li],
-                           G + net->param_off[li] + (size_t)9 * L.cin * L.cout);
+        const size_t row = (size_t)9 * L.cin * L.cout + L.cout;  // the layer's W then b, as in `grads`
+        hipLaunchKernelGGL(k_wgrad, dim3(9 * WG_SLICES), dim3(256), lds, s, A + in_off, L.cin, D + a.l[li], L.cout,
+                           n, H[lvl], Wd[lvl], WG_SLICES, net->wg_part.as<float>());
+        hipLaunchKernelGGL(k_wgrad_sum, dim3((unsigned)((row + 255) / 256)), dim3(256), 0, s,
+                           net->wg_part.as<float>(), row, WG_SLICES, G + net->param_off[li]);
         if (d_in_off == (size_t)-1) return LUCIDA_OK;
         return launch_conv(s, D + a.l[li], L.cout, L.cin, FB + net->frag_bwd_off[li], nullptr, 0, D + d_in_off, n,
                            H[lvl], Wd[lvl]);
@@ -815,7 +837,8 @@ void lucida_net_destroy(lucida_net* net) {
     if (!net) return;
     (void)hipStreamSynchronize(net->ctx->stream);
     DevBuf* b[] = {&net->params, &net->grads, &net->adam_m, &net->adam_v, &net->frag_fwd, &net->frag_bwd,
-                   &net->acts, &net->dacts, &net->target, &net->loss_partial, &net->base, &net->base_log};
+                   &net->acts, &net->dacts, &net->target, &net->loss_partial, &net->base, &net->base_log,
+                   &net->wg_part};
     for (DevBuf* x : b) x->release();
     delete net;
 }

@@ -89,10 +89,16 @@ def make_training_set(gpu: L.Gpu, scenes: int = 24, width: int = 256, height: in
 
 
 def train(gpu: L.Gpu, net: L.LearnedDenoiser, pairs, steps: int = 1500, batch: int = 8, crop_w: int = 128,
-          crop_h: int = 128, lr: float = 1e-3, seed: int = 11, log_every: int = 0, log=None):
-    """Adam on random crops of random pairs; returns the per-step losses."""
+          crop_h: int = 128, lr: float = 1e-3, seed: int = 11, log_every: int = 0, log=None, ema: float = 0.0,
+          ema_every: int = 10):
+    """Adam on random crops of random pairs; returns the per-step losses.
+    With ema > 0 the network ends with a moving average of its weights over
+    the second half of training (decay `ema` per step, sampled every
+    `ema_every` steps) instead of the last step's weights."""
     rng = np.random.default_rng(seed)
     losses = []
+    avg = None
+    beta = ema ** ema_every if ema > 0 else 0.0
     for step in range(steps):
         idx = rng.integers(0, len(pairs), size=batch)
         ins = [pairs[i][0] for i in idx]
@@ -102,8 +108,13 @@ def train(gpu: L.Gpu, net: L.LearnedDenoiser, pairs, steps: int = 1500, batch: i
         # cosine decay to a tenth of the initial rate
         rate = lr * (0.1 + 0.9 * 0.5 * (1.0 + math.cos(math.pi * step / max(steps, 1))))
         losses.append(net.train_step(ins, refs, crop_w, crop_h, crops=crops, lr=rate))
+        if ema > 0 and step >= steps // 2 and (step + 1) % ema_every == 0:
+            p = net.params().astype(np.float64)
+            avg = p if avg is None else beta * avg + (1.0 - beta) * p
         if log and log_every and (step + 1) % log_every == 0:
             log(f"step {step + 1}: loss {np.mean(losses[-log_every:]):.5f}")
+    if avg is not None:
+        net.set_params(avg.astype(np.float32))
     gpu.check_errors()
     return losses
 
@@ -169,6 +180,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--crop", type=int, default=128, help="training crop side")
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--ema", type=float, default=0.0,
+                    help="end with a moving average of the weights over the second half (decay per step, e.g. 0.999)")
     ap.add_argument("--out", type=str, default="", help="write the trained parameters (f32) here")
     ap.add_argument("--eval", action="store_true", help="score the network on the held-out scenes")
     ap.add_argument("--json", type=str, default="", help="write a JSON summary of the run here")
@@ -181,13 +194,13 @@ def main() -> int:
     say(f"data: {len(pairs)} pairs, {paths / 1e9:.2f} Gpaths in {t1 - t0:.1f} s")
     net = L.LearnedDenoiser(gpu, seed=1)
     losses = train(gpu, net, pairs, a.steps, a.batch, crop_w=a.crop, crop_h=a.crop, lr=a.lr, log_every=500,
-                   log=say)
+                   log=say, ema=a.ema)
     t2 = time.perf_counter()
     del pairs
     say(f"trained {a.steps} steps in {t2 - t1:.1f} s, final loss {np.mean(losses[-100:]):.5f}")
     if a.out:
         net.params().tofile(a.out)
-    summary = {"scenes": a.scenes, "input_spp": list(INPUT_SPP), "target_spp": a.ref_spp, "crop": a.crop,
+    summary = {"scenes": a.scenes, "input_spp": list(INPUT_SPP), "target_spp": a.ref_spp, "crop": a.crop, "ema": a.ema,
                "batch": a.batch, "lr": a.lr, "steps": a.steps, "params": L.LearnedDenoiser.param_count(),
                "input_channels": 16, "data_paths": paths, "data_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 1),
                "loss_first_500": float(np.mean(losses[:500])), "loss_last_500": float(np.mean(losses[-500:])),

@@ -18,10 +18,9 @@ holds: it then compares two noisy estimates of the reference's own noise,
 so it is reported and not asserted.
 
 An untrained network (output layer at zero) must return the a-trous estimate
-itself, so any gain is the training's. (The weight gradients are summed with
-float atomics, so two training runs differ in their last bits and drift
-apart over the steps; the run is sized so that the margins hold across
-runs.)"""
+itself, so any gain is the training's. (Training is deterministic: the
+weight gradients are summed in a fixed order, so this test scores the same
+network on every run.)"""
 from __future__ import annotations
 
 import sys

@@ -222,6 +222,29 @@ def test_training_reduces_the_loss(gpu):
     assert last < 0.5 * first, (first, last)
 
 
+def test_training_is_deterministic(gpu):
+    """The weight gradients are summed in a fixed order (per-slice partials,
+    then one ordered sum), so two training runs from the same start on the
+    same crops end with bit-identical weights."""
+    ins, refs = [], []
+    for s in (41, 42):
+        f = L.Film(gpu, FW, FH)
+        f.upload(synthetic_accum(FW, FH, s, noise=0.6))
+        r = L.Film(gpu, FW, FH)
+        r.upload(synthetic_accum(FW, FH, s, noise=0.0))
+        ins.append(f)
+        refs.append(r)
+    say("net determinism case: two runs of 20 Adam steps, batch 2 crops 128x16")
+    runs = []
+    for _ in range(2):
+        net = L.LearnedDenoiser(gpu, seed=26)
+        losses = [net.train_step(ins, refs, CW, CH, crops=CROPS, lr=2e-3) for _ in range(20)]
+        runs.append((net.params().copy(), net.params(gradients=True).copy(), losses))
+    np.testing.assert_array_equal(runs[0][1].view(np.uint32), runs[1][1].view(np.uint32))
+    np.testing.assert_array_equal(runs[0][0].view(np.uint32), runs[1][0].view(np.uint32))
+    assert not np.array_equal(runs[0][0], L.LearnedDenoiser(gpu, seed=26).params())  # the steps moved the weights
+
+
 def test_denoise_matches_oracle(gpu):
     ins, raw = films(gpu, 128, 32, [31])
     out = L.Film(gpu, 128, 32)
