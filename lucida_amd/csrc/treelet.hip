@@ -9,10 +9,11 @@
 // bucketed by depth and one launch per level, deepest first, handles that
 // level. A node of at least LUCIDA_TREELET_MIN_SIZE primitives gets a whole
 // wave, which grows a treelet of up to 7 leaves under the node (always
-// opening the leaf of largest surface area), finds the treelet topology of
-// least cost by dynamic
-// programming over the 127 leaf subsets (subset boxes and counts in LDS, one
-// subset size after another), and rewrites the treelet's internal nodes in
+// opening the leaf of largest surface area; the three levels under the
+// node are fetched across lanes first), finds the treelet topology of least
+// cost by dynamic programming over the 127 leaf subsets (subset boxes and
+// counts in LDS, one subset size after another, each subset's partitions
+// spread over a group of lanes), and rewrites the treelet's internal nodes in
 // place. Smaller nodes only take their cost from their children. A node
 // costs CI x area plus its children's costs, or CT x area x primitives as a
 // leaf when it holds at most LUCIDA_TREELET_MAX_LEAF; the cheaper one wins,
@@ -125,17 +126,24 @@ __device__ inline uint32_t deposit(uint32_t j, uint32_t mask) {
 // their proper subsets (already in LDS). A subset's partitions keep its
 // lowest leaf on the left; partition j's left side is that leaf plus the
 // bits of j deposited on the others, so ascending j is ascending mask. The
-// lanes sharing a subset take every G-th partition, keep their first least
-// cost, and then agree on the least (cost, mask) pair: the first least cost
-// in ascending mask order. Costs that do not compare below infinity (NaN) are
-// never chosen; with none below infinity the split is the lowest leaf alone.
+// split chosen is the least (cost, imbalance, mask), imbalance being the
+// difference of the two sides' primitive counts, with costs compared to 20
+// mantissa bits (cost_key): among costs equal up to their rounding (boxes
+// that coincide, summed in different orders) the most even split, so ties do
+// not build chains. The lanes sharing a subset take every G-th partition,
+// keep their first least triple, and then agree on the least. A NaN cost is
+// never chosen; with every cost NaN the split is the lowest leaf alone.
+// A cost with its 4 lowest mantissa bits cleared: monotone in the cost
+// (costs are not negative), so comparing keys is a total order.
+__device__ inline float cost_key(float c) { return __uint_as_float(__float_as_uint(c) & 0xFFFFFFF0u); }
+
 __device__ inline void solve_size(TreeletLds& L, uint32_t lane, int m, uint32_t full) {
     const uint32_t gs = GROUP_SHIFT[m];
     const uint32_t g = lane >> gs, r = lane & ((1u << gs) - 1u);
     const uint32_t first = c_subsets.off[m], nsub = c_subsets.off[m + 1] - first;
     const float inf = __builtin_huge_valf();
     float key = inf;
-    uint32_t bp = 0xFFu, s = 0;
+    uint32_t bp = 0xFFu, bim = 0xFFFFFFFFu, s = 0;
     if (g < nsub) {
         s = c_subsets.mask[first + g];
         if (s <= full) {
@@ -144,8 +152,13 @@ __device__ inline void solve_size(TreeletLds& L, uint32_t lane, int m, uint32_t 
             for (uint32_t j = r; j < parts; j += 1u << gs) {
                 const uint32_t pm = low | deposit(j, rest);
                 const float cand = L.cost[pm] + L.cost[s ^ pm];
-                if (cand < key) {
-                    key = cand;
+                if (!(cand == cand)) continue;  // NaN
+                const float qc = cost_key(cand);
+                const uint32_t np = L.count[pm], nq = L.count[s ^ pm];
+                const uint32_t im = np > nq ? np - nq : nq - np;
+                if (qc < key || (qc == key && im < bim)) {
+                    key = qc;
+                    bim = im;
                     bp = pm;
                 }
             }
@@ -153,16 +166,22 @@ __device__ inline void solve_size(TreeletLds& L, uint32_t lane, int m, uint32_t 
     }
     for (uint32_t o = 1; o < (1u << gs); o <<= 1) {
         const float ok = __shfl_xor(key, o, 64);
+        const uint32_t oi = __shfl_xor(bim, o, 64);
         const uint32_t op = __shfl_xor(bp, o, 64);
-        if (ok < key || (ok == key && op < bp)) {
+        if (ok < key || (ok == key && (oi < bim || (oi == bim && op < bp)))) {
             key = ok;
+            bim = oi;
             bp = op;
         }
     }
     if (r == 0 && g < nsub && s <= full) {
-        if (bp == 0xFFu) bp = s & (0u - s);
+        float best = inf;
+        if (bp == 0xFFu)
+            bp = s & (0u - s);
+        else
+            best = L.cost[bp] + L.cost[s ^ bp];
         const float a = area(L.box[0][s], L.box[1][s], L.box[2][s], L.box[3][s], L.box[4][s], L.box[5][s]);
-        const float inner = CI * a + key;
+        const float inner = CI * a + best;
         const float leaf = CT * a * (float)L.count[s];
         const bool as_leaf = L.count[s] <= MAX_LEAF && leaf <= inner;
         L.cost[s] = as_leaf ? leaf : inner;
@@ -187,16 +206,20 @@ __device__ void restructure(const TreeArrays& t, TreeletLds& L, uint32_t lane, u
             hid = (lane & 1u) ? c.y : c.x;
         }
     }
-    const float harea = lane < 14 && hid < t.leaf_base ? node_area(t.box, hid) : -1.0f;
-    // Grow: lane i < k holds leaf i (id, area, heap slot or NONE); the leaf of
-    // largest area (the first such) is replaced by its left child and its
-    // right child appended.
+    const bool hin = lane < 14 && hid < t.leaf_base;
+    const float harea = hin ? node_area(t.box, hid) : -1.0f;
+    const uint32_t hsize = hin ? t.size[hid] : 0u;
+    // Grow: lane i < k holds leaf i (id, area, primitives, heap slot or NONE);
+    // the leaf of largest area, then most primitives (the first such), is
+    // replaced by its left child and its right child appended.
     uint32_t id = __shfl(hid, lane < 2 ? lane : 0u, 64);
     float id_area = __shfl(harea, lane < 2 ? lane : 0u, 64);
+    uint32_t id_size = __shfl(hsize, lane < 2 ? lane : 0u, 64);
     uint32_t hpos = lane;
     if (lane >= 2) {
         id = NONE;
         id_area = -1.0f;
+        id_size = 0;
         hpos = NONE;
     }
     uint32_t inner_id = lane == 0 ? x : NONE;
@@ -204,17 +227,20 @@ __device__ void restructure(const TreeArrays& t, TreeletLds& L, uint32_t lane, u
     while (k < LEAVES) {
         int best = -1;
         float best_a = -1.0f;
+        uint32_t best_n = 0;
         for (int i = 0; i < k; ++i) {
             const float a = __shfl(id_area, i, 64);
-            if (a > best_a) {
+            const uint32_t sz = __shfl(id_size, i, 64);
+            if (a >= 0.0f && (a > best_a || (a == best_a && sz > best_n))) {
                 best_a = a;
+                best_n = sz;
                 best = i;
             }
         }
         if (best < 0) break;
         const uint32_t e = __shfl(id, best, 64);
         const uint32_t ep = __shfl(hpos, best, 64);
-        uint32_t cx, cy, px = NONE, py = NONE;
+        uint32_t cx, cy, px = NONE, py = NONE, nx, ny;
         float ax, ay;
         if (ep < 6) {  // its children are in the heap
             px = 2 * ep + 2;
@@ -223,22 +249,28 @@ __device__ void restructure(const TreeArrays& t, TreeletLds& L, uint32_t lane, u
             cy = __shfl(hid, py, 64);
             ax = __shfl(harea, px, 64);
             ay = __shfl(harea, py, 64);
+            nx = __shfl(hsize, px, 64);
+            ny = __shfl(hsize, py, 64);
         } else {
             const uint2 ce = t.child[e];
             cx = ce.x;
             cy = ce.y;
             ax = cx < t.leaf_base ? node_area(t.box, cx) : -1.0f;
             ay = cy < t.leaf_base ? node_area(t.box, cy) : -1.0f;
+            nx = cx < t.leaf_base ? t.size[cx] : 0u;
+            ny = cy < t.leaf_base ? t.size[cy] : 0u;
         }
         if (lane == (uint32_t)(k - 1)) inner_id = e;
         if (lane == (uint32_t)best) {
             id = cx;
             id_area = ax;
+            id_size = nx;
             hpos = px;
         }
         if (lane == (uint32_t)k) {
             id = cy;
             id_area = ay;
+            id_size = ny;
             hpos = py;
         }
         ++k;

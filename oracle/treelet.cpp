@@ -13,8 +13,10 @@
 //
 // Parity with the device pass is exact: both follow one schedule (children
 // before parents, so each node sees its subtree's final state), one set of
-// tie-breaks (first largest area, smallest partition mask, leaf on equal
-// cost), and the same float operations with contraction off. The optimality
+// tie-breaks (largest area then most primitives, then the first, when
+// growing a treelet; least cost, then the most even split, then the smallest
+// mask, when splitting; a leaf on equal cost), and the same float operations
+// with contraction off. The optimality
 // of the subset search is checked against brute force over every binary tree
 // of up to 6 leaves, and the trees it emits against validate() and against
 // closest hits over the unrestructured LBVH.
@@ -47,6 +49,17 @@ inline void grow(Box& a, const Box& b) {
 }
 
 inline int popcount(uint32_t v) { return __builtin_popcount(v); }
+
+// A cost with its 4 lowest mantissa bits cleared (costs are not negative, so
+// the keys order as the costs do).
+inline float cost_key(float c) {
+    uint32_t u;
+    std::memcpy(&u, &c, 4);
+    u &= 0xFFFFFFF0u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
 inline int lowest(uint32_t v) { return __builtin_ctz(v); }
 
 struct Tree {
@@ -78,9 +91,13 @@ void combine(Tree& t, const Params& p, uint32_t x) {
 
 // Optimal topology over k <= 7 treelet leaves: for every subset s of the
 // leaves, c[s] is the least cost of a subtree over exactly those leaves and
-// part[s] the left side of its best split (the partition masks are tried in
-// ascending order and the first least cost is kept; a partition's left side
-// holds the subset's lowest leaf).
+// part[s] the left side of its best split. A partition's left side holds the
+// subset's lowest leaf; the split kept is the least (cost, imbalance, mask),
+// imbalance being the difference of the two sides' primitive counts and
+// costs compared to 20 mantissa bits (cost_key), so costs equal up to their
+// rounding (coinciding boxes, summed in different orders) take the most even
+// split instead of building chains. A NaN cost is never kept; with every
+// cost NaN the split is the lowest leaf alone.
 void treelet_dp(int k, const Box* leaf_box, const float* leaf_cost, const uint32_t* leaf_size, float ci, float ct,
                 uint32_t max_leaf, Box* box, float* c, uint8_t* part, uint8_t* leafy) {
     const uint32_t full = (1u << k) - 1;
@@ -106,16 +123,26 @@ void treelet_dp(int k, const Box* leaf_box, const float* leaf_cost, const uint32
     for (uint32_t s = 1; s <= full; ++s) {
         if (popcount(s) < 2) continue;
         const uint32_t low = s & (0u - s), rest = s ^ low;
-        float best = std::numeric_limits<float>::infinity();
-        uint32_t best_p = low;  // a valid split even if no cost compares below infinity
+        float best_key = std::numeric_limits<float>::infinity();
+        uint32_t best_p = 0, best_im = 0xFFFFFFFFu;
         for (uint32_t q = 0; q != rest; q = (q - rest) & rest) {
             const uint32_t pm = low | q;
             const float cand = c[pm] + c[s ^ pm];
-            if (cand < best) {
-                best = cand;
+            if (!(cand == cand)) continue;  // NaN
+            const float qc = cost_key(cand);
+            const uint32_t np = count[pm], nq = count[s ^ pm];
+            const uint32_t im = np > nq ? np - nq : nq - np;
+            if (qc < best_key || (qc == best_key && im < best_im)) {
+                best_key = qc;
+                best_im = im;
                 best_p = pm;
             }
         }
+        float best = std::numeric_limits<float>::infinity();
+        if (best_p == 0)
+            best_p = low;  // every cost NaN
+        else
+            best = c[best_p] + c[s ^ best_p];
         const float inner = ci * a[s] + best;
         const float leaf = ct * a[s] * (float)count[s];
         const bool as_leaf = count[s] <= max_leaf && leaf <= inner;
@@ -134,13 +161,17 @@ void restructure(Tree& t, const Params& p, uint32_t x) {
     T[1] = t.right[x];
     I[0] = x;
     while (k < 7) {
+        // the leaf of largest area, then most primitives (the first such)
         int best = -1;
         float best_a = -1.0f;
+        uint32_t best_n = 0;
         for (int i = 0; i < k; ++i) {
             if (T[i] >= t.leaf_base) continue;
             const float a = area(t.box[T[i]]);
-            if (a > best_a) {
+            const uint32_t sz = t.size[T[i]];
+            if (a >= 0.0f && (a > best_a || (a == best_a && sz > best_n))) {
                 best_a = a;
+                best_n = sz;
                 best = i;
             }
         }

@@ -130,6 +130,29 @@ def test_subset_search_prefers_a_leaf_on_a_tie(oracle):
     assert leafy[3] == 0 and c[3] == pytest.approx(19.2, rel=1e-6)
 
 
+def test_subset_search_breaks_ties_towards_even_splits(oracle):
+    # Four coinciding boxes of two primitives each (too many for one leaf):
+    # every topology costs the same, and the even split {0,1} | {2,3} is kept
+    # (of the even splits, the one of smallest mask), not a chain.
+    boxes = np.zeros(4, L.AABB)
+    boxes["hi"] = 1.0
+    c, part, leafy = oracle.treelet_dp(boxes, np.full(4, 12.0, np.float32), np.full(4, 2, np.uint32))
+    assert part[15] == 0b0011 and leafy[15] == 0
+
+
+def test_coinciding_triangles_keep_the_tree_shallow(oracle):
+    # Thousands of identical triangles: all boxes coincide, so every treelet
+    # topology ties; the restructured tree stays about as deep as the LBVH.
+    for n in (1000, 20000):
+        pos = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [0, 1, 0, 0]], np.float32)
+        tris = np.zeros(n, L.TRIANGLE)
+        tris["i1"], tris["i2"] = 1, 2
+        t = oracle.treelet_build(tris, pos)
+        assert oracle.validate(t["nodes"], t["prim_indices"], n) == 0
+        lb = depth_of_leaves(oracle.lbvh_build(tris, pos)["nodes"]).max()
+        assert depth_of_leaves(t["nodes"]).max() <= lb + 2, (n, lb)
+
+
 def depth_of_leaves(nodes):
     d = np.zeros(len(nodes), np.int64)
     for i in range(len(nodes)):

@@ -37,10 +37,21 @@ CASES = ["icosphere-0", "icosphere-3", "icosphere-5", "cornell-mesh", "box-c1", 
          "n=1", "n=2", "n=3", "n=5", "n=33", "n=1023", "n=1025", "n=4097"]
 
 
-@pytest.mark.parametrize("case", CASES)
+def coinciding(n: int):
+    pos = np.array([[0, 0, 0, 0], [1, 0, 0, 0], [0, 1, 0, 0]], np.float32)
+    tris = np.zeros(n, L.TRIANGLE)
+    tris["i1"], tris["i2"] = 1, 2
+    return tris, pos
+
+
+@pytest.mark.parametrize("case", CASES + ["coinciding=20000"])
 def test_treelet_tree_matches_oracle(gpu, oracle, case):
-    tris, pos = case_geometry(case) if case in ("flat", "duplicates", "nan") or case.startswith("n=") \
-        else geometry(case)
+    if case.startswith("coinciding="):
+        tris, pos = coinciding(int(case.split("=")[1]))
+    elif case in ("flat", "duplicates", "nan") or case.startswith("n="):
+        tris, pos = case_geometry(case)
+    else:
+        tris, pos = geometry(case)
     for rounds in (1, 3):
         say(f"treelet case: {case} n={len(tris)} rounds={rounds}")
         nodes, prims = L.LbvhBuilder(gpu).build_treelet(tris, pos, rounds)
