@@ -154,12 +154,14 @@ __device__ inline void solve_size(TreeletLds& L, uint32_t lane, int m, uint32_t 
                 const float cand = L.cost[pm] + L.cost[s ^ pm];
                 if (!(cand == cand)) continue;  // NaN
                 const float qc = cost_key(cand);
-                const uint32_t np = L.count[pm], nq = L.count[s ^ pm];
-                const uint32_t im = np > nq ? np - nq : nq - np;
-                if (qc < key || (qc == key && im < bim)) {
-                    key = qc;
-                    bim = im;
-                    bp = pm;
+                if (qc <= key) {  // the counts are read only for a lower or equal key
+                    const uint32_t np = L.count[pm], nq = L.count[s ^ pm];
+                    const uint32_t im = np > nq ? np - nq : nq - np;
+                    if (qc < key || im < bim) {
+                        key = qc;
+                        bim = im;
+                        bp = pm;
+                    }
                 }
             }
         }
