@@ -518,7 +518,7 @@ typedef struct lucida_wavefront_config {
     uint32_t profile_kernels;   /* time every kernel with HIP events (single renders)             */
     uint32_t ray_order;         /* 0: sort each bounce's extension and shadow queues by ray        *
                                  * direction and origin when the scene's BVH has at least 4096     *
-                                 * nodes and a batch holds at least 2^20 paths; 1: never;          *
+                                 * nodes and a batch holds at least 2^21 paths; 1: never;          *
                                  * 2: whenever there is a BVH.                                     *
                                  * The order changes which lanes trace which rays together, never  *
                                  * a path's result: images are bit-identical either way. Rounds of *

@@ -1741,13 +1741,14 @@ static void launch_connect(int mode, dim3 grid, hipStream_t s, const SceneView& 
 // (lucida_wavefront_config.ray_order). The sorts cost a fixed ~0.1 ms per
 // bounce plus their passes; they pay off when the walks are long (a large
 // tree) and the queues long enough to group similar rays. On the 1.05M-
-// triangle scene at 1024^2 (two bands of rows per one-batch render): 1 spp,
-// 2^19 paths per band, 7.2 ms unsorted against 9.1 sorted; 2 spp 12.4
-// against 12.2; 4 spp 23.8 against 19.2.
+// triangle scene's restructured tree at 1024^2, 8 bounces (two bands of rows
+// per one-batch render): 1 spp, 2^19 paths per band, 5.3 ms unsorted against
+// 7.2 sorted; 2 spp, 2^20 per band, 8.6 against 9.5; 4 spp, 2^21 per band,
+// 16.2 against 14.2.
 static bool want_ray_order(const lucida_wavefront_config& cfg, const SceneView& sc, size_t paths) {
     if (sc.num_nodes == 0 || cfg.ray_order == 1) return false;
     if (cfg.ray_order == 2) return true;
-    return sc.num_nodes >= 4096 && paths >= ((size_t)1 << 20);
+    return sc.num_nodes >= 4096 && paths >= ((size_t)1 << 21);
 }
 
 // The sorts after SHADE of bounce b over a batch of P paths, by `bits` key
