@@ -46,6 +46,27 @@ struct DevBuf {
     T* as() const { return reinterpret_cast<T*>(ptr); }
 };
 
+// The timing events of one call, destroyed on every return path.
+template <int N>
+struct StageEvents {
+    hipEvent_t ev[N] = {};
+    hipError_t create() {
+        for (auto& e : ev) {
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+    hipEvent_t& operator[](int k) { return ev[k]; }
+    StageEvents() = default;
+    StageEvents(const StageEvents&) = delete;
+    StageEvents& operator=(const StageEvents&) = delete;
+    ~StageEvents() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 // One wavefront pipeline: a stream for GENERATE, EXTEND, SHADE, the
 // extension sort and RESOLVE, a second one (default priority, like the
 // first) for the shadow sort and CONNECT, the events ordering them, and the

@@ -575,6 +575,17 @@ __global__ __launch_bounds__(256) void k_depth(uint32_t n, const uint32_t* __res
         if (h[i]) atomicAdd(&level_count[i], h[i]);
 }
 
+// Bucket cursors: the exclusive prefix sum of the per-level counts (one lane;
+// MAX_LEVELS entries).
+__global__ void k_level_offsets(const uint32_t* __restrict__ count, uint32_t* __restrict__ cursor) {
+    if (threadIdx.x != 0) return;
+    uint32_t acc = 0;
+    for (int l = 0; l < MAX_LEVELS; ++l) {
+        cursor[l] = acc;
+        acc += count[l];
+    }
+}
+
 // Scatters node ids into per-level lists: cursor[d] starts at level d's offset.
 __global__ __launch_bounds__(256) void k_bucket(uint32_t n, const uint32_t* __restrict__ depth,
                                                 uint32_t* __restrict__ cursor, uint32_t* __restrict__ list) {
@@ -1090,8 +1101,8 @@ static lucida_status fit_levels(lucida_ctx* ctx, size_t n) {
         acc += host[l];
         if (host[l]) deepest = l;
     }
-    // Blocking copy: the stream is idle here (synchronised above) and `off` is a stack array.
-    LUCIDA_HIP(hipMemcpy(cursor, off, sizeof off, hipMemcpyHostToDevice), LUCIDA_VALIDATION);
+    // the device forms the same offsets as its bucket cursors, in stream order
+    hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, s, level_count, cursor);
     hipLaunchKernelGGL(k_bucket, dim3(blocks_for(internal, 256)), dim3(256), 0, s, (uint32_t)n, depth, cursor, list);
     for (int l = deepest; l >= 0; --l) {
         if (!host[l]) continue;
@@ -1159,10 +1170,9 @@ static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size
 
     uint32_t* misc = ctx->lb_misc.as<uint32_t>();  // [8..16) scene box
     LucidaAabb* scene_box = reinterpret_cast<LucidaAabb*>(misc + 8);
-    hipEvent_t ev[6];
+    StageEvents<5> ev;
     const bool timed = stage_us != nullptr;
-    if (timed)
-        for (auto& e : ev) LUCIDA_HIP(hipEventCreate(&e), LUCIDA_VALIDATION);
+    if (timed) LUCIDA_HIP(ev.create(), LUCIDA_VALIDATION);
     if (timed) LUCIDA_HIP(hipEventRecord(ev[0], s), LUCIDA_VALIDATION);
 
     hipLaunchKernelGGL(k_prepare, dim3(nb), dim3(PREP_BLOCK), 0, s, tris, reinterpret_cast<const float4*>(positions),
@@ -1213,7 +1223,6 @@ static lucida_status build_stages(lucida_ctx* ctx, const GpuTriangle* tris, size
         LUCIDA_HIP(hipEventSynchronize(ev[4]), LUCIDA_VALIDATION);
         for (int k = 0; k < 4; ++k) LUCIDA_HIP(hipEventElapsedTime(&stage_us[k], ev[k], ev[k + 1]), LUCIDA_VALIDATION);
         for (int k = 0; k < 4; ++k) stage_us[k] *= 1000.0f;
-        for (auto& e : ev) (void)hipEventDestroy(e);
     }
     return LUCIDA_OK;
 }
@@ -1255,10 +1264,11 @@ lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t
     float us[4] = {0, 0, 0, 0};
     lucida_status st = build_stages(ctx, tris, n, positions, stage_us ? us : nullptr, false);
     if (st != LUCIDA_OK) return st;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    StageEvents<2> ev;
+    hipEvent_t& e0 = ev[0];
+    hipEvent_t& e1 = ev[1];
     if (stage_us) {
-        LUCIDA_HIP(hipEventCreate(&e0), LUCIDA_VALIDATION);
-        LUCIDA_HIP(hipEventCreate(&e1), LUCIDA_VALIDATION);
+        LUCIDA_HIP(ev.create(), LUCIDA_VALIDATION);
         LUCIDA_HIP(hipEventRecord(e0, ctx->stream), LUCIDA_VALIDATION);
     }
     if (n == 1) {
@@ -1289,8 +1299,6 @@ lucida_status lbvh_build_device(lucida_ctx* ctx, const GpuTriangle* tris, size_t
         LUCIDA_HIP(hipEventElapsedTime(&ms, e0, e1), LUCIDA_VALIDATION);
         for (int k = 0; k < 4; ++k) stage_us[k] = us[k];
         stage_us[4] = ms * 1000.0f;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
     }
     return LUCIDA_OK;
 }

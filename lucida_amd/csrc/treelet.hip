@@ -469,6 +469,17 @@ __global__ __launch_bounds__(256) void k_clear_words(uint32_t* __restrict__ p, u
     if (i < count) p[i] = 0u;
 }
 
+// Bucket cursors: the exclusive prefix sum of the per-(level, class) counts
+// (one lane; 256 entries).
+__global__ void k_tl_offsets(const uint32_t* __restrict__ count, uint32_t* __restrict__ cursor) {
+    if (threadIdx.x != 0) return;
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < 2 * MAX_LEVELS; ++k) {
+        cursor[k] = acc;
+        acc += count[k];
+    }
+}
+
 // Layout: an emitted internal node of pre-order rank R among emitted internal
 // nodes has its children at 1 + 2R and 2 + 2R. Each node finds its rank, its
 // parent's rank and its first primitive by walking to the root: R counts each
@@ -548,9 +559,9 @@ lucida_status bvh_build_treelet_device(lucida_ctx* ctx, const GpuTriangle* tris,
         return lbvh_build_device(ctx, tris, n, positions, nodes, num_nodes, prim_indices, tris_sorted, nullptr);
     if (n > (1u << 30)) return set_error(LUCIDA_INVALID_ARG, "treelet BVH: more than 2^30 triangles");
     hipStream_t s = ctx->stream;
-    hipEvent_t ev[4] = {};
+    StageEvents<4> ev;
     if (stage_us) {
-        for (auto& e : ev) LUCIDA_HIP(hipEventCreate(&e), LUCIDA_VALIDATION);
+        LUCIDA_HIP(ev.create(), LUCIDA_VALIDATION);
         LUCIDA_HIP(hipEventRecord(ev[0], s), LUCIDA_VALIDATION);
     }
     lucida_status st = lbvh_fit_device(ctx, tris, n, positions);
@@ -593,8 +604,8 @@ lucida_status bvh_build_treelet_device(lucida_ctx* ctx, const GpuTriangle* tris,
             acc += host[k];
             if (host[k]) deepest = (int)(k / 2);
         }
-        // Blocking copy: the stream is idle here (synchronised above) and `off` is a stack array.
-        LUCIDA_HIP(hipMemcpy(cursor, off, sizeof off, hipMemcpyHostToDevice), LUCIDA_VALIDATION);
+        // the device forms the same offsets as its bucket cursors, in stream order
+        hipLaunchKernelGGL(k_tl_offsets, dim3(1), dim3(64), 0, s, level_count, cursor);
         hipLaunchKernelGGL(k_tl_bucket, dim3(blocks_for(internal, 256)), dim3(256), 0, s, (uint32_t)internal, key,
                            cursor, list);
         for (int d = deepest; d >= 0; --d) {
@@ -628,7 +639,6 @@ lucida_status bvh_build_treelet_device(lucida_ctx* ctx, const GpuTriangle* tris,
             LUCIDA_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), LUCIDA_VALIDATION);
             stage_us[k] = ms * 1000.0f;
         }
-        for (auto& e : ev) (void)hipEventDestroy(e);
     }
     return LUCIDA_OK;
 }
