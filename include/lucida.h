@@ -354,15 +354,17 @@ lucida_status lucida_lbvh_build(lucida_ctx* ctx, const GpuTriangle* tris, size_t
  * A node costs LUCIDA_TREELET_CI x its surface area plus its children's
  * costs, a leaf LUCIDA_TREELET_CT x area x primitives. Output layout as
  * lucida_lbvh_build's (children of an internal node adjacent, depth-first
- * left-first order, leaves index prim_indices_out); rounds 0 is taken as 1.
- * The closest hit along any ray is the one over the LBVH, up to ties at
- * equal distance. */
+ * left-first order, leaves index prim_indices_out); rounds 0 is taken as 1,
+ * more than LUCIDA_TREELET_MAX_ROUNDS is LUCIDA_INVALID_ARG (the cost stops
+ * falling after a few rounds). The closest hit along any ray is the one over
+ * the LBVH, up to ties at equal distance. */
 #define LUCIDA_TREELET_LEAVES 7
 #define LUCIDA_TREELET_MIN_SIZE 7
 #define LUCIDA_TREELET_MAX_LEAF 4
 #define LUCIDA_TREELET_CI 1.2f
 #define LUCIDA_TREELET_CT 1.0f
 #define LUCIDA_TREELET_ROUNDS 3
+#define LUCIDA_TREELET_MAX_ROUNDS 64
 lucida_status lucida_bvh_build_treelet(lucida_ctx* ctx, const GpuTriangle* tris, size_t n, const float* positions,
                                        size_t num_positions, uint32_t rounds, GpuBvhNode* nodes_out,
                                        size_t* num_nodes, uint32_t* prim_indices_out);

@@ -555,6 +555,8 @@ lucida_status bvh_build_treelet_device(lucida_ctx* ctx, const GpuTriangle* tris,
                                        uint32_t rounds, GpuBvhNode* nodes, size_t* num_nodes,
                                        uint32_t* prim_indices, GpuTriangle* tris_sorted, float* stage_us) {
     if (stage_us) stage_us[0] = stage_us[1] = stage_us[2] = 0.0f;
+    if (rounds > LUCIDA_TREELET_MAX_ROUNDS)
+        return set_error(LUCIDA_INVALID_ARG, "treelet BVH: more than LUCIDA_TREELET_MAX_ROUNDS (64) rounds");
     if (n < 2)  // nothing to restructure: the LBVH's single leaf
         return lbvh_build_device(ctx, tris, n, positions, nodes, num_nodes, prim_indices, tris_sorted, nullptr);
     if (n > (1u << 30)) return set_error(LUCIDA_INVALID_ARG, "treelet BVH: more than 2^30 triangles");

@@ -85,6 +85,20 @@ def test_treelet_after_other_builds_reuses_buffers(gpu, oracle):
     np.testing.assert_array_equal(nodes.view(np.uint32), ref["nodes"].view(np.uint32))
 
 
+def test_treelet_rounds_are_bounded(gpu, oracle):
+    """More than 64 rounds is rejected before any work; 64 and 0 (taken as
+    one) still build the oracle's tree."""
+    tris, pos = adversarial("n=33")
+    b = L.LbvhBuilder(gpu)
+    say("treelet rounds case: n=33 rounds=65")
+    with pytest.raises(L.GpuError, match="rounds"):
+        b.build_treelet(tris, pos, 65)
+    for rounds in (64, 0):
+        say(f"treelet rounds case: n=33 rounds={rounds}")
+        nodes, prims = b.build_treelet(tris, pos, rounds)
+        assert_same_tree(nodes, prims, oracle.treelet_build(tris, pos, max(rounds, 1)), f"rounds={rounds}")
+
+
 def test_scene_treelet_is_device_resident_and_compacted(gpu, oracle):
     b = L.Blob("cornell-mesh")
     a = b.arrays()
