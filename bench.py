@@ -390,6 +390,8 @@ def main() -> int:
     ap.add_argument("--net-ref-spp", type=int, default=4096, help="target samples per pixel of the training pairs")
     ap.add_argument("--net-steps", type=int, default=4000, help="Adam steps (0: no learned denoiser)")
     ap.add_argument("--net-crop", type=int, default=256, help="training crop side (8 crops per step)")
+    ap.add_argument("--net-ema", type=float, default=0.9995,
+                    help="decay of the moving average of the weights over the second half of training (0: last weights)")
     ap.add_argument("--serialize", action="store_true",
                     help="time the steps serialized (one pipeline, every kernel alone on one stream), as the "
                          "rocprofv3 counter runs see them")
@@ -634,13 +636,13 @@ def main() -> int:
             t1 = time.perf_counter()
             net = L.LearnedDenoiser(gpu, seed=1)
             losses = T.train(gpu, net, pairs, a.net_steps, crop_w=a.net_crop, crop_h=a.net_crop, log_every=1000,
-                             log=progress)
+                             log=progress, ema=a.net_ema)
             t2 = time.perf_counter()
             del pairs
             out["learned_denoiser"] = {
                 "params": L.LearnedDenoiser.param_count(),
                 "training_scenes": a.net_scenes, "crops": f"8 x {a.net_crop}x{a.net_crop} per step",
-                "steps": a.net_steps,
+                "steps": a.net_steps, "weight_average_decay": a.net_ema,
                 "target_spp": a.net_ref_spp, "data_paths": data_paths, "data_s": round(t1 - t0, 2),
                 "train_s": round(t2 - t1, 2), "ms_per_step": round((t2 - t1) / a.net_steps * 1e3, 3),
                 "loss_first_100": float(np.mean(losses[:100])), "loss_last_100": float(np.mean(losses[-100:])),
