@@ -522,14 +522,17 @@ typedef struct lucida_wavefront_config {
                                  * direction and origin when the scene's BVH has at least 4096     *
                                  * nodes and a batch holds at least 2^21 paths; 1: never;          *
                                  * 2: whenever there is a BVH.                                     *
-                                 * The order changes which lanes trace which rays together, never  *
-                                 * a path's result: images are bit-identical either way. Rounds of *
-                                 * a batched render that mix jobs sort by (job, ray), up to 256    *
-                                 * jobs per round.                                                 */
+                                 * The order changes which lanes trace which rays together, not    *
+                                 * what a ray hits, up to rounding: a ray meeting two triangles at *
+                                 * distances equal to within rounding may keep either (the walk's  *
+                                 * speculative steps depend on the rays beside it). Images were    *
+                                 * bit-identical either way on every scene tested. Rounds of a     *
+                                 * batched render that mix jobs sort by (job, ray), up to 256 jobs *
+                                 * per round.                                                     */
     uint32_t serialize;         /* single renders, for measurement: one pipeline, and CONNECT and  *
                                  * the shadow sort on the main stream instead of beside the next  *
                                  * EXTEND, so per-kernel times (profile_kernels) are solo times.   *
-                                 * Films are bit-identical either way.                             */
+                                 * Films were bit-identical either way on every scene tested.     */
 } lucida_wavefront_config;
 
 /* wavefront::render_native_on (crates/gpu/src/wavefront.rs:134): renders
