@@ -173,10 +173,11 @@ def test_batch_size_does_not_change_the_image(gpu, oracle):
 
 @pytest.mark.parametrize("name,width", [("sphere-c2", 0), ("sphere-c2", 2), ("instances-c3", 0)])
 def test_ray_order_does_not_change_the_image(gpu, oracle, name, width):
-    """Sorting the extension queues only regroups rays into waves: every path
-    computes the same values, so films and ray counts are bit-identical (node
-    visits and triangle tests differ a little: a lane's speculative visits
-    past a postponed leaf depend on which rays share its wave)."""
+    """Sorting the extension queues only regroups rays into waves, so on these
+    scenes films and ray counts are bit-identical (node visits and triangle
+    tests differ a little: a lane's speculative visits past a postponed leaf
+    depend on which rays share its wave; for the same reason a ray meeting two
+    triangles at distances equal to within rounding could keep either)."""
     if name.startswith("instances"):
         blob = L.Blob(name)
         scene = L.Scene(gpu, blob)
