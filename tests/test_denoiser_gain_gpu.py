@@ -13,10 +13,11 @@ trained network must beat the a-trous filter alone (its fallback, and the
 estimate it corrects) at 4 and 16 samples per pixel on the sphere scene
 (about 0.86 and 0.67 of its relMSE) and at 1 and 4 on the instanced one
 (about 0.45 and 0.44). At 1 sample on the sphere this small training set
-leaves the two about even: over small changes of the training renders
-(the order rays are traced in can move a hit on a box boundary) the ratio
-measured 0.83 to 1.03 with the last step's weights and 0.99 with the
-average, so there the network must stay within 5% of the filter. From 16
+leaves the two about even: the ratio measured 0.83 to 1.03 with the last
+step's weights over runs and small changes of the training renders (one
+target film of 192 rendered differently moved it from below 1 to 1.03),
+and 0.99 with the average, so there the network must stay within 5% of the
+filter. From 16
 samples on, the instanced scene's relMSE is set by caustic fireflies of
 the 1e4 hot spots seen through glass, which the 4096-sample reference still
 holds: it then compares two noisy estimates of the reference's own noise,
