@@ -462,12 +462,13 @@ __device__ inline RayStack make_stack(uint32_t* lds_base, uint32_t* spill_base, 
 // then serve most of its lanes at once. The leaf step does not wait for the
 // last four searching lanes (the lanes holding leaves would otherwise keep
 // stepping speculatively through those few lanes' long searches: EXTEND
-// 24.8 -> 22.6 ms per 16-spp step at 1024^2, films unchanged). Leaves are
-// still intersected in the
-// walk's order, so the hit is the one the reference's walk returns; the
-// inner nodes visited past a postponed leaf are tested against the closest
-// hit before that leaf's triangles, so the walk may visit nodes the
-// reference's walk culls (never fewer). Occlusion rays mostly end at their
+// 24.8 -> 22.6 ms per 16-spp step at 1024^2, that film unchanged). Leaves
+// are still intersected in the walk's order, so the hit is the one the
+// reference's walk returns, up to rounding: the inner nodes visited past a
+// postponed leaf are tested against the closest hit before that leaf's
+// triangles, so the walk may visit nodes the reference's walk culls (never
+// fewer), and a triangle there whose hit ties the closest one to within
+// rounding may then be kept. Occlusion rays mostly end at their
 // first hit leaf, where one loop that takes either kind of visit per step
 // measured faster. A tree visits each node at most once per ray; the cap
 // only guarantees termination on a malformed (cyclic) node array.
